@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Direct-sum throughput of the `hip` engine on one MI355X.
+
+Workload (the direct-sum configuration the metric is quoted on): N = 262144
+bodies of a Plummer sphere (scale radius 1, total mass 1, seed 2), FP64 state,
+FP32 force tiles with compensated accumulation, driven by the Dormand-Prince
+5(4) solver (rkdp) with error_threshold 1e-9 and dt = 1e-3.  A step is one
+rkdp advise(); the adaptive rule may split it into 8 substeps (one level, so a
+step costs 7 or 63 force evaluations).  value = N^2 pair interactions per
+force evaluation x evaluations / wall time of the timed steps, inputs resident
+in HBM.
+
+Also reported on the same line:
+  roofline      the FP32 direct kernel's average duration (HIP events on the
+                engine's stream) against the FP32 vector peak, 20 flop/pair;
+  force_error   relative error of the FP32 forces vs the FP64 kernel at the
+                final state (RMS and max over all bodies);
+  cpu_baseline  the CPU restatement of the reference's `openmp` engine on the
+                host cores, over a bounded sample of targets.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "pair-interactions/s (direct); BH steps/s at N=4M, force err ≤1e-3 vs FP64"
+FLOP_PER_PAIR = 20.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(y, m, n, budget_s: float, threads: int) -> dict:
+    """Pair rate of the CPU restatement (OpenMP over targets) on a sample of targets."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    ntgt = 256
+    t = 0.0
+    while True:
+        idx = np.arange(ntgt, dtype=np.uint64)
+        t0 = time.perf_counter()
+        oracle_lib.accel_targets(y, m, idx, threads)
+        t = time.perf_counter() - t0
+        if t >= budget_s or ntgt >= n:
+            break
+        grow = max(2.0, min(16.0, budget_s / max(t, 1e-3)))
+        ntgt = min(n, int(ntgt * grow))
+    return {"value": ntgt * n / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+            "sample": f"{ntgt} targets x {n} sources of the same Plummer state, FP64, "
+                      f"{t:.1f} s on {threads} host threads"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 18)
+    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding is not part of it "
+              "(use --gpus 1)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import nbody_amd
+    from nbody_amd import native, nbody_create_engine, nbody_create_solver, nbody_data
+
+    if nbody_amd.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 2
+    hip = native.hip_lib()
+    n = a.n
+    log(f"generating Plummer N={n}")
+    data = nbody_data()
+    data.set_quiet(True)
+    data.generate("plummer", n, 2)
+    e = nbody_create_engine({"engine": "hip", "force_precision": a.precision})
+    if e is None or not e.init(data):
+        print(f"bench.py: engine init failed: {nbody_amd.log_last()}", file=sys.stderr)
+        return 1
+    s = nbody_create_solver({"solver": "rkdp", "error_threshold": 1e-9, "max_step": a.dt,
+                             "max_recursion": 1, "substep_subdivisions": 8})
+    s.set_engine(e)
+    ctx = e.hip_context()
+    native.check(hip.nbh_timing_enable(ctx, 1))
+
+    for i in range(a.warmup):
+        s.advise(a.dt)
+        native.check(hip.nbh_ctx_synchronize(ctx))
+        log(f"warmup step {i + 1}/{a.warmup} done")
+    native.check(hip.nbh_timing_reset(ctx))
+    cc0 = e.get_compute_count()
+    sub0 = s.rk_substeps()
+    native.check(hip.nbh_ctx_synchronize(ctx))
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        s.advise(a.dt)
+        if (i + 1) % max(1, a.steps // 5) == 0:
+            log(f"step {i + 1}/{a.steps}")
+    native.check(hip.nbh_ctx_synchronize(ctx))
+    wall = time.perf_counter() - t0
+    if e.has_error():
+        print(f"bench.py: HIP error during the run: {nbody_amd.log_last()}", file=sys.stderr)
+        return 1
+    fcomputes = e.get_compute_count() - cc0
+    substeps = s.rk_substeps() - sub0
+    pairs = float(fcomputes) * n * n
+    value = pairs / wall
+
+    kname = b"direct_fp32" if a.precision == "fp32" else b"direct_fp64"
+    tot = C.c_double(0)
+    cnt = C.c_uint64(0)
+    native.check(hip.nbh_timing_read(ctx, kname, C.byref(tot), C.byref(cnt)))
+    avg_ms = tot.value / max(1, cnt.value)
+    achieved = FLOP_PER_PAIR * n * n / (avg_ms * 1e-3) / 1e12
+    peak = PEAK_FP32_TFLOPS if a.precision == "fp32" else 78.6
+    native.check(hip.nbh_timing_enable(ctx, 0))
+
+    # force error of the final state: FP32 tiles vs the FP64 kernel
+    log("force error check")
+    y = e.get_y()
+    f32m = e.create_buffer(48 * n)
+    f64m = e.create_buffer(48 * n)
+    yp = y.device_ptr()
+    yh = e.read_buffer(y)
+    _, _, mass, _ = data.bodies()
+    dm = e.create_buffer(8 * n)
+    e.write_buffer(dm, mass)
+    massp = dm.device_ptr()
+    native.check(hip.nbh_fcompute_direct(ctx, yp, massp, f32m.device_ptr(), n, 1, 0))
+    native.check(hip.nbh_fcompute_direct(ctx, yp, massp, f64m.device_ptr(), n, 0, 256))
+    f32 = e.read_buffer(f32m)[3 * n:].reshape(3, n)
+    f64 = e.read_buffer(f64m)[3 * n:].reshape(3, n)
+    rel = np.linalg.norm(f32 - f64, axis=0) / np.linalg.norm(f64, axis=0)
+    e.free_buffers([f32m, f64m, dm])
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.precision,
+        "data": "synthetic",
+        "config": {
+            "workload": f"direct all-pairs, Plummer N={n} (a=1, M=1, seed 2), rkdp error_threshold 1e-9, "
+                        f"dt={a.dt:g}, max_recursion 1 x 8 substeps; {a.precision.upper()} force tiles, FP64 state",
+            "n": n,
+            "fcompute_per_step": fcomputes / a.steps,
+            "accepted_substeps": substeps,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": kname.decode(),
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "avg_kernel_ms": avg_ms,
+            "launches": cnt.value,
+            "pairs_per_s_kernel": n * n / (avg_ms * 1e-3),
+        },
+        "force_error": {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "max_rel": float(rel.max()),
+                        "vs": "FP64 direct kernel (bit-exact with the CPU oracle)"},
+    }
+    if not a.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline ({threads} threads, ~{a.cpu_seconds:g} s)")
+        out["cpu_baseline"] = cpu_baseline(yh, mass, n, a.cpu_seconds, threads)
+    print(json.dumps(out), flush=True)
+    s.destroy()
+    e.destroy()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

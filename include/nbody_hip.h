@@ -1,0 +1,159 @@
+/*
+ * nbody_hip.h -- C ABI of the MI355X (gfx950) N-body kernels (libnbody_hip.so).
+ *
+ * Plain pointers and sizes only.  Device pointers come from nbh_malloc.  Every
+ * call returns an nbh_status; on failure nbh_last_error() holds a message for
+ * the calling thread.  All work of one context is ordered on that context's
+ * single HIP stream; calls that hand a result to host memory (nbh_memcpy_d2h,
+ * nbh_maxabs, nbh_stats, nbh_bh_get_tree) return only after it is complete.
+ *
+ * Each entry point names the reference interface it stands in for (paths are
+ * relative to the drons/nbody repository root).  The reference's de-facto C
+ * layer is the set of host launchers in nbody/nbody_engine_cuda_impl.h:19-115.
+ */
+#ifndef NBODY_HIP_H
+#define NBODY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nbh_status {
+	NBH_OK = 0,
+	NBH_ERR_INVALID = 1,      /* bad argument: NULL pointer, size, count, ... */
+	NBH_ERR_HIP = 2,          /* a HIP runtime call failed */
+	NBH_ERR_NOMEM = 3,        /* device allocation failed */
+	NBH_ERR_UNSUPPORTED = 4   /* valid request this build does not support */
+} nbh_status;
+
+/* Arithmetic of the force kernels. */
+typedef enum nbh_precision {
+	NBH_FP64 = 0,   /* IEEE binary64, same operation order as the reference CPU engines */
+	NBH_FP32 = 1    /* FP32 interaction math on relative coordinates, compensated sums,
+	                   FP64 state in and out */
+} nbh_precision;
+
+/* Tree layouts of the Barnes-Hut engine (nbody/nbody_engine_simple_bh.h:13-20). */
+typedef enum nbh_tree_layout {
+	NBH_LAYOUT_HEAP = 1,            /* stack traversal, left subtree first */
+	NBH_LAYOUT_HEAP_STACKLESS = 2   /* skip/next_up preorder */
+} nbh_tree_layout;
+
+typedef struct nbh_ctx nbh_ctx;     /* one device + one stream */
+typedef struct nbh_tree nbh_tree;   /* device-resident heap kd-tree */
+
+/* ---- context, errors ---------------------------------------------------- */
+const char* nbh_last_error(void);
+int nbh_version(void);
+/* Number of visible HIP devices (0 when there is no GPU). */
+nbh_status nbh_device_count(int* count);
+/* Replaces nbody_engine_cuda::select_devices (nbody/nbody_engine_cuda.cpp:56-89)
+   for a single device id. */
+nbh_status nbh_ctx_create(int device, nbh_ctx** ctx);
+nbh_status nbh_ctx_destroy(nbh_ctx* ctx);
+nbh_status nbh_ctx_synchronize(nbh_ctx* ctx);
+/* Device name and compute-unit count, for reports. */
+nbh_status nbh_ctx_info(nbh_ctx* ctx, char* name, size_t name_len, int* cu_count);
+
+/* ---- memory: nbody_engine::create_buffer/free_buffer/read_buffer/write_buffer/
+        copy_buffer (nbody/nbody_engine.h:54-63; CUDA nbody_engine_cuda.cpp:267-355) ---- */
+nbh_status nbh_malloc(nbh_ctx* ctx, size_t bytes, void** dptr);
+nbh_status nbh_free(nbh_ctx* ctx, void* dptr);
+nbh_status nbh_memcpy_h2d(nbh_ctx* ctx, void* dst, const void* src, size_t bytes);
+nbh_status nbh_memcpy_d2h(nbh_ctx* ctx, void* dst, const void* src, size_t bytes);
+nbh_status nbh_memcpy_d2d(nbh_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- FP64 vector ops over n doubles ------------------------------------- */
+/* fill_buffer (nbody_engine_cuda_impl.cu:716-721) */
+nbh_status nbh_fill(nbh_ctx* ctx, double* a, double value, size_t n);
+/* Fused Butcher-table combination, one pass over memory:
+     acc = (init ? init[i] : 0);  for t in 0..k-1: acc = fma(src[t][i], coef[t], acc);  a[i] = acc
+   Terms are applied in the order given; the caller drops zero coefficients.
+   init may equal a; src[t] may equal a.  k may be 0 (then a = init or 0).
+   Replaces the fmadd / fmadd_inplace chains of nbody_engine::fmaddn and
+   fmaddn_inplace (nbody/nbody_engine.cpp:47-113; kernels kfmadd,
+   kfmadd_inplace, nbody_engine_cuda_impl.cu:750-805). */
+nbh_status nbh_fmaddn(nbh_ctx* ctx, double* a, const double* init,
+                      const double* const* src, const double* coef, size_t k, size_t n);
+/* Kahan-compensated a[i] += sum_t src[t][i]*coef[t] with per-element
+   correction corr[i], terms in order (summation_k, nbody/summation.h:7-14).
+   Replaces the per-term kfmadd_inplace_corr launches
+   (nbody/nbody_engine_cuda.cpp:402-459, nbody_engine_cuda_impl.cu:766-787). */
+nbh_status nbh_fmaddn_corr(nbh_ctx* ctx, double* a, double* corr,
+                           const double* const* src, const double* coef, size_t k, size_t n);
+/* max |a[i]| returned to host (fmaxabs, nbody_engine_cuda_impl.cu:808-817). n == 0 gives 0. */
+nbh_status nbh_maxabs(nbh_ctx* ctx, const double* a, size_t n, double* result);
+/* Periodic wrap of the first 3*nbodies entries of y by +-2b
+   (kclamp_coord, nbody_engine_cuda_impl.cu:724-747). */
+nbh_status nbh_clamp(nbh_ctx* ctx, double* y, size_t nbodies, double b);
+
+/* ---- direct all-pairs force ---------------------------------------------
+   y: 6n doubles [rx ry rz vx vy vz]; mass: n doubles; f: 6n doubles, written
+   as [vx vy vz ax ay az].  Replaces fcompute_block + fcompute_xyz
+   (nbody_engine_cuda_impl.cu:10-124; host nbody/nbody_engine_cuda.cpp:202-247).
+   NBH_FP64 reproduces nbody_engine_simple::fcompute (nbody_engine_simple.cpp:116-173)
+   operation for operation.  block_size: work-group size of the FP64 kernel
+   (64..1024, multiple of 64), 0 = default. */
+nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const double* mass, double* f,
+                               size_t n, nbh_precision precision, int block_size);
+
+/* ---- Barnes-Hut heap kd-tree --------------------------------------------
+   n must be a power of two (the heap layout of nbody/nbody_space_heap.cpp:23
+   puts the leaves at [n, 2n)).  The tree is built on the GPU with the same
+   median splits as nbody_space_heap::build (nbody_space_heap.cpp:13-37, 136-191). */
+nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** tree);
+nbh_status nbh_tree_destroy(nbh_tree* tree);
+/* Full build: body order per node, moments, boxes, crit_r2 = (ratio*r)^2. */
+nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* tree, const double* y, const double* mass,
+                          double ratio);
+/* Refit with the existing body order (nbody_space_heap::rebuild, :39-62;
+   kupdate_leaf_bh / kupdate_node_bh, nbody_engine_cuda_impl.cu:477-641). */
+nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* tree, const double* y, double ratio);
+/* Traversal: f[6n] = [v ; F/m] for every body, targets taken in heap leaf
+   order.  FP64 reproduces nbody_space_heap::traverse (layout HEAP) and
+   nbody_space_heap_stackless::traverse (layout HEAP_STACKLESS) with the
+   nested_tree target order of nbody_engine_simple_bh.cpp:78-92.
+   Replaces kfcompute_heap_bh / kfcompute_heap_bh_stackless (+ _tex),
+   nbody_engine_cuda_impl.cu:130-451. */
+nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* tree, const double* y,
+                           const double* mass, double* f, nbh_tree_layout layout,
+                           nbh_precision precision);
+/* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may
+   be NULL.  cm/bmin/bmax are xyz-interleaved [2n*3]; body_n holds -1 on
+   internal nodes.  Used to check the layout against the reference heap
+   (nbody_space_heap::get_body_n / get_mass_center / get_radius_sqr). */
+nbh_status nbh_tree_get(nbh_ctx* ctx, const nbh_tree* tree, int32_t* body_n, double* cm,
+                        double* mass, double* crit_r2, double* bmin, double* bmax);
+/* Number of accepted node interactions and of node visits of the last
+   nbh_bh_fcompute (summed over targets); either pointer may be NULL. */
+nbh_status nbh_bh_counters(nbh_ctx* ctx, const nbh_tree* tree, uint64_t* interactions,
+                           uint64_t* visits);
+
+/* ---- kernel timers -------------------------------------------------------
+   When enabled, the main kernel of each entry point is bracketed by HIP events
+   on the context's stream.  nbh_timing_read synchronises the stream and returns
+   the summed device time (ms) and launch count of one kernel class since the
+   last reset.  Classes: "direct_fp64", "direct_fp32", "tree_build",
+   "tree_refit", "bh_traverse", "fmaddn", "stats".  No counterpart in the
+   reference (its timing is host wall clock, test/bench/bench.h:105-115). */
+nbh_status nbh_timing_enable(nbh_ctx* ctx, int enable);
+nbh_status nbh_timing_reset(nbh_ctx* ctx);
+nbh_status nbh_timing_read(nbh_ctx* ctx, const char* kernel_class, double* total_ms, uint64_t* launches);
+
+/* ---- conservation checks ------------------------------------------------
+   FP64 compensated reductions of the P/L/E/V quantities of
+   nbody_data::print_statistics (nbody/nbody_data.cpp:57-102, proxies
+   nbody/summation_proxy.h:6-87) over the device state y[6n], mass[n].
+   out[12]: P(3) L(3) KE PE CM(3, divided by total mass) total_mass.
+   with_energy = 0 skips the O(n^2) potential energy (out[7] = 0). */
+nbh_status nbh_stats(nbh_ctx* ctx, const double* y, const double* mass, size_t n,
+                     int with_energy, double out[12]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_HIP_H */

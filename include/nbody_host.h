@@ -1,0 +1,148 @@
+/*
+ * nbody_host.h -- C ABI of the host layer (libnbody.so): bodies, the engine
+ * interface, the solvers and the synthetic workloads, for hosts that cannot
+ * link C++ (Python ctypes in this repository's tests and bench).
+ *
+ * Handles are opaque pointers to the C++ objects of nbody_amd/host:
+ *   nbs_data   -> nbody_data    (nbody/nbody_data.h:10-93)
+ *   nbs_engine -> nbody_engine  (nbody/nbody_engine.h:16-96)
+ *   nbs_memory -> nbody_engine::memory
+ *   nbs_solver -> nbody_solver  (nbody/nbody_solver.h:12-41)
+ * Engine and solver calls keep the reference's error behaviour: misuse is
+ * logged (see nbs_log_count) and the call returns without effect.
+ */
+#ifndef NBODY_HOST_H
+#define NBODY_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nbs_data nbs_data;
+typedef struct nbs_engine nbs_engine;
+typedef struct nbs_memory nbs_memory;
+typedef struct nbs_solver nbs_solver;
+
+/* ---- diagnostics ---- */
+size_t      nbs_log_count(void);
+const char* nbs_log_last(void);
+void        nbs_log_quiet(int quiet);
+
+/* ---- bodies (nbody_data) ---- */
+nbs_data* nbs_data_create(void);
+void      nbs_data_destroy(nbs_data* d);
+/* type: "G1", "SI", "ADK" or "Zeno" (nbody_data::load_initial); returns 1 on success */
+int       nbs_data_load(nbs_data* d, const char* path, const char* type);
+int       nbs_data_save(const nbs_data* d, const char* path);
+void      nbs_data_make_universe(nbs_data* d, size_t star_count, double sx, double sy, double sz);
+/* kind: "ball" | "plummer" | "two_plummer" | "cube"; returns 1 on success */
+int       nbs_data_generate(nbs_data* d, const char* kind, size_t n, uint64_t seed);
+/* Replace the bodies: pos/vel are n x 3 (AoS), radius may be NULL */
+void      nbs_data_set(nbs_data* d, size_t n, const double* pos, const double* vel,
+                       const double* mass, const double* radius);
+size_t    nbs_data_count(const nbs_data* d);
+void      nbs_data_get(const nbs_data* d, double* pos, double* vel, double* mass, double* radius);
+double    nbs_data_time(const nbs_data* d);
+size_t    nbs_data_step(const nbs_data* d);
+void      nbs_data_set_time(nbs_data* d, double t);
+void      nbs_data_set_step(nbs_data* d, size_t s);
+void      nbs_data_advise_time(nbs_data* d, double dt);
+size_t    nbs_data_box_size(const nbs_data* d);
+void      nbs_data_set_check_list(nbs_data* d, const char* list);
+void      nbs_data_set_quiet(nbs_data* d, int quiet);
+void      nbs_data_print_statistics(nbs_data* d, nbs_engine* e);
+/* out[12]: P(3) L(3) KE PE CM(3) M; e may be NULL (host sums over d) */
+void      nbs_data_statistics(nbs_data* d, nbs_engine* e, double out[12]);
+/* out[4]: dP%, dL%, dE%, |Vcm| from the last print_statistics */
+void      nbs_data_errors(const nbs_data* d, double out[4]);
+int       nbs_data_is_equal(const nbs_data* a, const nbs_data* b, double eps);
+/* n bodies at the origin with zero mass (nbody_data::resize) */
+void      nbs_data_resize(nbs_data* d, size_t n);
+
+/* ---- engines (nbody_engine) ---- */
+/* params: "key=value;key=value" with the keys of nbody_create_engine */
+nbs_engine* nbs_engine_create(const char* params);
+void        nbs_engine_destroy(nbs_engine* e);
+const char* nbs_engine_type_name(const nbs_engine* e);
+int         nbs_engine_init(nbs_engine* e, nbs_data* d);
+void        nbs_engine_get_data(nbs_engine* e, nbs_data* d);
+size_t      nbs_engine_problem_size(const nbs_engine* e);
+nbs_memory* nbs_engine_get_y(nbs_engine* e);
+void        nbs_engine_advise_time(nbs_engine* e, double dt);
+double      nbs_engine_get_time(const nbs_engine* e);
+void        nbs_engine_set_time(nbs_engine* e, double t);
+size_t      nbs_engine_get_step(const nbs_engine* e);
+void        nbs_engine_set_step(nbs_engine* e, size_t s);
+void        nbs_engine_fcompute(nbs_engine* e, double t, const nbs_memory* y, nbs_memory* f);
+void        nbs_engine_clamp(nbs_engine* e, nbs_memory* y, double b);
+nbs_memory* nbs_engine_create_buffer(nbs_engine* e, size_t bytes);
+void        nbs_engine_free_buffer(nbs_engine* e, nbs_memory* m);
+void        nbs_engine_read_buffer(nbs_engine* e, void* dst, const nbs_memory* src);
+void        nbs_engine_write_buffer(nbs_engine* e, nbs_memory* dst, const void* src);
+void        nbs_engine_copy_buffer(nbs_engine* e, nbs_memory* a, const nbs_memory* b);
+void        nbs_engine_fill_buffer(nbs_engine* e, nbs_memory* a, double value);
+void        nbs_engine_fmadd_inplace(nbs_engine* e, nbs_memory* a, const nbs_memory* b, double c);
+void        nbs_engine_fmadd(nbs_engine* e, nbs_memory* a, const nbs_memory* b, const nbs_memory* c, double d);
+/* b has nb entries (entries may be NULL); c may be NULL */
+void        nbs_engine_fmaddn_inplace(nbs_engine* e, nbs_memory* a, nbs_memory* const* b, size_t nb,
+                                      const double* c, size_t csize);
+void        nbs_engine_fmaddn_corr(nbs_engine* e, nbs_memory* a, nbs_memory* corr, nbs_memory* const* b,
+                                   size_t nb, const double* c, size_t csize);
+void        nbs_engine_fmaddn(nbs_engine* e, nbs_memory* a, const nbs_memory* b, nbs_memory* const* c,
+                              size_t nc, const double* d, size_t dsize);
+void        nbs_engine_fmaxabs(nbs_engine* e, const nbs_memory* a, double* result);
+size_t      nbs_engine_compute_count(const nbs_engine* e);
+void        nbs_engine_print_info(const nbs_engine* e);
+/* 1 once a HIP call of this engine failed */
+int         nbs_engine_has_error(const nbs_engine* e);
+/* Device context / tree of a hip engine (NULL otherwise), for direct C-ABI use */
+void*       nbs_engine_hip_context(nbs_engine* e);
+const void* nbs_engine_hip_tree(nbs_engine* e);
+/* Device pointer of a hip buffer (NULL otherwise) */
+void*       nbs_memory_device_ptr(nbs_memory* m);
+size_t      nbs_memory_size(const nbs_memory* m);
+/* A memory object of a foreign type: every engine must reject it. */
+nbs_memory* nbs_memory_fake_create(size_t bytes);
+void        nbs_memory_fake_destroy(nbs_memory* m);
+
+/* ---- solvers (nbody_solver) ---- */
+/* params: "key=value;..." with the keys of nbody_create_solver */
+nbs_solver* nbs_solver_create(const char* params);
+void        nbs_solver_destroy(nbs_solver* s);
+const char* nbs_solver_type_name(const nbs_solver* s);
+void        nbs_solver_set_engine(nbs_solver* s, nbs_engine* e);
+void        nbs_solver_set_time_step(nbs_solver* s, double min_step, double max_step);
+int         nbs_solver_run(nbs_solver* s, nbs_data* d, double max_time, double dump_dt, double check_dt);
+void        nbs_solver_advise(nbs_solver* s, double dt);
+void        nbs_solver_print_info(const nbs_solver* s);
+/* Runge-Kutta (Butcher table) solvers only; return 0 for other solvers */
+int         nbs_solver_rk_configure(nbs_solver* s, size_t max_recursion, size_t substep_subdivisions,
+                                    size_t refine_steps_count, double error_threshold);
+int         nbs_solver_rk_table(const nbs_solver* s, size_t* steps, int* implicit, int* embedded);
+/* a: steps*steps row-major (zero where the table has no entry) */
+int         nbs_solver_rk_coeffs(const nbs_solver* s, double* a, double* b1, double* b2, double* c);
+size_t      nbs_solver_rk_substeps(const nbs_solver* s);
+/* Generic Runge-Kutta solver on a named table ("rk4", "rkdp", ...), reference
+   defaults (nbody_solver_rk_butcher constructor); NULL for an unknown table */
+nbs_solver* nbs_solver_create_rk_butcher(const char* table);
+/* Adams-Bashforth of the given rank with an euler starter */
+nbs_solver* nbs_solver_create_adams(size_t rank, int correction);
+
+/* ---- extrapolators (nbody/nbody_extrapolator.h) ---- */
+typedef struct nbs_extrapolator nbs_extrapolator;
+/* type: "neville" | "berrut"; NULL for anything else */
+nbs_extrapolator* nbs_extrapolator_create(const char* type, nbs_engine* e, size_t order,
+                                          const size_t* substeps_count, size_t levels);
+void      nbs_extrapolator_destroy(nbs_extrapolator* x);
+void      nbs_extrapolator_update_table(nbs_extrapolator* x, size_t level, nbs_memory* y);
+double    nbs_extrapolator_estimate_error(const nbs_extrapolator* x, size_t level);
+void      nbs_extrapolator_extrapolate(const nbs_extrapolator* x, size_t level, nbs_memory* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_HOST_H */

@@ -1,0 +1,278 @@
+// nbh_direct.hip -- direct all-pairs accelerations on gfx950.
+//
+// FP64 kernel: one target per work-item, sources staged in LDS tiles, the
+// source loop runs in index order and each pair is evaluated with exactly the
+// operations of nbody_data::force followed by the division by m1 of
+// nbody_engine_simple::fcompute (nbody/nbody_data.cpp:35-44,
+// nbody/nbody_engine_simple.cpp:152-171).  This file is compiled with
+// -ffp-contract=off, so the FP64 result equals the CPU restatement bit for bit
+// (IEEE sqrt and division are correctly rounded on both sides).
+//
+// FP32 kernel (the throughput path): each work-group owns 256*TGT targets and a
+// slice of the sources.  Sources are converted once per work-group from FP64 to
+// FP32 coordinates relative to the work-group's origin (the FP64 position of
+// its first target), staged in LDS as float4 (x, y, z, m) and read back as
+// wave-wide broadcasts; each work-item reuses every source for TGT targets
+// held in registers.  Per pair: 3 sub, 3 mul/fma for r^2, 1 max (the
+// reference's r^2 >= 1e-8 clamp), 1 v_rsq_f32, 3 mul, 3 fma.  Partial sums of
+// one LDS tile stay in FP32 and are folded into a Kahan (summation_k) FP32
+// accumulator per tile; the source slices are then added in FP64 in a fixed
+// order by a second kernel, which also copies the velocities into f[0:3n].
+
+#include <cmath>
+
+#include "nbh_common.h"
+
+namespace {
+
+constexpr double kMinDistance = 1e-8;   // nbody/nbtype.h:79
+
+// ---------------------------------------------------------------------------
+// FP64, reference operation order
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024)
+k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass,
+              double* __restrict__ f, size_t n)
+{
+	extern __shared__ double sh[];
+	const int bs = blockDim.x;
+	double* sx = sh;
+	double* sy = sh + bs;
+	double* sz = sh + 2 * bs;
+	double* sm = sh + 3 * bs;
+
+	const size_t i = static_cast<size_t>(blockIdx.x) * bs + threadIdx.x;
+	const bool active = i < n;
+	const double x1 = active ? y[i] : 0.0;
+	const double y1 = active ? y[n + i] : 0.0;
+	const double z1 = active ? y[2 * n + i] : 0.0;
+	const double m1 = active ? mass[i] : 1.0;
+	const double gm1 = -1.0 * m1;            // (-GravityConst * mass1), G = 1
+	double tx = 0.0, ty = 0.0, tz = 0.0;
+
+	for(size_t base = 0; base < n; base += bs)
+	{
+		const size_t j = base + threadIdx.x;
+		__syncthreads();
+		sx[threadIdx.x] = j < n ? y[j] : 0.0;
+		sy[threadIdx.x] = j < n ? y[n + j] : 0.0;
+		sz[threadIdx.x] = j < n ? y[2 * n + j] : 0.0;
+		sm[threadIdx.x] = j < n ? mass[j] : 0.0;
+		__syncthreads();
+		const int cnt = n - base < static_cast<size_t>(bs) ? static_cast<int>(n - base) : bs;
+		const int self = (active && i >= base && i < base + cnt) ? static_cast<int>(i - base) : -1;
+		for(int jj = 0; jj < cnt; ++jj)
+		{
+			if(jj == self)
+			{
+				continue;
+			}
+			const double dx = x1 - sx[jj];
+			const double dy = y1 - sy[jj];
+			const double dz = z1 - sz[jj];
+			double r2 = dx * dx + dy * dy + dz * dz;
+			if(r2 < kMinDistance)
+			{
+				r2 = kMinDistance;
+			}
+			const double coeff = (gm1 * sm[jj]) / (r2 * sqrt(r2));
+			tx += dx * coeff;
+			ty += dy * coeff;
+			tz += dz * coeff;
+		}
+	}
+	if(active)
+	{
+		f[i] = y[3 * n + i];
+		f[n + i] = y[4 * n + i];
+		f[2 * n + i] = y[5 * n + i];
+		f[3 * n + i] = tx / m1;
+		f[4 * n + i] = ty / m1;
+		f[5 * n + i] = tz / m1;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// FP32 tiles
+// ---------------------------------------------------------------------------
+constexpr int kF32Threads = 256;   // work-items per work-group (4 waves)
+constexpr int kF32Tgt = 4;         // targets per work-item
+constexpr int kF32Tile = 256;      // sources per LDS tile (one per work-item)
+constexpr int kF32TgtPerGroup = kF32Threads * kF32Tgt;
+
+__device__ inline void kahan_f32(float& s, float& c, float v)
+{
+	float corrected = v - c;
+	float ns = s + corrected;
+	c = (ns - s) - corrected;
+	s = ns;
+}
+
+// part: [nsplit][3][n] doubles, slice blockIdx.y of the sources.
+__global__ void __launch_bounds__(kF32Threads)
+k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass,
+              double* __restrict__ part, size_t n, size_t jchunk)
+{
+	__shared__ float4 tile[kF32Tile];
+
+	const size_t tbase = static_cast<size_t>(blockIdx.x) * kF32TgtPerGroup;
+	const double ox = y[tbase];
+	const double oy = y[n + tbase];
+	const double oz = y[2 * n + tbase];
+
+	float xi[kF32Tgt], yi[kF32Tgt], zi[kF32Tgt];
+	float sx[kF32Tgt], sy[kF32Tgt], sz[kF32Tgt];
+	float cx[kF32Tgt], cy[kF32Tgt], cz[kF32Tgt];
+#pragma unroll
+	for(int k = 0; k < kF32Tgt; ++k)
+	{
+		size_t i = tbase + static_cast<size_t>(k) * kF32Threads + threadIdx.x;
+		bool ok = i < n;
+		xi[k] = ok ? static_cast<float>(y[i] - ox) : 0.0f;
+		yi[k] = ok ? static_cast<float>(y[n + i] - oy) : 0.0f;
+		zi[k] = ok ? static_cast<float>(y[2 * n + i] - oz) : 0.0f;
+		sx[k] = sy[k] = sz[k] = 0.0f;
+		cx[k] = cy[k] = cz[k] = 0.0f;
+	}
+
+	const size_t jbeg = static_cast<size_t>(blockIdx.y) * jchunk;
+	const size_t jend = jbeg + jchunk < n ? jbeg + jchunk : n;
+	const float rmin2 = static_cast<float>(kMinDistance);
+
+	for(size_t base = jbeg; base < jend; base += kF32Tile)
+	{
+		const size_t j = base + threadIdx.x;
+		__syncthreads();
+		if(j < jend)
+		{
+			tile[threadIdx.x] = make_float4(static_cast<float>(y[j] - ox),
+			                                static_cast<float>(y[n + j] - oy),
+			                                static_cast<float>(y[2 * n + j] - oz),
+			                                static_cast<float>(mass[j]));
+		}
+		else
+		{
+			tile[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // zero mass: no force
+		}
+		__syncthreads();
+
+		float px[kF32Tgt], py[kF32Tgt], pz[kF32Tgt];
+#pragma unroll
+		for(int k = 0; k < kF32Tgt; ++k) { px[k] = py[k] = pz[k] = 0.0f; }
+
+#pragma unroll 4
+		for(int jj = 0; jj < kF32Tile; ++jj)
+		{
+			const float4 s = tile[jj];
+#pragma unroll
+			for(int k = 0; k < kF32Tgt; ++k)
+			{
+				const float dx = s.x - xi[k];
+				const float dy = s.y - yi[k];
+				const float dz = s.z - zi[k];
+				float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+				r2 = fmaxf(r2, rmin2);
+				const float ri = __builtin_amdgcn_rsqf(r2);
+				const float w = (s.w * ri) * (ri * ri);
+				px[k] = fmaf(dx, w, px[k]);
+				py[k] = fmaf(dy, w, py[k]);
+				pz[k] = fmaf(dz, w, pz[k]);
+			}
+		}
+#pragma unroll
+		for(int k = 0; k < kF32Tgt; ++k)
+		{
+			kahan_f32(sx[k], cx[k], px[k]);
+			kahan_f32(sy[k], cy[k], py[k]);
+			kahan_f32(sz[k], cz[k], pz[k]);
+		}
+	}
+
+	double* out = part + static_cast<size_t>(blockIdx.y) * 3 * n;
+#pragma unroll
+	for(int k = 0; k < kF32Tgt; ++k)
+	{
+		size_t i = tbase + static_cast<size_t>(k) * kF32Threads + threadIdx.x;
+		if(i < n)
+		{
+			out[i] = static_cast<double>(sx[k]) - static_cast<double>(cx[k]);
+			out[n + i] = static_cast<double>(sy[k]) - static_cast<double>(cy[k]);
+			out[2 * n + i] = static_cast<double>(sz[k]) - static_cast<double>(cz[k]);
+		}
+	}
+}
+
+// f[0:3n] = v, f[3n:6n] = sum over slices (fixed order).
+__global__ void __launch_bounds__(256)
+k_direct_fp32_finish(const double* __restrict__ y, const double* __restrict__ part,
+                     double* __restrict__ f, size_t n, int nsplit)
+{
+	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		double ax = 0.0, ay = 0.0, az = 0.0;
+		for(int s = 0; s < nsplit; ++s)
+		{
+			const double* p = part + static_cast<size_t>(s) * 3 * n;
+			ax += p[i];
+			ay += p[n + i];
+			az += p[2 * n + i];
+		}
+		f[i] = y[3 * n + i];
+		f[n + i] = y[4 * n + i];
+		f[2 * n + i] = y[5 * n + i];
+		f[3 * n + i] = ax;
+		f[4 * n + i] = ay;
+		f[5 * n + i] = az;
+	}
+}
+
+}  // namespace
+
+extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const double* mass,
+                                          double* f, size_t n, nbh_precision precision,
+                                          int block_size)
+{
+	NBH_TRY(nbh::enter(ctx));
+	if(n == 0) { return NBH_OK; }
+	NBH_CHECK(y != nullptr && mass != nullptr && f != nullptr, "null buffer");
+	NBH_CHECK(y != f, "y and f must not alias");
+	NBH_CHECK(n < (static_cast<size_t>(1) << 31), "n = %zu too large", n);
+	if(precision == NBH_FP64)
+	{
+		int bs = block_size == 0 ? 256 : block_size;
+		NBH_CHECK(bs >= 64 && bs <= 1024 && bs % 64 == 0,
+		          "block_size %d: must be a multiple of 64 in [64, 1024]", block_size);
+		unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
+		nbh::scoped_timer timer(ctx, "direct_fp64");
+		hipLaunchKernelGGL(k_direct_fp64, dim3(grid), dim3(bs), 4 * bs * sizeof(double),
+		                   ctx->stream, y, mass, f, n);
+		NBH_LAUNCHED();
+		return NBH_OK;
+	}
+	NBH_CHECK(precision == NBH_FP32, "unknown precision %d", static_cast<int>(precision));
+	// Split the sources so that the grid holds ~8 work-groups per CU.
+	size_t gx = (n + kF32TgtPerGroup - 1) / kF32TgtPerGroup;
+	size_t want = static_cast<size_t>(ctx->cu_count > 0 ? ctx->cu_count : 256) * 8;
+	size_t nsplit = (want + gx - 1) / gx;
+	size_t max_split = (n + kF32Tile - 1) / kF32Tile;
+	if(nsplit > max_split) { nsplit = max_split; }
+	if(nsplit > 64) { nsplit = 64; }
+	if(nsplit < 1) { nsplit = 1; }
+	size_t jchunk = (n + nsplit - 1) / nsplit;
+	jchunk = (jchunk + kF32Tile - 1) / kF32Tile * kF32Tile;
+	nsplit = (n + jchunk - 1) / jchunk;
+	void* sp = nullptr;
+	NBH_TRY(nbh::scratch(ctx, nsplit * 3 * n * sizeof(double), &sp));
+	double* part = static_cast<double*>(sp);
+	{
+		nbh::scoped_timer timer(ctx, "direct_fp32");
+		hipLaunchKernelGGL(k_direct_fp32, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(nsplit)),
+		                   dim3(kF32Threads), 0, ctx->stream, y, mass, part, n, jchunk);
+		NBH_LAUNCHED();
+	}
+	hipLaunchKernelGGL(k_direct_fp32_finish, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0,
+	                   ctx->stream, y, part, f, n, static_cast<int>(nsplit));
+	NBH_LAUNCHED();
+	return NBH_OK;
+}

@@ -1,0 +1,723 @@
+// nbody_engine_hip.cpp -- see nbody_engine_hip.h.
+#include "nbody_engine_hip.h"
+
+#include <cstring>
+#include <vector>
+
+#include "nbody_log.h"
+
+// ---------------------------------------------------------------------------
+// buffers
+// ---------------------------------------------------------------------------
+nbody_engine_hip::smemory::smemory(nbh_ctx* ctx, size_t size) : m_ctx(ctx), m_data(nullptr), m_size(size)
+{
+	if(size != 0 && nbh_malloc(ctx, size, &m_data) != NBH_OK)
+	{
+		nbody_log("hip: allocation of %zu bytes failed: %s", size, nbh_last_error());
+		m_data = nullptr;
+	}
+}
+
+nbody_engine_hip::smemory::~smemory()
+{
+	if(m_data != nullptr && nbh_free(m_ctx, m_data) != NBH_OK)
+	{
+		nbody_log("hip: free failed: %s", nbh_last_error());
+	}
+}
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+nbody_engine_hip::nbody_engine_hip() :
+	m_ctx(nullptr),
+	m_device(-1),
+	m_block_size(NBODY_DATA_BLOCK_SIZE),
+	m_precision(NBH_FP64),
+	m_data(nullptr),
+	m_mass(nullptr),
+	m_y(nullptr),
+	m_error(false)
+{
+}
+
+nbody_engine_hip::~nbody_engine_hip()
+{
+	delete m_mass;
+	delete m_y;
+	if(m_ctx != nullptr)
+	{
+		nbh_ctx_destroy(m_ctx);
+	}
+}
+
+bool nbody_engine_hip::check(nbh_status s, const char* what)
+{
+	if(s == NBH_OK)
+	{
+		return true;
+	}
+	if(s != NBH_ERR_INVALID)
+	{
+		m_error = true;
+	}
+	nbody_log("hip: %s: %s", what, nbh_last_error());
+	return false;
+}
+
+int nbody_engine_hip::select_device(int device)
+{
+	if(m_ctx != nullptr)
+	{
+		nbh_ctx_destroy(m_ctx);
+		m_ctx = nullptr;
+	}
+	nbh_status s = nbh_ctx_create(device, &m_ctx);
+	if(s != NBH_OK)
+	{
+		nbody_log("hip: can't use device %d: %s", device, nbh_last_error());
+		m_ctx = nullptr;
+		return -1;
+	}
+	m_device = device;
+	return 0;
+}
+
+void nbody_engine_hip::set_block_size(int block_size)
+{
+	m_block_size = block_size;
+}
+
+void nbody_engine_hip::set_precision(nbh_precision p)
+{
+	m_precision = p;
+}
+
+const char* nbody_engine_hip::type_name() const
+{
+	return "nbody_engine_hip";
+}
+
+// nbody/nbody_engine_cuda.cpp:91-139: AoS host bodies -> SoA device state.
+bool nbody_engine_hip::init(nbody_data* data)
+{
+	if(m_ctx == nullptr)
+	{
+		nbody_log("hip: no device selected");
+		return false;
+	}
+	m_data = data;
+	size_t count = data->get_count();
+	delete m_mass;
+	delete m_y;
+	m_mass = create_buffer(sizeof(nbcoord_t) * count);
+	m_y = create_buffer(sizeof(nbcoord_t) * problem_size());
+	if(m_mass == nullptr || m_y == nullptr)
+	{
+		return false;
+	}
+	std::vector<nbcoord_t> ybuf(6 * count);
+	const nbvertex_t* vrt = data->get_vertites();
+	const nbvertex_t* vel = data->get_velosites();
+	for(size_t i = 0; i != count; ++i)
+	{
+		ybuf[i] = vrt[i].x;
+		ybuf[count + i] = vrt[i].y;
+		ybuf[2 * count + i] = vrt[i].z;
+		ybuf[3 * count + i] = vel[i].x;
+		ybuf[4 * count + i] = vel[i].y;
+		ybuf[5 * count + i] = vel[i].z;
+	}
+	write_buffer(m_y, ybuf.data());
+	write_buffer(m_mass, data->get_mass());
+	return !m_error;
+}
+
+// nbody/nbody_engine_cuda.cpp:141-165
+void nbody_engine_hip::get_data(nbody_data* data)
+{
+	if(m_y == nullptr)
+	{
+		nbody_log("hip: engine not initialised");
+		return;
+	}
+	size_t count = m_data->get_count();
+	std::vector<nbcoord_t> ybuf(6 * count);
+	read_buffer(ybuf.data(), m_y);
+	nbvertex_t* vrt = data->get_vertites();
+	nbvertex_t* vel = data->get_velosites();
+	for(size_t i = 0; i != count; ++i)
+	{
+		vrt[i] = nbvertex_t(ybuf[i], ybuf[count + i], ybuf[2 * count + i]);
+		vel[i] = nbvertex_t(ybuf[3 * count + i], ybuf[4 * count + i], ybuf[5 * count + i]);
+	}
+}
+
+size_t nbody_engine_hip::problem_size() const
+{
+	return m_data == nullptr ? 0 : 6 * m_data->get_count();
+}
+
+nbody_engine::memory* nbody_engine_hip::get_y()
+{
+	return m_y;
+}
+
+void nbody_engine_hip::advise_time(const nbcoord_t& dt)
+{
+	m_data->advise_time(dt);
+}
+
+nbcoord_t nbody_engine_hip::get_time() const
+{
+	return m_data->get_time();
+}
+
+void nbody_engine_hip::set_time(nbcoord_t t)
+{
+	m_data->set_time(t);
+}
+
+size_t nbody_engine_hip::get_step() const
+{
+	return m_data->get_step();
+}
+
+void nbody_engine_hip::set_step(size_t s)
+{
+	m_data->set_step(s);
+}
+
+bool nbody_engine_hip::check_f(const memory* _y, const memory* _f, const smemory** ys, smemory** fs)
+{
+	const smemory* y = dynamic_cast<const smemory*>(_y);
+	smemory* f = dynamic_cast<smemory*>(const_cast<memory*>(_f));
+	if(y == nullptr)
+	{
+		nbody_log("y is not smemory");
+		return false;
+	}
+	if(f == nullptr)
+	{
+		nbody_log("f is not smemory");
+		return false;
+	}
+	if(m_data == nullptr || m_mass == nullptr)
+	{
+		nbody_log("hip: engine not initialised");
+		return false;
+	}
+	size_t need = sizeof(nbcoord_t) * problem_size();
+	if(y->size() < need || f->size() < need)
+	{
+		nbody_log("fcompute: buffers smaller than problem size");
+		return false;
+	}
+	if(y == f)
+	{
+		nbody_log("fcompute: y and f must be different buffers");
+		return false;
+	}
+	*ys = y;
+	*fs = f;
+	return true;
+}
+
+// nbody/nbody_engine_cuda.cpp:202-247
+void nbody_engine_hip::fcompute(const nbcoord_t& t, const memory* _y, memory* _f)
+{
+	(void)t;
+	const smemory* y = nullptr;
+	smemory* f = nullptr;
+	if(!check_f(_y, _f, &y, &f))
+	{
+		return;
+	}
+	advise_compute_count();
+	check(nbh_fcompute_direct(m_ctx, y->data(), m_mass->data(), f->data(), m_data->get_count(),
+	                          m_precision, m_block_size >= 64 && m_block_size % 64 == 0 &&
+	                          m_block_size <= 1024 ? m_block_size : 0),
+	      "fcompute");
+}
+
+void nbody_engine_hip::clamp(memory* _y, nbcoord_t b)
+{
+	smemory* y = dynamic_cast<smemory*>(_y);
+	if(y == nullptr)
+	{
+		nbody_log("y is not smemory");
+		return;
+	}
+	size_t count = m_data->get_count();
+	if(y->count() < 3 * count)
+	{
+		nbody_log("clamp: buffer smaller than 3*count");
+		return;
+	}
+	check(nbh_clamp(m_ctx, y->data(), count, b), "clamp");
+}
+
+nbody_engine_hip::smemory* nbody_engine_hip::create_buffer(size_t bytes)
+{
+	if(m_ctx == nullptr)
+	{
+		nbody_log("hip: no device selected");
+		return nullptr;
+	}
+	smemory* m = new smemory(m_ctx, bytes);
+	if(!m->valid())
+	{
+		delete m;
+		return nullptr;
+	}
+	return m;
+}
+
+void nbody_engine_hip::free_buffer(memory* m)
+{
+	delete m;
+}
+
+void nbody_engine_hip::read_buffer(void* dst, const memory* _src)
+{
+	const smemory* src = dynamic_cast<const smemory*>(_src);
+	if(src == nullptr)
+	{
+		nbody_log("src is not smemory");
+		return;
+	}
+	if(dst == nullptr)
+	{
+		nbody_log("dst is NULL");
+		return;
+	}
+	check(nbh_memcpy_d2h(m_ctx, dst, src->data(), src->size()), "read_buffer");
+}
+
+void nbody_engine_hip::write_buffer(memory* _dst, const void* src)
+{
+	smemory* dst = dynamic_cast<smemory*>(_dst);
+	if(dst == nullptr)
+	{
+		nbody_log("dst is not smemory");
+		return;
+	}
+	if(src == nullptr)
+	{
+		nbody_log("src is NULL");
+		return;
+	}
+	check(nbh_memcpy_h2d(m_ctx, dst->data(), src, dst->size()), "write_buffer");
+}
+
+void nbody_engine_hip::copy_buffer(memory* _a, const memory* _b)
+{
+	smemory* a = dynamic_cast<smemory*>(_a);
+	const smemory* b = dynamic_cast<const smemory*>(_b);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	if(b == nullptr)
+	{
+		nbody_log("b is not smemory");
+		return;
+	}
+	if(a->size() != b->size())
+	{
+		nbody_log("Size does not match");
+		return;
+	}
+	check(nbh_memcpy_d2d(m_ctx, a->data(), b->data(), a->size()), "copy_buffer");
+}
+
+void nbody_engine_hip::fill_buffer(memory* _a, const nbcoord_t& value)
+{
+	smemory* a = dynamic_cast<smemory*>(_a);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	check(nbh_fill(m_ctx, a->data(), value, a->count()), "fill_buffer");
+}
+
+// a[i] += b[i]*c
+void nbody_engine_hip::fmadd_inplace(memory* _a, const memory* _b, const nbcoord_t& c)
+{
+	smemory* a = dynamic_cast<smemory*>(_a);
+	const smemory* b = dynamic_cast<const smemory*>(_b);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	if(b == nullptr)
+	{
+		nbody_log("b is not smemory");
+		return;
+	}
+	if(b->count() < a->count())
+	{
+		nbody_log("b is smaller than a");
+		return;
+	}
+	const double* src[1] = { b->data() };
+	double coef[1] = { c };
+	check(nbh_fmaddn(m_ctx, a->data(), a->data(), src, coef, 1, a->count()), "fmadd_inplace");
+}
+
+// a[i] = b[i] + c[i]*d
+void nbody_engine_hip::fmadd(memory* _a, const memory* _b, const memory* _c, const nbcoord_t& d)
+{
+	smemory* a = dynamic_cast<smemory*>(_a);
+	const smemory* b = dynamic_cast<const smemory*>(_b);
+	const smemory* c = dynamic_cast<const smemory*>(_c);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	if(b == nullptr)
+	{
+		nbody_log("b is not smemory");
+		return;
+	}
+	if(c == nullptr)
+	{
+		nbody_log("c is not smemory");
+		return;
+	}
+	if(b->count() < a->count() || c->count() < a->count())
+	{
+		nbody_log("b or c is smaller than a");
+		return;
+	}
+	const double* src[1] = { c->data() };
+	double coef[1] = { d };
+	check(nbh_fmaddn(m_ctx, a->data(), b->data(), src, coef, 1, a->count()), "fmadd");
+}
+
+namespace {
+
+// Gather the terms with non-zero coefficients (the only ones the reference's
+// decomposition touches, nbody/nbody_engine.cpp:59-65, 94-107).  Any such
+// term that is not a device buffer of at least `count` doubles rejects the
+// whole call.
+bool gather_terms(const nbody_engine::memory_array& b, const nbcoord_t* c, size_t csize,
+                  size_t count, std::vector<const double*>* src, std::vector<double>* coef)
+{
+	for(size_t k = 0; k != csize; ++k)
+	{
+		if(c[k] == 0.0)
+		{
+			continue;
+		}
+		const nbody_engine_hip::smemory* m = dynamic_cast<const nbody_engine_hip::smemory*>(b[k]);
+		if(m == nullptr)
+		{
+			nbody_log("b is not smemory");
+			return false;
+		}
+		if(m->count() < count)
+		{
+			nbody_log("b[%zu] is smaller than a", k);
+			return false;
+		}
+		src->push_back(m->data());
+		coef->push_back(c[k]);
+	}
+	return true;
+}
+
+}  // namespace
+
+// a[i] += sum(b[k][i]*c[k]) in one pass (base class: one fmadd_inplace per term)
+void nbody_engine_hip::fmaddn_inplace(memory* _a, const memory_array& b, const nbcoord_t* c, size_t csize)
+{
+	if(c == nullptr)
+	{
+		return;
+	}
+	if(csize > b.size())
+	{
+		nbody_log("csize > b.size()");
+		return;
+	}
+	smemory* a = dynamic_cast<smemory*>(_a);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	std::vector<const double*> src;
+	std::vector<double> coef;
+	if(!gather_terms(b, c, csize, a->count(), &src, &coef))
+	{
+		return;
+	}
+	if(src.empty())
+	{
+		return;
+	}
+	check(nbh_fmaddn(m_ctx, a->data(), a->data(), src.data(), coef.data(), src.size(), a->count()),
+	      "fmaddn_inplace");
+}
+
+// nbody/nbody_engine_simple.cpp:341-394 semantics, all terms in one pass.
+void nbody_engine_hip::fmaddn_corr(memory* _a, memory* _corr, const memory_array& b,
+                                   const nbcoord_t* c, size_t csize)
+{
+	smemory* a = dynamic_cast<smemory*>(_a);
+	smemory* corr = dynamic_cast<smemory*>(_corr);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	if(corr == nullptr)
+	{
+		nbody_log("corr is not smemory");
+		return;
+	}
+	if(c == nullptr)
+	{
+		nbody_log("c is NULL");
+		return;
+	}
+	if(csize > b.size())
+	{
+		nbody_log("csize > b.size()");
+		return;
+	}
+	for(size_t k = 0; k < csize; ++k)
+	{
+		if(dynamic_cast<const smemory*>(b[k]) == nullptr)
+		{
+			nbody_log("b is not smemory");
+			return;
+		}
+	}
+	if(corr->count() < a->count())
+	{
+		nbody_log("corr is smaller than a");
+		return;
+	}
+	std::vector<const double*> src;
+	std::vector<double> coef;
+	if(!gather_terms(b, c, csize, a->count(), &src, &coef))
+	{
+		return;
+	}
+	if(src.empty())
+	{
+		return;
+	}
+	check(nbh_fmaddn_corr(m_ctx, a->data(), corr->data(), src.data(), coef.data(), src.size(), a->count()),
+	      "fmaddn_corr");
+}
+
+// a[i] = b[i] + sum(c[k][i]*d[k]) in one pass, b may be NULL (then 0).
+// As in the base class, with b != NULL and no non-zero d the call leaves a as is.
+void nbody_engine_hip::fmaddn(memory* _a, const memory* _b, const memory_array& c, const nbcoord_t* d, size_t dsize)
+{
+	if(d == nullptr)
+	{
+		nbody_log("d == NULL");
+		return;
+	}
+	if(dsize > c.size())
+	{
+		nbody_log("dsize > c.size()");
+		return;
+	}
+	smemory* a = dynamic_cast<smemory*>(_a);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	const smemory* b = nullptr;
+	if(_b != nullptr)
+	{
+		b = dynamic_cast<const smemory*>(_b);
+		if(b == nullptr)
+		{
+			nbody_log("b is not smemory");
+			return;
+		}
+		if(b->count() < a->count())
+		{
+			nbody_log("b is smaller than a");
+			return;
+		}
+	}
+	std::vector<const double*> src;
+	std::vector<double> coef;
+	if(!gather_terms(c, d, dsize, a->count(), &src, &coef))
+	{
+		return;
+	}
+	if(b != nullptr && src.empty())
+	{
+		return;
+	}
+	check(nbh_fmaddn(m_ctx, a->data(), b == nullptr ? nullptr : b->data(), src.data(), coef.data(),
+	                 src.size(), a->count()),
+	      "fmaddn");
+}
+
+void nbody_engine_hip::fmaxabs(const memory* _a, nbcoord_t& result)
+{
+	const smemory* a = dynamic_cast<const smemory*>(_a);
+	if(a == nullptr)
+	{
+		nbody_log("a is not smemory");
+		return;
+	}
+	double r = 0;
+	if(check(nbh_maxabs(m_ctx, a->data(), a->count(), &r), "fmaxabs"))
+	{
+		result = r;
+	}
+}
+
+void nbody_engine_hip::print_info() const
+{
+	char name[128] = {0};
+	int cus = 0;
+	if(m_ctx != nullptr)
+	{
+		nbh_ctx_info(m_ctx, name, sizeof(name), &cus);
+	}
+	nbody_print("\tdevice: %d (%s, %d CUs)", m_device, name, cus);
+	nbody_print("\tblock_size: %d", m_block_size);
+	nbody_print("\tforce_precision: %s", m_precision == NBH_FP64 ? "fp64" : "fp32");
+}
+
+bool nbody_engine_hip::compute_statistics(bool with_energy, double out[12])
+{
+	if(m_y == nullptr || m_mass == nullptr)
+	{
+		return false;
+	}
+	return check(nbh_stats(m_ctx, m_y->data(), m_mass->data(), m_data->get_count(), with_energy ? 1 : 0, out),
+	             "statistics");
+}
+
+// ---------------------------------------------------------------------------
+// Barnes-Hut
+// ---------------------------------------------------------------------------
+const char* tree_layout_name(e_tree_layout tree_layout)
+{
+	switch(tree_layout)
+	{
+	case etl_tree:
+		return "tree";
+	case etl_heap:
+		return "heap";
+	case etl_heap_stackless:
+		return "heap_stackless";
+	default:
+		return "";
+	}
+}
+
+e_tree_layout tree_layout_from_str(const std::string& name)
+{
+	if(name == "tree") { return etl_tree; }
+	if(name == "heap") { return etl_heap; }
+	if(name == "heap_stackless") { return etl_heap_stackless; }
+	return etl_unknown;
+}
+
+nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, size_t rate, e_tree_layout tl) :
+	m_ratio(ratio),
+	m_traverse_type(tt),
+	m_tree_build_rate(rate),
+	m_tree_layout(tl),
+	m_tree(nullptr),
+	m_tree_built(false)
+{
+}
+
+nbody_engine_hip_bh::~nbody_engine_hip_bh()
+{
+	if(m_tree != nullptr)
+	{
+		nbh_tree_destroy(m_tree);
+	}
+}
+
+const char* nbody_engine_hip_bh::type_name() const
+{
+	return "nbody_engine_hip_bh";
+}
+
+bool nbody_engine_hip_bh::init(nbody_data* data)
+{
+	size_t n = data->get_count();
+	if(n == 0 || (n & (n - 1)) != 0)
+	{
+		nbody_log("hip_bh: body count %zu is not a power of two (heap layout needs 2^k leaves)", n);
+		return false;
+	}
+	if(!nbody_engine_hip::init(data))
+	{
+		return false;
+	}
+	if(m_tree != nullptr)
+	{
+		nbh_tree_destroy(m_tree);
+		m_tree = nullptr;
+	}
+	m_tree_built = false;
+	return check(nbh_tree_create(m_ctx, n, &m_tree), "tree allocation");
+}
+
+// Build or refit, then traverse (nbody/nbody_engine_simple_bh.cpp:16-93,
+// nbody/nbody_engine_cuda_bh_tex.cpp:46-201).  Targets are always walked in
+// heap leaf order; `cycle` gives the same numbers (same positions and masses).
+void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory* _f)
+{
+	(void)t;
+	const smemory* y = nullptr;
+	smemory* f = nullptr;
+	if(!check_f(_y, _f, &y, &f))
+	{
+		return;
+	}
+	if(m_tree == nullptr)
+	{
+		nbody_log("hip_bh: engine not initialised");
+		return;
+	}
+	advise_compute_count();
+	// tree.is_empty() || rate == 0 || step % rate == 0 -> full build
+	bool build = !m_tree_built || m_tree_build_rate == 0 || (m_data->get_step() % m_tree_build_rate) == 0;
+	if(build)
+	{
+		if(!check(nbh_tree_build(m_ctx, m_tree, y->data(), m_mass->data(), m_ratio), "tree build"))
+		{
+			return;
+		}
+		m_tree_built = true;
+	}
+	else if(!check(nbh_tree_refit(m_ctx, m_tree, y->data(), m_ratio), "tree refit"))
+	{
+		return;
+	}
+	check(nbh_bh_fcompute(m_ctx, m_tree, y->data(), m_mass->data(), f->data(),
+	                      m_tree_layout == etl_heap ? NBH_LAYOUT_HEAP : NBH_LAYOUT_HEAP_STACKLESS, m_precision),
+	      "bh fcompute");
+}
+
+void nbody_engine_hip_bh::print_info() const
+{
+	nbody_engine_hip::print_info();
+	nbody_print("\tdistance_to_node_radius_ratio: %g", m_ratio);
+	nbody_print("\ttraverse_type: %s", m_traverse_type == ett_cycle ? "cycle" : "nested_tree");
+	nbody_print("\ttree_layout: %s", tree_layout_name(m_tree_layout));
+	nbody_print("\ttree_build_rate: %zu", m_tree_build_rate);
+}

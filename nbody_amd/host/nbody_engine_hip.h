@@ -1,0 +1,128 @@
+// nbody_engine_hip.h -- the `hip` (direct all-pairs) and `hip_bh` (Barnes-Hut)
+// engines: nbody_engine implementations whose buffers live in MI355X HBM and
+// whose every operation is one or a few launches of the gfx950 kernels behind
+// the C ABI of include/nbody_hip.h.
+//
+// Reference counterparts: nbody_engine_cuda (nbody/nbody_engine_cuda.{h,cpp})
+// and nbody_engine_cuda_bh / nbody_engine_cuda_bh_tex
+// (nbody/nbody_engine_cuda_bh*.{h,cpp}).  Differences by design: one device
+// per engine, the multi-term ops are single fused launches, the Barnes-Hut
+// tree is built on the GPU, and HIP failures are logged and latched in
+// has_error() instead of terminating the process.
+#pragma once
+
+#include <string>
+
+#include "nbody_engine.h"
+#include "nbody_hip.h"
+
+enum e_traverse_type
+{
+	ett_cycle,
+	ett_nested_tree
+};
+
+enum e_tree_layout
+{
+	etl_tree,
+	etl_heap,
+	etl_heap_stackless,
+	etl_unknown = 0xffff
+};
+
+const char* tree_layout_name(e_tree_layout tree_layout);
+e_tree_layout tree_layout_from_str(const std::string& name);
+
+class nbody_engine_hip : public nbody_engine
+{
+public:
+	class smemory : public memory
+	{
+		nbh_ctx* m_ctx;
+		void*    m_data;
+		size_t   m_size;
+	public:
+		smemory(nbh_ctx* ctx, size_t size);
+		~smemory() override;
+		bool valid() const { return m_size == 0 || m_data != nullptr; }
+		double* data() { return static_cast<double*>(m_data); }
+		const double* data() const { return static_cast<const double*>(m_data); }
+		size_t size() const override { return m_size; }
+		size_t count() const { return m_size / sizeof(double); }
+	};
+
+protected:
+	nbh_ctx*      m_ctx;
+	int           m_device;
+	int           m_block_size;
+	nbh_precision m_precision;
+	nbody_data*   m_data;
+	smemory*      m_mass;
+	smemory*      m_y;
+	bool          m_error;
+
+	bool check(nbh_status s, const char* what);
+	bool check_f(const memory* y, const memory* f, const smemory** ys, smemory** fs);
+public:
+	nbody_engine_hip();
+	~nbody_engine_hip() override;
+
+	//! Select the HIP device; returns 0 on success (nbody_engine_cuda::select_devices).
+	int select_device(int device);
+	void set_block_size(int block_size);
+	void set_precision(nbh_precision p);
+	nbh_precision get_precision() const { return m_precision; }
+	//! True once any HIP call of this engine has failed.
+	bool has_error() const { return m_error; }
+	nbh_ctx* context() { return m_ctx; }
+	const smemory* mass_buffer() const { return m_mass; }
+
+	const char* type_name() const override;
+	bool init(nbody_data* data) override;
+	void get_data(nbody_data* data) override;
+	size_t problem_size() const override;
+	memory* get_y() override;
+	void advise_time(const nbcoord_t& dt) override;
+	nbcoord_t get_time() const override;
+	void set_time(nbcoord_t t) override;
+	size_t get_step() const override;
+	void set_step(size_t s) override;
+	void fcompute(const nbcoord_t& t, const memory* y, memory* f) override;
+	void clamp(memory* y, nbcoord_t b) override;
+
+	smemory* create_buffer(size_t bytes) override;
+	void free_buffer(memory* m) override;
+	void read_buffer(void* dst, const memory* src) override;
+	void write_buffer(memory* dst, const void* src) override;
+	void copy_buffer(memory* a, const memory* b) override;
+	void fill_buffer(memory* a, const nbcoord_t& value) override;
+	void fmadd_inplace(memory* a, const memory* b, const nbcoord_t& c) override;
+	void fmadd(memory* a, const memory* b, const memory* c, const nbcoord_t& d) override;
+	void fmaddn_inplace(memory* a, const memory_array& b, const nbcoord_t* c, size_t csize) override;
+	void fmaddn_corr(memory* a, memory* corr, const memory_array& b, const nbcoord_t* c, size_t csize) override;
+	void fmaddn(memory* a, const memory* b, const memory_array& c, const nbcoord_t* d, size_t dsize) override;
+	void fmaxabs(const memory* a, nbcoord_t& result) override;
+	void print_info() const override;
+	bool compute_statistics(bool with_energy, double out[12]) override;
+};
+
+class nbody_engine_hip_bh : public nbody_engine_hip
+{
+	nbcoord_t       m_ratio;
+	e_traverse_type m_traverse_type;
+	size_t          m_tree_build_rate;
+	e_tree_layout   m_tree_layout;
+	nbh_tree*       m_tree;
+	bool            m_tree_built;
+public:
+	nbody_engine_hip_bh(nbcoord_t distance_to_node_radius_ratio, e_traverse_type tt,
+	                    size_t tree_build_rate, e_tree_layout tl);
+	~nbody_engine_hip_bh() override;
+	const char* type_name() const override;
+	bool init(nbody_data* data) override;
+	void fcompute(const nbcoord_t& t, const memory* y, memory* f) override;
+	void print_info() const override;
+	const nbh_tree* tree() const { return m_tree; }
+	nbcoord_t ratio() const { return m_ratio; }
+	e_tree_layout layout() const { return m_tree_layout; }
+};

@@ -1,0 +1,110 @@
+// nbody_engines.cpp -- see nbody_engines.h.
+#include "nbody_engines.h"
+
+#include <cstdlib>
+
+#include "nbody_engine_hip.h"
+#include "nbody_log.h"
+
+namespace {
+
+bool parse_device(const std::string& s, int* device)
+{
+	if(s.empty() || s.find_first_not_of("0123456789") != std::string::npos)
+	{
+		return false;
+	}
+	*device = std::atoi(s.c_str());
+	return true;
+}
+
+bool common_hip_params(const nbody_params& param, nbody_engine_hip* engine)
+{
+	int device = 0;
+	std::string dev = param.value("device", "0");
+	if(!parse_device(dev, &device))
+	{
+		nbody_log("Invalid device '%s': one device id is expected", dev.c_str());
+		return false;
+	}
+	long long block_size = 0;
+	if(!param.get_long("block_size", NBODY_DATA_BLOCK_SIZE, &block_size) || block_size <= 0)
+	{
+		nbody_log("Invalid block_size");
+		return false;
+	}
+	std::string prec = param.value("force_precision", "fp64");
+	if(prec != "fp64" && prec != "fp32")
+	{
+		nbody_log("Invalid force_precision. Allowed values are 'fp64' or 'fp32'");
+		return false;
+	}
+	if(engine->select_device(device) != 0)
+	{
+		return false;
+	}
+	engine->set_block_size(static_cast<int>(block_size));
+	engine->set_precision(prec == "fp64" ? NBH_FP64 : NBH_FP32);
+	return true;
+}
+
+}  // namespace
+
+nbody_engine* nbody_create_engine(const nbody_params& param)
+{
+	const std::string type(param.value("engine", ""));
+	if(type == "hip")
+	{
+		nbody_engine_hip* engine = new nbody_engine_hip();
+		if(!common_hip_params(param, engine))
+		{
+			delete engine;
+			return nullptr;
+		}
+		return engine;
+	}
+	if(type == "hip_bh")
+	{
+		double ratio = 0;
+		long long rate = 0;
+		if(!param.get_double("distance_to_node_radius_ratio", 10, &ratio) || !(ratio > 0))
+		{
+			nbody_log("Invalid distance_to_node_radius_ratio");
+			return nullptr;
+		}
+		if(!param.get_long("tree_build_rate", 0, &rate) || rate < 0)
+		{
+			nbody_log("Invalid tree_build_rate");
+			return nullptr;
+		}
+		e_tree_layout tl = tree_layout_from_str(param.value("tree_layout", "heap"));
+		if(tl != etl_heap && tl != etl_heap_stackless)
+		{
+			nbody_log("Invalid tree_layout. Allowed values are 'heap' or 'heap_stackless'");
+			return nullptr;
+		}
+		std::string strtt = param.value("traverse_type", "cycle");
+		e_traverse_type tt;
+		if(strtt == "cycle")
+		{
+			tt = ett_cycle;
+		}
+		else if(strtt == "nested_tree")
+		{
+			tt = ett_nested_tree;
+		}
+		else
+		{
+			nbody_log("Invalid traverse_type. Allowed values are 'cycle' or 'nested_tree'");
+			return nullptr;
+		}
+		nbody_engine_hip_bh* engine = new nbody_engine_hip_bh(ratio, tt, static_cast<size_t>(rate), tl);
+		if(!common_hip_params(param, engine))
+		{
+			delete engine;
+			return nullptr;
+		}
+		return engine;
+	}
+	return nullptr;
+}
