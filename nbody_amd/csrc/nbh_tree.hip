@@ -1,55 +1,837 @@
-// nbh_tree.hip -- Barnes-Hut heap kd-tree (build, refit, traversal).
+// nbh_tree.hip -- Barnes-Hut heap kd-tree on the GPU: exact build, refit and
+// traversal (gfx950).
+//
+// Layout (nbody/nbody_space_heap.cpp:13-37): n = 2^L bodies, heap indices
+// 1..2n-1 (root 1, children 2i and 2i+1, leaves at [n, 2n)).
+//
+// Build = the reference's recursive median split (nbody_space_heap.cpp:136-191)
+// done level by level for every segment at once:
+//   1. three stable LSD radix sorts give the bodies in x, y and z order (keys:
+//      order-preserving 64-bit images of the FP64 coordinates; equal
+//      coordinates keep index order);
+//   2. at level l every segment of S = n >> l consecutive list entries holds the
+//      bodies of one node; along axis a = l % 3 the first S/2 entries of the
+//      a-sorted list are exactly the count/2 smallest, i.e. std::nth_element's
+//      left part.  Each body is tagged left/right, and the two other lists are
+//      stably partitioned inside every segment (segmented scan + scatter), so
+//      every child segment again holds its bodies sorted along all three axes;
+//   3. after L levels entry s of any list is the body of leaf n + s;
+//   4. moments bottom-up with the operations of nbody_space_heap::update
+//      (:119-134), in FP64 with contraction off: bit-identical node records.
+// Traversal: one target per work-item, targets in heap-leaf order (spatially
+// coherent), nodes as 32-byte AoS records {cm.xyz, crit_r2} plus mass.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "nbh_common.h"
 
+namespace {
+
+constexpr int kRoot = 1;               // NBODY_HEAP_ROOT_INDEX
+constexpr int kMaxStack = 64;          // MAX_STACK_SIZE of the reference traversal
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 4;
+constexpr int kSortTile = kSortThreads * kSortItems;   // 1024 keys per work-group
+constexpr int kPartThreads = 256;
+constexpr int kPartItems = 4;
+constexpr int kPartTile = kPartThreads * kPartItems;   // 1024 entries per work-group
+constexpr int kScanThreads = 1024;
+
+}  // namespace
+
 struct nbh_tree {
-	nbh_ctx* ctx = nullptr;
-	size_t   n = 0;
+	nbh_ctx*  ctx = nullptr;
+	size_t    n = 0;
+	int       levels = 0;
+	// per-axis sorted body lists and scratch
+	uint32_t* list[3] = {nullptr, nullptr, nullptr};
+	uint32_t* tmp = nullptr;
+	uint8_t*  side = nullptr;
+	uint64_t* keys = nullptr;      // [2][n] radix sort ping-pong
+	uint32_t* vals = nullptr;      // [2][n]
+	uint32_t* counts = nullptr;    // radix / partition block counts
+	// nodes, heap index 0..2n-1 (0 unused)
+	double4*  xyzr = nullptr;      // cm.x, cm.y, cm.z, crit_r2
+	double*   mass = nullptr;
+	double*   bmin = nullptr;      // [2n*3]
+	double*   bmax = nullptr;      // [2n*3]
+	int32_t*  body_n = nullptr;    // -1 on internal nodes
+	unsigned long long* counters = nullptr;   // [2] interactions, visits
+	bool      built = false;
 };
 
-extern "C" {
+namespace {
 
-nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** tree)
+// ---------------------------------------------------------------------------
+// radix sort of (64-bit key, 32-bit value), stable, 8 bits per pass
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t order_key(double v)
 {
-	(void)ctx; (void)n; (void)tree;
-	nbh::set_error("Barnes-Hut tree not available in this build");
-	return NBH_ERR_UNSUPPORTED;
+	v = v + 0.0;                                   // -0 -> +0: equal values compare equal
+	uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+	return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
 
-nbh_status nbh_tree_destroy(nbh_tree* tree)
+__global__ void __launch_bounds__(256)
+k_make_keys(const double* __restrict__ coord, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, size_t n)
 {
-	delete tree;
+	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		keys[i] = order_key(coord[i]);
+		vals[i] = static_cast<uint32_t>(i);
+	}
+}
+
+// counts[d * nblocks + b] = number of keys of tile b with digit d
+__global__ void __launch_bounds__(kSortThreads)
+k_radix_hist(const uint64_t* __restrict__ keys, size_t n, int shift, uint32_t* __restrict__ counts, int nblocks)
+{
+	__shared__ uint32_t hist[256];
+	hist[threadIdx.x] = 0;
+	__syncthreads();
+	size_t base = static_cast<size_t>(blockIdx.x) * kSortTile;
+	for(int k = 0; k < kSortItems; ++k)
+	{
+		size_t i = base + static_cast<size_t>(k) * kSortThreads + threadIdx.x;
+		if(i < n)
+		{
+			atomicAdd(&hist[(keys[i] >> shift) & 0xFF], 1u);
+		}
+	}
+	__syncthreads();
+	counts[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Exclusive scan in place of `m` uint32 counts by one work-group.
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_single(uint32_t* __restrict__ c, size_t m)
+{
+	__shared__ uint32_t sh[kScanThreads];
+	size_t per = (m + kScanThreads - 1) / kScanThreads;
+	size_t b = threadIdx.x * per;
+	size_t e = b + per < m ? b + per : m;
+	uint32_t s = 0;
+	for(size_t i = b; i < e; ++i) { s += c[i]; }
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	for(int off = 1; off < kScanThreads; off <<= 1)
+	{
+		uint32_t v = threadIdx.x >= static_cast<unsigned>(off) ? sh[threadIdx.x - off] : 0;
+		__syncthreads();
+		sh[threadIdx.x] += v;
+		__syncthreads();
+	}
+	uint32_t run = threadIdx.x == 0 ? 0 : sh[threadIdx.x - 1];
+	for(size_t i = b; i < e; ++i)
+	{
+		uint32_t v = c[i];
+		c[i] = run;
+		run += v;
+	}
+}
+
+// Stable scatter: rows of 256 keys are ranked in index order, wave by wave,
+// with a ballot match on the 8 digit bits.
+__global__ void __launch_bounds__(kSortThreads)
+k_radix_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
+                const uint32_t* __restrict__ offsets, int nblocks)
+{
+	constexpr int kWaves = kSortThreads / 64;
+	__shared__ uint32_t running[256];
+	__shared__ uint32_t wcount[kWaves][256];
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	running[threadIdx.x] = offsets[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x];
+	const size_t base = static_cast<size_t>(blockIdx.x) * kSortTile;
+	const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	for(int k = 0; k < kSortItems; ++k)
+	{
+		for(int w = 0; w < kWaves; ++w) { wcount[w][threadIdx.x] = 0; }
+		__syncthreads();
+		const size_t i = base + static_cast<size_t>(k) * kSortThreads + threadIdx.x;
+		const bool ok = i < n;
+		uint64_t key = ok ? kin[i] : 0;
+		uint32_t val = ok ? vin[i] : 0;
+		uint32_t d = static_cast<uint32_t>((key >> shift) & 0xFF);
+		uint64_t active = __ballot(ok);
+		uint64_t peers = active;
+		for(int bit = 0; bit < 8; ++bit)
+		{
+			uint64_t bal = __ballot((d >> bit) & 1u);
+			peers &= ((d >> bit) & 1u) ? bal : ~bal;
+		}
+		uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt_mask));
+		uint32_t cnt = static_cast<uint32_t>(__popcll(peers));
+		bool leader = ok && (peers & lt_mask) == 0;
+		if(leader) { wcount[wave][d] = cnt; }
+		__syncthreads();
+		if(ok)
+		{
+			uint32_t pos = running[d] + rank;
+			for(int w = 0; w < wave; ++w) { pos += wcount[w][d]; }
+			kout[pos] = key;
+			vout[pos] = val;
+		}
+		__syncthreads();
+		uint32_t add = 0;
+		for(int w = 0; w < kWaves; ++w) { add += wcount[w][threadIdx.x]; }
+		running[threadIdx.x] += add;
+		__syncthreads();
+	}
+}
+
+// ---------------------------------------------------------------------------
+// per-level split
+// ---------------------------------------------------------------------------
+// side[list_a[p]] = 1 for the upper half of p's segment
+__global__ void __launch_bounds__(256)
+k_mark_side(const uint32_t* __restrict__ list_a, uint8_t* __restrict__ side, size_t n, size_t seg)
+{
+	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	size_t half = seg >> 1;
+	for(size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
+	{
+		side[list_a[p]] = (p & (seg - 1)) >= half ? 1 : 0;
+	}
+}
+
+// rights[b] = number of upper-half bodies in tile b of `list`
+__global__ void __launch_bounds__(kPartThreads)
+k_part_count(const uint32_t* __restrict__ list, const uint8_t* __restrict__ side, size_t n,
+             uint32_t* __restrict__ rights)
+{
+	__shared__ uint32_t wsum[kPartThreads / 64];
+	size_t base = static_cast<size_t>(blockIdx.x) * kPartTile + static_cast<size_t>(threadIdx.x) * kPartItems;
+	uint32_t c = 0;
+	for(int k = 0; k < kPartItems; ++k)
+	{
+		size_t p = base + k;
+		if(p < n) { c += side[list[p]]; }
+	}
+	for(int off = 32; off > 0; off >>= 1) { c += __shfl_down(c, off, 64); }
+	if((threadIdx.x & 63) == 0) { wsum[threadIdx.x >> 6] = c; }
+	__syncthreads();
+	if(threadIdx.x == 0)
+	{
+		uint32_t t = 0;
+		for(int w = 0; w < kPartThreads / 64; ++w) { t += wsum[w]; }
+		rights[blockIdx.x] = t;
+	}
+}
+
+// Exclusive scan of the tile counts, restarting every `per_seg` tiles.
+__global__ void __launch_bounds__(256)
+k_part_scan(uint32_t* __restrict__ rights, size_t ntiles, size_t per_seg)
+{
+	size_t nseg = ntiles / per_seg;
+	for(size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < nseg;
+	    s += static_cast<size_t>(gridDim.x) * blockDim.x)
+	{
+		uint32_t run = 0;
+		for(size_t b = s * per_seg; b < (s + 1) * per_seg; ++b)
+		{
+			uint32_t v = rights[b];
+			rights[b] = run;
+			run += v;
+		}
+	}
+}
+
+// Stable partition of every segment of `list` into lower-half then upper-half
+// bodies (flags in `side`), written to `out`.  `tile_off` holds, per tile, the
+// number of upper-half bodies before the tile inside its segment (segments of
+// at least one tile); smaller segments are handled inside the tile.
+__global__ void __launch_bounds__(kPartThreads)
+k_part_scatter(const uint32_t* __restrict__ list, const uint8_t* __restrict__ side, size_t n, size_t seg,
+               const uint32_t* __restrict__ tile_off, uint32_t* __restrict__ out)
+{
+	__shared__ uint32_t excl[kPartTile];
+	__shared__ uint32_t tsum[kPartThreads];
+	const size_t tile = static_cast<size_t>(blockIdx.x) * kPartTile;
+	const int tb = threadIdx.x * kPartItems;
+	uint32_t body[kPartItems];
+	uint8_t fl[kPartItems];
+	uint32_t c = 0;
+	for(int k = 0; k < kPartItems; ++k)
+	{
+		size_t p = tile + tb + k;
+		body[k] = p < n ? list[p] : 0;
+		fl[k] = p < n ? side[body[k]] : 0;
+		excl[tb + k] = c;
+		c += fl[k];
+	}
+	tsum[threadIdx.x] = c;
+	__syncthreads();
+	for(int off = 1; off < kPartThreads; off <<= 1)
+	{
+		uint32_t v = threadIdx.x >= static_cast<unsigned>(off) ? tsum[threadIdx.x - off] : 0;
+		__syncthreads();
+		tsum[threadIdx.x] += v;
+		__syncthreads();
+	}
+	uint32_t add = threadIdx.x == 0 ? 0 : tsum[threadIdx.x - 1];
+	for(int k = 0; k < kPartItems; ++k) { excl[tb + k] += add; }
+	__syncthreads();
+	const size_t half = seg >> 1;
+	for(int k = 0; k < kPartItems; ++k)
+	{
+		size_t p = tile + tb + k;
+		if(p >= n) { continue; }
+		size_t seg_start = p & ~(seg - 1);
+		uint32_t before;
+		if(seg >= static_cast<size_t>(kPartTile))
+		{
+			before = tile_off[blockIdx.x] + excl[tb + k];
+		}
+		else
+		{
+			before = excl[tb + k] - excl[seg_start - tile];
+		}
+		size_t pos = fl[k] ? seg_start + half + before : p - before;
+		out[pos] = body[k];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// moments (nbody/nbody_space_heap.cpp:119-134), FP64, contraction off
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_leaves(const uint32_t* __restrict__ order, const double* __restrict__ y, const double* __restrict__ mass,
+         size_t n, double4* __restrict__ xyzr, double* __restrict__ nmass, double* __restrict__ bmin,
+         double* __restrict__ bmax, int32_t* __restrict__ body_n, int set_body)
+{
+	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < n; s += stride)
+	{
+		size_t idx = n + s;
+		uint32_t b = set_body ? order[s] : static_cast<uint32_t>(body_n[idx]);
+		double x = y[b], yy = y[n + b], z = y[2 * n + b];
+		xyzr[idx] = make_double4(x, yy, z, 0.0);
+		if(set_body)
+		{
+			body_n[idx] = static_cast<int32_t>(b);
+			nmass[idx] = mass[b];
+		}
+		bmin[3 * idx] = x; bmin[3 * idx + 1] = yy; bmin[3 * idx + 2] = z;
+		bmax[3 * idx] = x; bmax[3 * idx + 1] = yy; bmax[3 * idx + 2] = z;
+	}
+}
+
+__device__ inline double min_ref(double a, double b) { return (b < a) ? b : a; }   // std::min
+__device__ inline double max_ref(double a, double b) { return (a < b) ? b : a; }   // std::max
+
+__device__ inline void update_node(size_t idx, double4* __restrict__ xyzr, double* __restrict__ nmass,
+                                   double* __restrict__ bmin, double* __restrict__ bmax, double ratio_sqr)
+{
+	size_t l = 2 * idx, r = 2 * idx + 1;
+	double ml = nmass[l], mr = nmass[r];
+	double m = ml + mr;
+	double4 cl = xyzr[l], cr = xyzr[r];
+	double cx = (cl.x * ml + cr.x * mr) / m;
+	double cy = (cl.y * ml + cr.y * mr) / m;
+	double cz = (cl.z * ml + cr.z * mr) / m;
+	double lo[3], hi[3];
+	for(int d = 0; d < 3; ++d)
+	{
+		lo[d] = min_ref(bmin[3 * l + d], bmin[3 * r + d]);
+		hi[d] = max_ref(bmax[3 * l + d], bmax[3 * r + d]);
+		bmin[3 * idx + d] = lo[d];
+		bmax[3 * idx + d] = hi[d];
+	}
+	double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+	double gx = (hi[0] + lo[0]) / 2.0 - cx;
+	double gy = (hi[1] + lo[1]) / 2.0 - cy;
+	double gz = (hi[2] + lo[2]) / 2.0 - cz;
+	double rad = sqrt(ex * ex + ey * ey + ez * ez) * 0.5 + sqrt(gx * gx + gy * gy + gz * gz);
+	nmass[idx] = m;
+	xyzr[idx] = make_double4(cx, cy, cz, (rad * rad) * ratio_sqr);
+}
+
+// nodes [lo, hi) of one level
+__global__ void __launch_bounds__(256)
+k_level(size_t lo, size_t hi, double4* __restrict__ xyzr, double* __restrict__ nmass, double* __restrict__ bmin,
+        double* __restrict__ bmax, double ratio_sqr)
+{
+	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t idx = lo + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < hi; idx += stride)
+	{
+		update_node(idx, xyzr, nmass, bmin, bmax, ratio_sqr);
+	}
+}
+
+// all levels with fewer than 2 * 256 nodes in one work-group, bottom-up
+__global__ void __launch_bounds__(256)
+k_levels_top(size_t top, double4* __restrict__ xyzr, double* __restrict__ nmass, double* __restrict__ bmin,
+             double* __restrict__ bmax, double ratio_sqr)
+{
+	for(size_t lo = top; lo >= 1; lo >>= 1)
+	{
+		for(size_t idx = lo + threadIdx.x; idx < 2 * lo; idx += blockDim.x)
+		{
+			update_node(idx, xyzr, nmass, bmin, bmax, ratio_sqr);
+		}
+		__syncthreads();
+	}
+}
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+// heap index algebra (nbody/nbody_space_heap_func_priv.h:4-72)
+__device__ inline int next_down(int idx)
+{
+	idx = idx >> __builtin_ctz(~static_cast<unsigned>(idx));
+	return idx + 1;
+}
+
+__device__ inline int next_up(int idx, int tree_size)
+{
+	int left = idx << 1;
+	return left < tree_size ? left : next_down(idx);
+}
+
+// Counters are cleared by a kernel rather than a memset node, so a captured
+// sequence of calls replays correctly.
+__global__ void k_zero_counters(unsigned long long* counters)
+{
+	if(threadIdx.x < 2) { counters[threadIdx.x] = 0; }
+}
+
+__device__ inline void count_add(unsigned long long* counters, unsigned long long inter, unsigned long long vis)
+{
+	for(int off = 32; off > 0; off >>= 1)
+	{
+		inter += __shfl_down(inter, off, 64);
+		vis += __shfl_down(vis, off, 64);
+	}
+	if((threadIdx.x & 63) == 0)
+	{
+		atomicAdd(&counters[0], inter);
+		atomicAdd(&counters[1], vis);
+	}
+}
+
+// FP64 monopole term of nbody_data::force: tot += (v1 - cm) * (-(m1 M) / r^3)
+__device__ inline void add_force64(double x1, double y1, double z1, double gm1, const double4& nd, double M,
+                                   double& tx, double& ty, double& tz)
+{
+	double dx = x1 - nd.x;
+	double dy = y1 - nd.y;
+	double dz = z1 - nd.z;
+	double r2 = dx * dx + dy * dy + dz * dz;
+	if(r2 < 1e-8) { r2 = 1e-8; }
+	double coeff = (gm1 * M) / (r2 * sqrt(r2));
+	tx += dx * coeff;
+	ty += dy * coeff;
+	tz += dz * coeff;
+}
+
+__device__ inline void write_f(const double* __restrict__ y, const double* __restrict__ mass, double* __restrict__ f,
+                               size_t n, int b, double tx, double ty, double tz)
+{
+	double m1 = mass[b];
+	f[b] = y[3 * n + b];
+	f[n + b] = y[4 * n + b];
+	f[2 * n + b] = y[5 * n + b];
+	f[3 * n + b] = tx / m1;
+	f[4 * n + b] = ty / m1;
+	f[5 * n + b] = tz / m1;
+}
+
+// nbody_space_heap_stackless::traverse (nbody_space_heap_stackless.cpp:3-26)
+__global__ void __launch_bounds__(256)
+k_bh_stackless64(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
+                 const int32_t* __restrict__ body_n, const double* __restrict__ y, const double* __restrict__ mass,
+                 double* __restrict__ f, int n, unsigned long long* __restrict__ counters)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int tree_size = 2 * n;
+	unsigned long long inter = 0, vis = 0;
+	if(t < n)
+	{
+		const int tn = n + t;
+		const double4 me = xyzr[tn];
+		const double m1 = nmass[tn];
+		const double gm1 = -1.0 * m1;
+		double tx = 0.0, ty = 0.0, tz = 0.0;
+		int curr = kRoot;
+		do
+		{
+			const double4 nd = xyzr[curr];
+			const double dx = me.x - nd.x;
+			const double dy = me.y - nd.y;
+			const double dz = me.z - nd.z;
+			const double d2 = dx * dx + dy * dy + dz * dz;
+			++vis;
+			if(d2 > nd.w)
+			{
+				add_force64(me.x, me.y, me.z, gm1, nd, nmass[curr], tx, ty, tz);
+				++inter;
+				curr = next_down(curr);
+			}
+			else
+			{
+				curr = next_up(curr, tree_size);
+			}
+		}
+		while(curr != kRoot);
+		write_f(y, mass, f, n, body_n[tn], tx, ty, tz);
+	}
+	count_add(counters, inter, vis);
+}
+
+// nbody_space_heap::traverse (nbody_space_heap.cpp:64-97): explicit stack, the
+// right child pushed first so the left subtree is walked first.
+__global__ void __launch_bounds__(256)
+k_bh_stack64(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
+             const int32_t* __restrict__ body_n, const double* __restrict__ y, const double* __restrict__ mass,
+             double* __restrict__ f, int n, unsigned long long* __restrict__ counters)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int tree_size = 2 * n;
+	unsigned long long inter = 0, vis = 0;
+	if(t < n)
+	{
+		const int tn = n + t;
+		const double4 me = xyzr[tn];
+		const double m1 = nmass[tn];
+		const double gm1 = -1.0 * m1;
+		double tx = 0.0, ty = 0.0, tz = 0.0;
+		int stack[kMaxStack];
+		int sp = 0;
+		stack[sp++] = kRoot;
+		while(sp != 0)
+		{
+			const int curr = stack[--sp];
+			const double4 nd = xyzr[curr];
+			const double dx = me.x - nd.x;
+			const double dy = me.y - nd.y;
+			const double dz = me.z - nd.z;
+			const double d2 = dx * dx + dy * dy + dz * dz;
+			++vis;
+			if(d2 > nd.w)
+			{
+				add_force64(me.x, me.y, me.z, gm1, nd, nmass[curr], tx, ty, tz);
+				++inter;
+			}
+			else
+			{
+				const int left = curr << 1;
+				const int rght = left + 1;
+				if(rght < tree_size && sp < kMaxStack) { stack[sp++] = rght; }
+				if(left < tree_size && sp < kMaxStack) { stack[sp++] = left; }
+			}
+		}
+		write_f(y, mass, f, n, body_n[tn], tx, ty, tz);
+	}
+	count_add(counters, inter, vis);
+}
+
+// FP32 interaction math: FP64 offsets to the node centre are rounded to FP32,
+// the opening test and the monopole term run in FP32 with v_rsq_f32; sums in
+// FP32 with a Kahan carry.
+__global__ void __launch_bounds__(256)
+k_bh_stackless32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
+                 const int32_t* __restrict__ body_n, const double* __restrict__ y, const double* __restrict__ mass,
+                 double* __restrict__ f, int n, unsigned long long* __restrict__ counters)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int tree_size = 2 * n;
+	unsigned long long inter = 0, vis = 0;
+	if(t < n)
+	{
+		const int tn = n + t;
+		const double4 me = xyzr[tn];
+		float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+		int curr = kRoot;
+		do
+		{
+			const double4 nd = xyzr[curr];
+			const float dx = static_cast<float>(nd.x - me.x);
+			const float dy = static_cast<float>(nd.y - me.y);
+			const float dz = static_cast<float>(nd.z - me.z);
+			const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+			++vis;
+			if(d2 > static_cast<float>(nd.w))
+			{
+				const float r2 = fmaxf(d2, 1e-8f);
+				const float ri = __builtin_amdgcn_rsqf(r2);
+				const float w = (static_cast<float>(nmass[curr]) * ri) * (ri * ri);
+				float v = dx * w - cx; float s = ax + v; cx = (s - ax) - v; ax = s;
+				v = dy * w - cy; s = ay + v; cy = (s - ay) - v; ay = s;
+				v = dz * w - cz; s = az + v; cz = (s - az) - v; az = s;
+				++inter;
+				curr = next_down(curr);
+			}
+			else
+			{
+				curr = next_up(curr, tree_size);
+			}
+		}
+		while(curr != kRoot);
+		const int b = body_n[tn];
+		f[b] = y[3 * n + b];
+		f[n + b] = y[4 * n + b];
+		f[2 * n + b] = y[5 * n + b];
+		f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx);
+		f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy);
+		f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz);
+	}
+	count_add(counters, inter, vis);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+nbh_status radix_sort(nbh_ctx* ctx, nbh_tree* t, const double* coord, uint32_t* out_vals)
+{
+	const size_t n = t->n;
+	const int nblocks = static_cast<int>((n + kSortTile - 1) / kSortTile);
+	uint64_t* k0 = t->keys;
+	uint64_t* k1 = t->keys + n;
+	uint32_t* v0 = t->vals;
+	uint32_t* v1 = t->vals + n;
+	hipLaunchKernelGGL(k_make_keys, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, coord, k0, v0, n);
+	NBH_LAUNCHED();
+	for(int shift = 0; shift < 64; shift += 8)
+	{
+		hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, n, shift,
+		                   t->counts, nblocks);
+		NBH_LAUNCHED();
+		hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, ctx->stream, t->counts,
+		                   static_cast<size_t>(nblocks) * 256);
+		NBH_LAUNCHED();
+		hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, v0, k1, v1, n,
+		                   shift, t->counts, nblocks);
+		NBH_LAUNCHED();
+		std::swap(k0, k1);
+		std::swap(v0, v1);
+	}
+	// 8 passes: the result is back in the first half
+	NBH_HIP(hipMemcpyAsync(out_vals, v0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
 	return NBH_OK;
 }
 
-nbh_status nbh_tree_build(nbh_ctx*, nbh_tree*, const double*, const double*, double)
+nbh_status moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mass, double ratio, bool build)
 {
-	nbh::set_error("Barnes-Hut tree not available in this build");
-	return NBH_ERR_UNSUPPORTED;
+	const size_t n = t->n;
+	hipLaunchKernelGGL(k_leaves, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->list[0], y, mass,
+	                   n, t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, build ? 1 : 0);
+	NBH_LAUNCHED();
+	const double ratio_sqr = ratio * ratio;
+	size_t lo = n / 2;
+	for(; lo >= 512; lo >>= 1)
+	{
+		hipLaunchKernelGGL(k_level, dim3(nbh::grid_for(ctx, lo, 256)), dim3(256), 0, ctx->stream, lo, 2 * lo,
+		                   t->xyzr, t->mass, t->bmin, t->bmax, ratio_sqr);
+		NBH_LAUNCHED();
+	}
+	if(lo >= 1)
+	{
+		hipLaunchKernelGGL(k_levels_top, dim3(1), dim3(256), 0, ctx->stream, lo, t->xyzr, t->mass, t->bmin, t->bmax,
+		                   ratio_sqr);
+		NBH_LAUNCHED();
+	}
+	return NBH_OK;
 }
 
-nbh_status nbh_tree_refit(nbh_ctx*, nbh_tree*, const double*, double)
+}  // namespace
+
+extern "C" {
+
+nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 {
-	nbh::set_error("Barnes-Hut tree not available in this build");
-	return NBH_ERR_UNSUPPORTED;
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(out != nullptr, "null output pointer");
+	*out = nullptr;
+	NBH_CHECK(n >= 1 && (n & (n - 1)) == 0, "body count %zu is not a power of two", n);
+	NBH_CHECK(n <= (static_cast<size_t>(1) << 28), "body count %zu too large", n);
+	nbh_tree* t = new nbh_tree();
+	t->ctx = ctx;
+	t->n = n;
+	while((static_cast<size_t>(1) << t->levels) < n) { ++t->levels; }
+	size_t nb = (n + kSortTile - 1) / kSortTile;
+	size_t ncounts = std::max(nb * 256, (n + kPartTile - 1) / kPartTile) + 1;
+	hipError_t e = hipSuccess;
+	auto alloc = [&](void** p, size_t bytes) {
+		if(e == hipSuccess) { e = hipMalloc(p, bytes); }
+	};
+	for(int a = 0; a < 3; ++a) { alloc(reinterpret_cast<void**>(&t->list[a]), n * sizeof(uint32_t)); }
+	alloc(reinterpret_cast<void**>(&t->tmp), n * sizeof(uint32_t));
+	alloc(reinterpret_cast<void**>(&t->side), n);
+	alloc(reinterpret_cast<void**>(&t->keys), 2 * n * sizeof(uint64_t));
+	alloc(reinterpret_cast<void**>(&t->vals), 2 * n * sizeof(uint32_t));
+	alloc(reinterpret_cast<void**>(&t->counts), ncounts * sizeof(uint32_t));
+	alloc(reinterpret_cast<void**>(&t->xyzr), 2 * n * sizeof(double4));
+	alloc(reinterpret_cast<void**>(&t->mass), 2 * n * sizeof(double));
+	alloc(reinterpret_cast<void**>(&t->bmin), 2 * n * 3 * sizeof(double));
+	alloc(reinterpret_cast<void**>(&t->bmax), 2 * n * 3 * sizeof(double));
+	alloc(reinterpret_cast<void**>(&t->body_n), 2 * n * sizeof(int32_t));
+	alloc(reinterpret_cast<void**>(&t->counters), 2 * sizeof(unsigned long long));
+	if(e == hipSuccess) { e = hipMemsetAsync(t->xyzr, 0, 2 * n * sizeof(double4), ctx->stream); }
+	if(e == hipSuccess) { e = hipMemsetAsync(t->mass, 0, 2 * n * sizeof(double), ctx->stream); }
+	if(e == hipSuccess) { e = hipMemsetAsync(t->body_n, 0xff, 2 * n * sizeof(int32_t), ctx->stream); }
+	if(e == hipSuccess) { e = hipMemsetAsync(t->counters, 0, 2 * sizeof(unsigned long long), ctx->stream); }
+	if(e == hipSuccess) { e = hipStreamSynchronize(ctx->stream); }
+	if(e != hipSuccess)
+	{
+		nbh_tree_destroy(t);
+		return nbh::hip_fail(e, "tree allocation", __FILE__, __LINE__);
+	}
+	*out = t;
+	return NBH_OK;
 }
 
-nbh_status nbh_bh_fcompute(nbh_ctx*, const nbh_tree*, const double*, const double*, double*,
-                           nbh_tree_layout, nbh_precision)
+nbh_status nbh_tree_destroy(nbh_tree* t)
 {
-	nbh::set_error("Barnes-Hut tree not available in this build");
-	return NBH_ERR_UNSUPPORTED;
+	if(t == nullptr)
+	{
+		return NBH_OK;
+	}
+	if(t->ctx != nullptr)
+	{
+		(void)hipSetDevice(t->ctx->device);
+		(void)hipStreamSynchronize(t->ctx->stream);
+	}
+	void* ptrs[] = { t->list[0], t->list[1], t->list[2], t->tmp, t->side, t->keys, t->vals, t->counts,
+	                 t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, t->counters };
+	for(void* p : ptrs)
+	{
+		if(p != nullptr) { (void)hipFree(p); }
+	}
+	delete t;
+	return NBH_OK;
 }
 
-nbh_status nbh_tree_get(nbh_ctx*, const nbh_tree*, int32_t*, double*, double*, double*, double*, double*)
+nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mass, double ratio)
 {
-	nbh::set_error("Barnes-Hut tree not available in this build");
-	return NBH_ERR_UNSUPPORTED;
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(y != nullptr && mass != nullptr, "null buffer");
+	nbh::scoped_timer timer(ctx, "tree_build");
+	const size_t n = t->n;
+	for(int a = 0; a < 3; ++a)
+	{
+		NBH_TRY(radix_sort(ctx, t, y + static_cast<size_t>(a) * n, t->list[a]));
+	}
+	const size_t ntiles = (n + kPartTile - 1) / kPartTile;
+	for(int level = 0; level < t->levels; ++level)
+	{
+		const size_t seg = n >> level;
+		const int a = level % 3;
+		hipLaunchKernelGGL(k_mark_side, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->list[a],
+		                   t->side, n, seg);
+		NBH_LAUNCHED();
+		for(int b = 0; b < 3; ++b)
+		{
+			if(b == a) { continue; }
+			if(seg >= static_cast<size_t>(kPartTile))
+			{
+				hipLaunchKernelGGL(k_part_count, dim3(static_cast<unsigned>(ntiles)), dim3(kPartThreads), 0,
+				                   ctx->stream, t->list[b], t->side, n, t->counts);
+				NBH_LAUNCHED();
+				size_t per_seg = seg / kPartTile;
+				size_t nseg = ntiles / per_seg;
+				hipLaunchKernelGGL(k_part_scan, dim3(nbh::grid_for(ctx, nseg, 256)), dim3(256), 0, ctx->stream,
+				                   t->counts, ntiles, per_seg);
+				NBH_LAUNCHED();
+			}
+			hipLaunchKernelGGL(k_part_scatter, dim3(static_cast<unsigned>(ntiles)), dim3(kPartThreads), 0,
+			                   ctx->stream, t->list[b], t->side, n, seg, t->counts, t->tmp);
+			NBH_LAUNCHED();
+			std::swap(t->list[b], t->tmp);
+		}
+	}
+	NBH_TRY(moments(ctx, t, y, mass, ratio, true));
+	t->built = true;
+	return NBH_OK;
 }
 
-nbh_status nbh_bh_counters(nbh_ctx*, const nbh_tree*, uint64_t* a, uint64_t* b)
+nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* t, const double* y, double ratio)
 {
-	if(a) { *a = 0; }
-	if(b) { *b = 0; }
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(y != nullptr, "null buffer");
+	nbh::scoped_timer timer(ctx, "tree_refit");
+	return moments(ctx, t, y, nullptr, ratio, false);
+}
+
+nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* t, const double* y, const double* mass, double* f,
+                           nbh_tree_layout layout, nbh_precision precision)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(y != nullptr && mass != nullptr && f != nullptr && y != f, "bad buffers");
+	NBH_CHECK(layout == NBH_LAYOUT_HEAP || layout == NBH_LAYOUT_HEAP_STACKLESS, "unknown layout %d",
+	          static_cast<int>(layout));
+	NBH_CHECK(precision == NBH_FP64 || precision == NBH_FP32, "unknown precision %d", static_cast<int>(precision));
+	const int n = static_cast<int>(t->n);
+	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
+	NBH_LAUNCHED();
+	nbh::scoped_timer timer(ctx, "bh_traverse");
+	dim3 grid((n + 255) / 256);
+	if(precision == NBH_FP32)
+	{
+		hipLaunchKernelGGL(k_bh_stackless32, grid, dim3(256), 0, ctx->stream, t->xyzr, t->mass, t->body_n, y, mass,
+		                   f, n, t->counters);
+	}
+	else if(layout == NBH_LAYOUT_HEAP)
+	{
+		hipLaunchKernelGGL(k_bh_stack64, grid, dim3(256), 0, ctx->stream, t->xyzr, t->mass, t->body_n, y, mass, f,
+		                   n, t->counters);
+	}
+	else
+	{
+		hipLaunchKernelGGL(k_bh_stackless64, grid, dim3(256), 0, ctx->stream, t->xyzr, t->mass, t->body_n, y, mass,
+		                   f, n, t->counters);
+	}
+	NBH_LAUNCHED();
+	return NBH_OK;
+}
+
+nbh_status nbh_tree_get(nbh_ctx* ctx, const nbh_tree* t, int32_t* body_n, double* cm, double* mass,
+                        double* crit_r2, double* bmin, double* bmax)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	const size_t s = 2 * t->n;
+	NBH_HIP(hipStreamSynchronize(ctx->stream));
+	if(body_n != nullptr) { NBH_HIP(hipMemcpy(body_n, t->body_n, s * sizeof(int32_t), hipMemcpyDeviceToHost)); }
+	if(mass != nullptr) { NBH_HIP(hipMemcpy(mass, t->mass, s * sizeof(double), hipMemcpyDeviceToHost)); }
+	if(bmin != nullptr) { NBH_HIP(hipMemcpy(bmin, t->bmin, 3 * s * sizeof(double), hipMemcpyDeviceToHost)); }
+	if(bmax != nullptr) { NBH_HIP(hipMemcpy(bmax, t->bmax, 3 * s * sizeof(double), hipMemcpyDeviceToHost)); }
+	if(cm != nullptr || crit_r2 != nullptr)
+	{
+		std::vector<double> rec(4 * s);
+		NBH_HIP(hipMemcpy(rec.data(), t->xyzr, s * sizeof(double4), hipMemcpyDeviceToHost));
+		for(size_t i = 0; i < s; ++i)
+		{
+			if(cm != nullptr)
+			{
+				cm[3 * i] = rec[4 * i];
+				cm[3 * i + 1] = rec[4 * i + 1];
+				cm[3 * i + 2] = rec[4 * i + 2];
+			}
+			if(crit_r2 != nullptr) { crit_r2[i] = rec[4 * i + 3]; }
+		}
+	}
+	return NBH_OK;
+}
+
+nbh_status nbh_bh_counters(nbh_ctx* ctx, const nbh_tree* t, uint64_t* interactions, uint64_t* visits)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	unsigned long long c[2] = {0, 0};
+	NBH_HIP(hipMemcpyAsync(c, t->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+	NBH_HIP(hipStreamSynchronize(ctx->stream));
+	if(interactions != nullptr) { *interactions = c[0]; }
+	if(visits != nullptr) { *visits = c[1]; }
 	return NBH_OK;
 }
 

@@ -26,7 +26,7 @@ def universe():
     return d
 
 
-@pytest.fixture(scope="module", params=["hip"])
+@pytest.fixture(scope="module", params=["hip", "hip_bh_1e8"])
 def engine(request, universe):
     if request.param == "hip":
         e = nbody_create_engine({"engine": "hip"})
@@ -200,11 +200,13 @@ def test_fcompute_vs_simple(engine, universe):
     """test_nbody_engine.cpp:472-558: against `simple` at 1e-13, two rounds with
     y scaled by 1.99 in between; the FP64 direct kernel is bit-exact."""
     y, m = oracle_lib.state_from_data(universe)
+    # BH with lambda = 1e8 sums the same pairs in tree order: 1e-11 as in the reference (:1361-1396)
+    eps = 1e-13 if engine.type_name() == "nbody_engine_hip" else 1e-11
     for step in range(2):
         engine.set_step(step)
         f = _fcompute(engine, y)
         f0 = oracle_lib.fcompute_simple(y, m)
-        assert np.max(np.abs(f - f0)) <= 1e-13
+        assert np.max(np.abs(f - f0)) <= eps
         if engine.type_name() == "nbody_engine_hip":
             assert np.array_equal(f, f0), "FP64 direct sum must equal the CPU order bit for bit"
         y = y + y * 0.99
