@@ -100,6 +100,13 @@ nbh_status nbh_clamp(nbh_ctx* ctx, double* y, size_t nbodies, double b);
 nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const double* mass, double* f,
                                size_t n, nbh_precision precision, int block_size);
 
+/* FP64 accelerations of the m bodies listed in idx (device array of indices < n):
+   acc[3t..3t+2] for body idx[t], same operations and source order as
+   nbody_engine_simple::fcompute.  Used to measure the error of the FP32 and
+   Barnes-Hut paths on sampled targets where a full FP64 sum is too costly. */
+nbh_status nbh_direct_sample(nbh_ctx* ctx, const double* y, const double* mass, size_t n,
+                             const uint32_t* idx, size_t m, double* acc);
+
 /* ---- Barnes-Hut heap kd-tree --------------------------------------------
    n must be a power of two (the heap layout of nbody/nbody_space_heap.cpp:23
    puts the leaves at [n, 2n)).  The tree is built on the GPU with the same
@@ -121,6 +128,14 @@ nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* tree, const double* y, double 
 nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* tree, const double* y,
                            const double* mass, double* f, nbh_tree_layout layout,
                            nbh_precision precision);
+/* Group walk (FP32 interaction math, FP64 state): the 64 bodies under each heap
+   node of level log2(n/64) share one walk; a cell is taken when the distance
+   from the group's box to its mass centre exceeds its opening radius, so each
+   body sees a refinement of the cells its own walk would take (error <= that
+   of nbh_bh_fcompute at the same ratio).  Replaces the per-thread walk of
+   kfcompute_heap_bh_stackless (nbody_engine_cuda_impl.cu:372-451) on the fast
+   path; n >= 64. */
+nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f);
 /* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may
    be NULL.  cm/bmin/bmax are xyz-interleaved [2n*3]; body_n holds -1 on
    internal nodes.  Used to check the layout against the reference heap

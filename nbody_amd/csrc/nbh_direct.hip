@@ -92,6 +92,60 @@ k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass,
 	}
 }
 
+// Same sum for the targets idx[0..m): acc[3t..3t+2] = acceleration of body idx[t].
+__global__ void __launch_bounds__(256)
+k_direct_sample64(const double* __restrict__ y, const double* __restrict__ mass, size_t n,
+                  const uint32_t* __restrict__ idx, size_t m, double* __restrict__ acc)
+{
+	__shared__ double sx[256], sy[256], sz[256], sm[256];
+	const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	const bool active = t < m;
+	const size_t i = active ? idx[t] : 0;
+	const double x1 = active ? y[i] : 0.0;
+	const double y1 = active ? y[n + i] : 0.0;
+	const double z1 = active ? y[2 * n + i] : 0.0;
+	const double m1 = active ? mass[i] : 1.0;
+	const double gm1 = -1.0 * m1;
+	double tx = 0.0, ty = 0.0, tz = 0.0;
+	for(size_t base = 0; base < n; base += 256)
+	{
+		const size_t j = base + threadIdx.x;
+		__syncthreads();
+		sx[threadIdx.x] = j < n ? y[j] : 0.0;
+		sy[threadIdx.x] = j < n ? y[n + j] : 0.0;
+		sz[threadIdx.x] = j < n ? y[2 * n + j] : 0.0;
+		sm[threadIdx.x] = j < n ? mass[j] : 0.0;
+		__syncthreads();
+		const int cnt = n - base < 256 ? static_cast<int>(n - base) : 256;
+		const int self = (active && i >= base && i < base + cnt) ? static_cast<int>(i - base) : -1;
+		for(int jj = 0; jj < cnt; ++jj)
+		{
+			if(jj == self)
+			{
+				continue;
+			}
+			const double dx = x1 - sx[jj];
+			const double dy = y1 - sy[jj];
+			const double dz = z1 - sz[jj];
+			double r2 = dx * dx + dy * dy + dz * dz;
+			if(r2 < kMinDistance)
+			{
+				r2 = kMinDistance;
+			}
+			const double coeff = (gm1 * sm[jj]) / (r2 * sqrt(r2));
+			tx += dx * coeff;
+			ty += dy * coeff;
+			tz += dz * coeff;
+		}
+	}
+	if(active)
+	{
+		acc[3 * t] = tx / m1;
+		acc[3 * t + 1] = ty / m1;
+		acc[3 * t + 2] = tz / m1;
+	}
+}
+
 // ---------------------------------------------------------------------------
 // FP32 tiles
 // ---------------------------------------------------------------------------
@@ -228,6 +282,18 @@ k_direct_fp32_finish(const double* __restrict__ y, const double* __restrict__ pa
 }
 
 }  // namespace
+
+extern "C" nbh_status nbh_direct_sample(nbh_ctx* ctx, const double* y, const double* mass, size_t n,
+                                        const uint32_t* idx, size_t m, double* acc)
+{
+	NBH_TRY(nbh::enter(ctx));
+	if(m == 0 || n == 0) { return NBH_OK; }
+	NBH_CHECK(y != nullptr && mass != nullptr && idx != nullptr && acc != nullptr, "null buffer");
+	hipLaunchKernelGGL(k_direct_sample64, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0, ctx->stream,
+	                   y, mass, n, idx, m, acc);
+	NBH_LAUNCHED();
+	return NBH_OK;
+}
 
 extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const double* mass,
                                           double* f, size_t n, nbh_precision precision,

@@ -57,7 +57,7 @@ struct nbh_tree {
 	double*   bmin = nullptr;      // [2n*3]
 	double*   bmax = nullptr;      // [2n*3]
 	int32_t*  body_n = nullptr;    // -1 on internal nodes
-	unsigned long long* counters = nullptr;   // [2] interactions, visits
+	unsigned long long* counters = nullptr;   // [3] interactions, visits, stack spills
 	bool      built = false;
 };
 
@@ -397,7 +397,7 @@ __device__ inline int next_up(int idx, int tree_size)
 // sequence of calls replays correctly.
 __global__ void k_zero_counters(unsigned long long* counters)
 {
-	if(threadIdx.x < 2) { counters[threadIdx.x] = 0; }
+	if(threadIdx.x < 3) { counters[threadIdx.x] = 0; }
 }
 
 __device__ inline void count_add(unsigned long long* counters, unsigned long long inter, unsigned long long vis)
@@ -584,6 +584,160 @@ k_bh_stackless32(const double4* __restrict__ xyzr, const double* __restrict__ nm
 }
 
 // ---------------------------------------------------------------------------
+// group walk: one wave per subtree of 64 leaves
+// ---------------------------------------------------------------------------
+// The 64 targets of a wave are the leaves of one heap node g, whose tight box
+// bounds them all.  The wave walks the tree once for the whole group, 64 nodes
+// per step from an LDS stack: a leaf is always taken as a body; an internal node
+// is taken as a monopole when the distance from the group box to its mass
+// centre exceeds its opening radius (then every target of the group accepts it
+// too, so the cell set refines each target's own walk); otherwise its children
+// are pushed.  Accepted entries are buffered in LDS as FP32 offsets from the
+// box centre and swept by all 64 lanes, the same pair arithmetic as the direct
+// FP32 kernel.  No node is dropped: a child push that would overflow the stack
+// takes the node as a monopole instead (counted in counters[2]).
+constexpr int kGroup = 64;
+constexpr int kWalkStack = 2048;
+constexpr int kWalkBuf = 512;
+
+__device__ inline void sweep32(const float4* __restrict__ buf, int count, float tx, float ty, float tz,
+                               float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
+{
+	float px = 0.0f, py = 0.0f, pz = 0.0f;
+#pragma unroll 4
+	for(int j = 0; j < count; ++j)
+	{
+		const float4 s = buf[j];
+		const float dx = s.x - tx;
+		const float dy = s.y - ty;
+		const float dz = s.z - tz;
+		float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+		r2 = fmaxf(r2, 1e-8f);
+		const float ri = __builtin_amdgcn_rsqf(r2);
+		const float w = (s.w * ri) * (ri * ri);
+		px = fmaf(dx, w, px);
+		py = fmaf(dy, w, py);
+		pz = fmaf(dz, w, pz);
+	}
+	float v = px - cx; float t = ax + v; cx = (t - ax) - v; ax = t;
+	v = py - cy; t = ay + v; cy = (t - ay) - v; ay = t;
+	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
+}
+
+__global__ void __launch_bounds__(kGroup)
+k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
+             const double* __restrict__ bmax, const int32_t* __restrict__ body_n, const double* __restrict__ y,
+             double* __restrict__ f, int n, unsigned long long* __restrict__ counters)
+{
+	__shared__ int stack[kWalkStack];
+	__shared__ float4 buf[kWalkBuf];
+	const int lane = threadIdx.x;
+	const int g = n / kGroup + blockIdx.x;              // heap node of this group
+	const double gx0 = bmin[3 * g], gy0 = bmin[3 * g + 1], gz0 = bmin[3 * g + 2];
+	const double gx1 = bmax[3 * g], gy1 = bmax[3 * g + 1], gz1 = bmax[3 * g + 2];
+	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
+	const int tn = n + blockIdx.x * kGroup + lane;     // this lane's leaf
+	const double4 me = xyzr[tn];
+	const float tx = static_cast<float>(me.x - ccx);
+	const float ty = static_cast<float>(me.y - ccy);
+	const float tz = static_cast<float>(me.z - ccz);
+	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	unsigned long long vis = 0, taken = 0, spill = 0;
+
+	int sp = 1;
+	int nbuf = 0;
+	if(lane == 0) { stack[0] = kRoot; }
+	__syncthreads();
+	while(sp > 0)
+	{
+		const int cnt = sp < kGroup ? sp : kGroup;
+		const bool valid = lane < cnt;
+		const int node = valid ? stack[sp - cnt + lane] : kRoot;
+		sp -= cnt;
+		__syncthreads();
+		bool accept = false, open = false;
+		float4 ent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		if(valid)
+		{
+			const double4 rec = xyzr[node];
+			const double m = nmass[node];
+			ent = make_float4(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
+			                  static_cast<float>(rec.z - ccz), static_cast<float>(m));
+			if(node >= n)
+			{
+				accept = true;
+			}
+			else
+			{
+				// distance from the group box to the node's mass centre
+				const double ex = fmax(fmax(gx0 - rec.x, rec.x - gx1), 0.0);
+				const double ey = fmax(fmax(gy0 - rec.y, rec.y - gy1), 0.0);
+				const double ez = fmax(fmax(gz0 - rec.z, rec.z - gz1), 0.0);
+				accept = ex * ex + ey * ey + ez * ez > rec.w;
+				open = !accept;
+			}
+		}
+		const uint64_t bopen = __ballot(open);
+		const int nopen = __popcll(bopen);
+		if(sp + 2 * nopen > kWalkStack)
+		{
+			// no room for the children: take these nodes as monopoles
+			spill += open ? 1 : 0;
+			accept = accept || open;
+			open = false;
+		}
+		const uint64_t bacc = __ballot(accept);
+		const uint64_t bopen2 = __ballot(open);
+		if(accept)
+		{
+			buf[nbuf + __popcll(bacc & lt)] = ent;
+		}
+		if(open)
+		{
+			const int r = __popcll(bopen2 & lt);
+			stack[sp + 2 * r] = 2 * node + 1;
+			stack[sp + 2 * r + 1] = 2 * node;
+		}
+		nbuf += __popcll(bacc);
+		sp += 2 * __popcll(bopen2);
+		vis += valid ? 1 : 0;
+		taken += accept ? 1 : 0;
+		__syncthreads();
+		if(nbuf > kWalkBuf - kGroup)
+		{
+			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
+			nbuf = 0;
+			__syncthreads();
+		}
+	}
+	if(nbuf > 0)
+	{
+		sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
+	}
+	const int b = body_n[tn];
+	f[b] = y[3 * n + b];
+	f[n + b] = y[4 * n + b];
+	f[2 * n + b] = y[5 * n + b];
+	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx);
+	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy);
+	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz);
+	// counters: [0] pair interactions (entries x 64 targets), [1] node visits, [2] spills
+	for(int off = 32; off > 0; off >>= 1)
+	{
+		vis += __shfl_down(vis, off, 64);
+		taken += __shfl_down(taken, off, 64);
+		spill += __shfl_down(spill, off, 64);
+	}
+	if(lane == 0)
+	{
+		atomicAdd(&counters[0], taken * kGroup);
+		atomicAdd(&counters[1], vis);
+		if(spill) { atomicAdd(&counters[2], spill); }
+	}
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 nbh_status radix_sort(nbh_ctx* ctx, nbh_tree* t, const double* coord, uint32_t* out_vals)
@@ -670,11 +824,11 @@ nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 	alloc(reinterpret_cast<void**>(&t->bmin), 2 * n * 3 * sizeof(double));
 	alloc(reinterpret_cast<void**>(&t->bmax), 2 * n * 3 * sizeof(double));
 	alloc(reinterpret_cast<void**>(&t->body_n), 2 * n * sizeof(int32_t));
-	alloc(reinterpret_cast<void**>(&t->counters), 2 * sizeof(unsigned long long));
+	alloc(reinterpret_cast<void**>(&t->counters), 3 * sizeof(unsigned long long));
 	if(e == hipSuccess) { e = hipMemsetAsync(t->xyzr, 0, 2 * n * sizeof(double4), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->mass, 0, 2 * n * sizeof(double), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->body_n, 0xff, 2 * n * sizeof(int32_t), ctx->stream); }
-	if(e == hipSuccess) { e = hipMemsetAsync(t->counters, 0, 2 * sizeof(unsigned long long), ctx->stream); }
+	if(e == hipSuccess) { e = hipMemsetAsync(t->counters, 0, 3 * sizeof(unsigned long long), ctx->stream); }
 	if(e == hipSuccess) { e = hipStreamSynchronize(ctx->stream); }
 	if(e != hipSuccess)
 	{
@@ -794,6 +948,24 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* t, const double* y, con
 	return NBH_OK;
 }
 
+nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
+	NBH_CHECK(t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
+	NBH_CHECK(t->levels + 1 <= kWalkStack / kGroup, "tree too deep for the walk stack");
+	const int n = static_cast<int>(t->n);
+	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
+	NBH_LAUNCHED();
+	nbh::scoped_timer timer(ctx, "bh_traverse");
+	hipLaunchKernelGGL(k_bh_group32, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass, t->bmin,
+	                   t->bmax, t->body_n, y, f, n, t->counters);
+	NBH_LAUNCHED();
+	return NBH_OK;
+}
+
 nbh_status nbh_tree_get(nbh_ctx* ctx, const nbh_tree* t, int32_t* body_n, double* cm, double* mass,
                         double* crit_r2, double* bmin, double* bmax)
 {
@@ -827,7 +999,7 @@ nbh_status nbh_bh_counters(nbh_ctx* ctx, const nbh_tree* t, uint64_t* interactio
 {
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
-	unsigned long long c[2] = {0, 0};
+	unsigned long long c[3] = {0, 0, 0};
 	NBH_HIP(hipMemcpyAsync(c, t->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
 	NBH_HIP(hipStreamSynchronize(ctx->stream));
 	if(interactions != nullptr) { *interactions = c[0]; }

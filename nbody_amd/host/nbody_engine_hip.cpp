@@ -638,7 +638,8 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_tree_build_rate(rate),
 	m_tree_layout(tl),
 	m_tree(nullptr),
-	m_tree_built(false)
+	m_tree_built(false),
+	m_group_walk(false)
 {
 }
 
@@ -708,6 +709,11 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 	{
 		return;
 	}
+	if(m_group_walk && m_precision == NBH_FP32 && m_data->get_count() >= 64)
+	{
+		check(nbh_bh_fcompute_group(m_ctx, m_tree, y->data(), f->data()), "bh fcompute (group walk)");
+		return;
+	}
 	check(nbh_bh_fcompute(m_ctx, m_tree, y->data(), m_mass->data(), f->data(),
 	                      m_tree_layout == etl_heap ? NBH_LAYOUT_HEAP : NBH_LAYOUT_HEAP_STACKLESS, m_precision),
 	      "bh fcompute");
@@ -720,4 +726,5 @@ void nbody_engine_hip_bh::print_info() const
 	nbody_print("\ttraverse_type: %s", m_traverse_type == ett_cycle ? "cycle" : "nested_tree");
 	nbody_print("\ttree_layout: %s", tree_layout_name(m_tree_layout));
 	nbody_print("\ttree_build_rate: %zu", m_tree_build_rate);
+	nbody_print("\tbh_walk: %s", m_group_walk ? "group" : "body");
 }

@@ -114,9 +114,12 @@ class nbody_engine_hip_bh : public nbody_engine_hip
 	e_tree_layout   m_tree_layout;
 	nbh_tree*       m_tree;
 	bool            m_tree_built;
+	bool            m_group_walk;
 public:
 	nbody_engine_hip_bh(nbcoord_t distance_to_node_radius_ratio, e_traverse_type tt,
 	                    size_t tree_build_rate, e_tree_layout tl);
+	//! Use the shared 64-body group walk (FP32 only) instead of one walk per body.
+	void set_group_walk(bool g) { m_group_walk = g; }
 	~nbody_engine_hip_bh() override;
 	const char* type_name() const override;
 	bool init(nbody_data* data) override;

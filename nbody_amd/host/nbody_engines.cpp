@@ -98,7 +98,19 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("Invalid traverse_type. Allowed values are 'cycle' or 'nested_tree'");
 			return nullptr;
 		}
+		std::string walk = param.value("bh_walk", "body");
+		if(walk != "body" && walk != "group")
+		{
+			nbody_log("Invalid bh_walk. Allowed values are 'body' or 'group'");
+			return nullptr;
+		}
+		if(walk == "group" && param.value("force_precision", "fp64") != "fp32")
+		{
+			nbody_log("bh_walk=group needs force_precision=fp32");
+			return nullptr;
+		}
 		nbody_engine_hip_bh* engine = new nbody_engine_hip_bh(ratio, tt, static_cast<size_t>(rate), tl);
+		engine->set_group_walk(walk == "group");
 		if(!common_hip_params(param, engine))
 		{
 			delete engine;

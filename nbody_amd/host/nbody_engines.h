@@ -14,6 +14,9 @@
 //                                   = rebuild when step % k == 0, refit otherwise
 //   tree_layout                     "heap" (default) | "heap_stackless"
 //   traverse_type                   "cycle" (default) | "nested_tree"
+//   bh_walk                         "body" (default: one walk per body, the
+//                                   reference traversal) | "group" (one walk per
+//                                   64 bodies, fp32 only)
 // Any unknown engine name or invalid value returns NULL.
 #pragma once
 

@@ -49,11 +49,13 @@ HIP_API = {
     "nbh_maxabs": (i32, [vp, vp, sz, PD]),
     "nbh_clamp": (i32, [vp, vp, sz, dbl]),
     "nbh_fcompute_direct": (i32, [vp, vp, vp, vp, sz, i32, i32]),
+    "nbh_direct_sample": (i32, [vp, vp, vp, sz, vp, sz, vp]),
     "nbh_tree_create": (i32, [vp, sz, C.POINTER(vp)]),
     "nbh_tree_destroy": (i32, [vp]),
     "nbh_tree_build": (i32, [vp, vp, vp, vp, dbl]),
     "nbh_tree_refit": (i32, [vp, vp, vp, dbl]),
     "nbh_bh_fcompute": (i32, [vp, vp, vp, vp, vp, i32, i32]),
+    "nbh_bh_fcompute_group": (i32, [vp, vp, vp, vp]),
     "nbh_tree_get": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "nbh_bh_counters": (i32, [vp, vp, C.POINTER(u64), C.POINTER(u64)]),
     "nbh_stats": (i32, [vp, vp, vp, sz, i32, PD]),

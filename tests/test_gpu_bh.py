@@ -158,6 +158,73 @@ def test_traverse_fp32_close_to_fp64(ctx):
     t.close()
 
 
+def _sample_accel(ctx, dy, dm, n, idx):
+    di = ctx.alloc((len(idx) + 1) // 2)
+    raw = np.ascontiguousarray(idx, dtype=np.uint32)
+    native.check(ctx.lib.nbh_memcpy_h2d(ctx.h, di.ptr, raw.ctypes.data_as(C.c_void_p), raw.nbytes))
+    da = ctx.alloc(3 * len(idx))
+    native.check(ctx.lib.nbh_direct_sample(ctx.h, dy.ptr, dm.ptr, n, di.ptr, len(idx), da.ptr), "sample")
+    return da.read().reshape(-1, 3)
+
+
+def test_direct_sample_bit_exact(ctx):
+    n = 5000
+    y, m = _state("plummer", n, 12)
+    dy, dm = ctx.put(y), ctx.put(m)
+    idx = np.random.default_rng(0).choice(n, 777, replace=False)
+    got = _sample_accel(ctx, dy, dm, n, idx)
+    assert np.array_equal(got, oracle_lib.accel_targets(y, m, idx))
+
+
+def _rel(a, a0):
+    return np.linalg.norm(a - a0, axis=1) / np.linalg.norm(a0, axis=1)
+
+
+@pytest.mark.parametrize("ratio", [1.25, 2.0, 4.0])
+def test_group_walk_at_least_as_accurate(ctx, ratio):
+    """The group walk takes a refinement of each body's cells: its error vs the
+    FP64 direct sum is no larger than the per-body walk's (up to FP32 noise),
+    at the cost of more interactions."""
+    n = 1 << 15
+    y, m = _state("plummer", n, 14)
+    dy, dm = ctx.put(y), ctx.put(m)
+    t = GpuTree(ctx, n)
+    t.build(dy, dm, ratio)
+    f_body = t.fcompute(dy, dm, 2, 0)
+    inter_body, _ = t.counters()
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group(ctx.h, t.h, dy.ptr, df.ptr), "group")
+    f_group = df.read()
+    inter_group, visits = t.counters()
+    idx = np.random.default_rng(1).choice(n, 2048, replace=False)
+    a0 = _sample_accel(ctx, dy, dm, n, idx)
+    e_body = _rel(f_body[3 * n:].reshape(3, n).T[idx], a0)
+    e_group = _rel(f_group[3 * n:].reshape(3, n).T[idx], a0)
+    rms = lambda e: float(np.sqrt(np.mean(e ** 2)))
+    assert rms(e_group) <= rms(e_body) * 1.01 + 2e-6, (rms(e_group), rms(e_body))
+    assert np.percentile(e_group, 99) <= np.percentile(e_body, 99) * 1.01 + 2e-6
+    assert inter_group >= inter_body
+    assert np.array_equal(f_group[:3 * n], y[3 * n:])
+    t.close()
+
+
+def test_group_walk_ratio_1e8_is_direct(ctx):
+    n = 4096
+    y, m = _state("ball", n, 15)
+    dy, dm = ctx.put(y), ctx.put(m)
+    t = GpuTree(ctx, n)
+    t.build(dy, dm, 1e8)
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group(ctx.h, t.h, dy.ptr, df.ptr), "group")
+    inter, _ = t.counters()
+    assert inter == n * n                 # every leaf, own group included (self term is 0)
+    a = df.read()[3 * n:].reshape(3, n).T
+    a0 = oracle_lib.fcompute_simple(y, m)[3 * n:].reshape(3, n).T
+    e = _rel(a, a0)
+    assert np.sqrt(np.mean(e ** 2)) < 1e-5 and e.max() < 1e-3
+    t.close()
+
+
 @pytest.mark.parametrize("rate", [0, 2])
 @pytest.mark.parametrize("layout", ["heap", "heap_stackless"])
 @pytest.mark.parametrize("ratio", [3.1623, 1e8])
