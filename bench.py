@@ -61,6 +61,78 @@ def cpu_baseline(y, m, n, budget_s: float, threads: int) -> dict:
                       f"{t:.1f} s on {threads} host threads"}
 
 
+def bh_bench(n: int, steps: int, warmup: int, ratios, samples: int = 8192) -> dict:
+    """Barnes-Hut euler steps/s (one fcompute = tree build + group walk, plus one
+    fmadd_inplace, as the reference bench's step) on a Plummer sphere of n bodies,
+    with the relative force error on `samples` random bodies against FP64 direct sums."""
+    import numpy as np
+
+    from nbody_amd import native, nbody_create_engine, nbody_create_solver, nbody_data
+    hip = native.hip_lib()
+    data = nbody_data()
+    data.set_quiet(True)
+    log(f"BH: generating Plummer N={n}")
+    data.generate("plummer", n, 4)
+    _, _, mass, _ = data.bodies()
+    out = {"workload": f"Barnes-Hut heap kd-tree, group walk, FP32 forces / FP64 state, Plummer N={n} "
+                       f"(a=1, M=1, seed 4), euler dt=1e-4, tree rebuilt every step", "n": n, "runs": []}
+    idx = np.sort(np.random.default_rng(0).choice(n, samples, replace=False)).astype(np.uint32)
+    for ratio in ratios:
+        e = nbody_create_engine({"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
+                                 "distance_to_node_radius_ratio": ratio, "tree_layout": "heap_stackless"})
+        data.set_time(0.0)
+        data.set_step(0)
+        if e is None or not e.init(data):
+            raise RuntimeError("hip_bh engine init failed")
+        ctx = e.hip_context()
+        s = nbody_create_solver({"solver": "euler", "max_step": 1e-4})
+        s.set_engine(e)
+        for _ in range(warmup):
+            s.advise(1e-4)
+        native.check(hip.nbh_timing_enable(ctx, 1))
+        native.check(hip.nbh_timing_reset(ctx))
+        native.check(hip.nbh_ctx_synchronize(ctx))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.advise(1e-4)
+        native.check(hip.nbh_ctx_synchronize(ctx))
+        wall = time.perf_counter() - t0
+        tb, cb, tw, cw = C.c_double(), C.c_uint64(), C.c_double(), C.c_uint64()
+        native.check(hip.nbh_timing_read(ctx, b"tree_build", C.byref(tb), C.byref(cb)))
+        native.check(hip.nbh_timing_read(ctx, b"bh_traverse", C.byref(tw), C.byref(cw)))
+        native.check(hip.nbh_timing_enable(ctx, 0))
+        inter = C.c_uint64()
+        native.check(hip.nbh_bh_counters(ctx, e.hip_tree(), C.byref(inter), None))
+        # error of the last force evaluation's state: recompute f, compare sampled bodies
+        y = e.get_y()
+        f = e.create_buffer(48 * n)
+        e.fcompute(0.0, y, f)
+        fa = e.read_buffer(f)[3 * n:].reshape(3, n).T[idx.astype(np.int64)]
+        dm = e.create_buffer(8 * n)
+        e.write_buffer(dm, mass)
+        di = e.create_buffer(8 * ((samples + 1) // 2))
+        hbuf = np.zeros(2 * ((samples + 1) // 2), dtype=np.uint32)
+        hbuf[:samples] = idx
+        e.write_buffer(di, hbuf.view(np.float64))
+        da = e.create_buffer(24 * samples)
+        native.check(hip.nbh_direct_sample(ctx, y.device_ptr(), dm.device_ptr(), n, di.device_ptr(), samples,
+                                           da.device_ptr()))
+        a0 = e.read_buffer(da).reshape(-1, 3)
+        rel = np.linalg.norm(fa - a0, axis=1) / np.linalg.norm(a0, axis=1)
+        e.free_buffers([f, dm, di, da])
+        out["runs"].append({
+            "ratio": ratio, "theta": 1.0 / ratio, "steps_per_s": steps / wall, "ms_per_step": wall / steps * 1e3,
+            "tree_build_ms": tb.value / max(1, cb.value), "walk_ms": tw.value / max(1, cw.value),
+            "interactions_per_body": inter.value / n,
+            "force_rel_err": {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
+                              "max": float(rel.max()), "samples": samples},
+        })
+        log(f"BH ratio {ratio}: {steps / wall:.2f} steps/s, rms err {out['runs'][-1]['force_rel_err']['rms']:.2e}")
+        s.destroy()
+        e.destroy()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +143,9 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bh", action="store_true", help="skip the Barnes-Hut steps/s block")
+    ap.add_argument("--bh-n", type=int, default=1 << 22)
+    ap.add_argument("--bh-steps", type=int, default=5)
     a = ap.parse_args(argv)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -188,13 +263,15 @@ def main(argv=None) -> int:
         "force_error": {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "max_rel": float(rel.max()),
                         "vs": "FP64 direct kernel (bit-exact with the CPU oracle)"},
     }
+    s.destroy()
+    e.destroy()
     if not a.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline ({threads} threads, ~{a.cpu_seconds:g} s)")
         out["cpu_baseline"] = cpu_baseline(yh, mass, n, a.cpu_seconds, threads)
+    if not a.no_bh:
+        out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, (2.0, 1.25))
     print(json.dumps(out), flush=True)
-    s.destroy()
-    e.destroy()
     return 0
 
 

@@ -44,6 +44,7 @@ typedef enum nbh_tree_layout {
 
 typedef struct nbh_ctx nbh_ctx;     /* one device + one stream */
 typedef struct nbh_tree nbh_tree;   /* device-resident heap kd-tree */
+typedef struct nbh_mlp nbh_mlp;     /* correction network (parameters + Adam state) */
 
 /* ---- context, errors ---------------------------------------------------- */
 const char* nbh_last_error(void);
@@ -146,6 +147,62 @@ nbh_status nbh_tree_get(nbh_ctx* ctx, const nbh_tree* tree, int32_t* body_n, dou
    nbh_bh_fcompute (summed over targets); either pointer may be NULL. */
 nbh_status nbh_bh_counters(nbh_ctx* ctx, const nbh_tree* tree, uint64_t* interactions,
                            uint64_t* visits);
+
+/* Counters of the last traversal: out[0] pair interactions, out[1] node visits,
+   out[2] nodes taken as monopoles because the walk stack was full, out[3] far
+   cells corrected by the network and out[4] near cells given their quadrupole
+   term (both of nbh_bh_fcompute_group_mlp). */
+nbh_status nbh_bh_walk_counters(nbh_ctx* ctx, const nbh_tree* tree, uint64_t out[5]);
+
+/* ---- learned correction of the group walk ---------------------------------
+   No counterpart in the reference.  The internal cells that the group walk
+   takes as monopoles are split by the distance dc from the group's box centre
+   p to their mass centre, in units of the box's half-diagonal h:
+     near: dc < near_k * h.  The cell's second central moments Q are applied to
+           every body of the group (quadrupole term, FP32).
+     far:  otherwise, when the cell's bounding radius r exceeds rho_min * dc.  A
+           16-64-64-9 tanh MLP (BF16 operands on the matrix cores, FP32
+           accumulate) maps 16 features of (p, cell: mass, box, Q) to the
+           residual R(p) = (exact field of the cell's bodies - monopole field) /
+           scale and its Jacobian dR/dp * dc / (4 scale), scale = M r^2 / dc^4; each
+           body of the group gets the sum over far cells of R(p) + J (x - p).
+   nbh_mlp_* keep an FP32 master copy of the 6288 parameters (row-major
+   W1[16][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16], output columns 9..15
+   unused) and its BF16 matrix-core packing. */
+nbh_status nbh_mlp_create(nbh_ctx* ctx, uint64_t seed, nbh_mlp** net);
+nbh_status nbh_mlp_destroy(nbh_mlp* net);
+nbh_status nbh_mlp_param_count(int* count);
+/* Host arrays of nbh_mlp_param_count floats; set also resets the Adam state. */
+nbh_status nbh_mlp_set_params(nbh_ctx* ctx, nbh_mlp* net, const float* params);
+nbh_status nbh_mlp_get_params(nbh_ctx* ctx, const nbh_mlp* net, float* params);
+/* Device x[count*16] -> device out[count*9] (asynchronous). */
+nbh_status nbh_mlp_forward(nbh_ctx* ctx, const nbh_mlp* net, const float* x, size_t count,
+                           float* out);
+/* Gradient of L = sum |out - t|^2 / count over device x[count*16], t[count*9]
+   into host grad[param_count]; *loss = L (may be NULL).  Synchronous. */
+nbh_status nbh_mlp_gradient(nbh_ctx* ctx, nbh_mlp* net, const float* x, const float* t,
+                            size_t count, float* grad, float* loss);
+/* Adam (beta 0.9/0.999, eps 1e-8) with a cosine learning-rate decay from lr to 0
+   over epochs * ceil(count/batch) steps; batch b of an epoch holds the samples
+   b, b + nb, b + 2nb, ...  epoch_loss[epochs] (may be NULL) receives the mean
+   loss of the last batch of each epoch.  Synchronous. */
+nbh_status nbh_mlp_train(nbh_ctx* ctx, nbh_mlp* net, const float* x, const float* t,
+                         size_t count, int epochs, size_t batch, float lr, float* epoch_loss);
+/* Training samples from a built tree: for the groups listed in the host array
+   groups[n_groups] (group g = the 64 bodies under heap node n/64 + g), every far
+   cell's features x[16] and exact target t[9] (FP64 sums over the cell's bodies,
+   rounded to FP32) into device arrays of `capacity` rows; cells (device, may be
+   NULL) receives the (group, heap node) pair of each row.  *count = number of
+   far cells found (rows beyond capacity are not written; their order depends
+   on the schedule, their set does not).  Synchronous. */
+nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* tree, const uint32_t* groups,
+                               size_t n_groups, float rho_min, float near_k, float* x,
+                               float* target, uint32_t* cells, size_t capacity, size_t* count);
+/* nbh_bh_fcompute_group plus the near-cell quadrupole terms and the network's
+   correction of the far cells.  near_k = 0 and an all-zero network give
+   nbh_bh_fcompute_group's result exactly. */
+nbh_status nbh_bh_fcompute_group_mlp(nbh_ctx* ctx, const nbh_tree* tree, const nbh_mlp* net,
+                                     const double* y, double* f, float rho_min, float near_k);
 
 /* ---- kernel timers -------------------------------------------------------
    When enabled, the main kernel of each entry point is bracketed by HIP events

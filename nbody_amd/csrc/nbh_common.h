@@ -65,6 +65,11 @@ public:
 	~scoped_timer();
 };
 
+// Device buffers of a correction network (defined in nbh_mlp.hip).
+const uint2* mlp_frags(const nbh_mlp* m);
+const float* mlp_params(const nbh_mlp* m);
+const nbh_ctx* mlp_ctx(const nbh_mlp* m);
+
 // Number of work-groups for a grid-stride kernel over `work` items with
 // `per_group` items per work-group pass: enough to fill every CU several times.
 inline unsigned grid_for(const nbh_ctx* ctx, size_t work, size_t per_group)

@@ -1,6 +1,7 @@
 // nbody_engine_hip.cpp -- see nbody_engine_hip.h.
 #include "nbody_engine_hip.h"
 
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -639,7 +640,10 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_tree_layout(tl),
 	m_tree(nullptr),
 	m_tree_built(false),
-	m_group_walk(false)
+	m_group_walk(false),
+	m_net_rho(0.0f),
+	m_net_near(0.0f),
+	m_net(nullptr)
 {
 }
 
@@ -649,6 +653,50 @@ nbody_engine_hip_bh::~nbody_engine_hip_bh()
 	{
 		nbh_tree_destroy(m_tree);
 	}
+	if(m_net != nullptr)
+	{
+		nbh_mlp_destroy(m_net);
+	}
+}
+
+// Correction file: the 8 bytes "NBHMLP01", int32 parameter count, float32
+// rho_min, float32 near_k, then the float32 parameters (little-endian), as
+// written by nbody_amd.correction.save.
+bool nbody_engine_hip_bh::load_correction(const std::string& path, double rho, double near_k)
+{
+	FILE* fp = std::fopen(path.c_str(), "rb");
+	if(fp == nullptr)
+	{
+		nbody_log("learned_correction: cannot open '%s'", path.c_str());
+		return false;
+	}
+	char magic[8] = {0};
+	int32_t count = 0;
+	float file_rho = 0.0f;
+	float file_near = 0.0f;
+	int expect = 0;
+	nbh_mlp_param_count(&expect);
+	bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, "NBHMLP01", 8) == 0 &&
+	          std::fread(&count, sizeof(count), 1, fp) == 1 && count == expect &&
+	          std::fread(&file_rho, sizeof(file_rho), 1, fp) == 1 && file_rho >= 0.0f &&
+	          std::fread(&file_near, sizeof(file_near), 1, fp) == 1 && file_near >= 0.0f;
+	std::vector<float> params;
+	if(ok)
+	{
+		params.resize(static_cast<size_t>(count));
+		ok = std::fread(params.data(), sizeof(float), params.size(), fp) == params.size() && std::fgetc(fp) == EOF;
+	}
+	std::fclose(fp);
+	if(!ok)
+	{
+		nbody_log("learned_correction: '%s' is not a correction network file", path.c_str());
+		return false;
+	}
+	m_net_params.swap(params);
+	m_net_rho = rho >= 0.0 ? static_cast<float>(rho) : file_rho;
+	m_net_near = near_k >= 0.0 ? static_cast<float>(near_k) : file_near;
+	m_net_path = path;
+	return true;
 }
 
 const char* nbody_engine_hip_bh::type_name() const
@@ -674,7 +722,19 @@ bool nbody_engine_hip_bh::init(nbody_data* data)
 		m_tree = nullptr;
 	}
 	m_tree_built = false;
-	return check(nbh_tree_create(m_ctx, n, &m_tree), "tree allocation");
+	if(!check(nbh_tree_create(m_ctx, n, &m_tree), "tree allocation"))
+	{
+		return false;
+	}
+	if(!m_net_params.empty() && m_net == nullptr)
+	{
+		if(!check(nbh_mlp_create(m_ctx, 0, &m_net), "correction network allocation") ||
+		   !check(nbh_mlp_set_params(m_ctx, m_net, m_net_params.data()), "correction network upload"))
+		{
+			return false;
+		}
+	}
+	return true;
 }
 
 // Build or refit, then traverse (nbody/nbody_engine_simple_bh.cpp:16-93,
@@ -711,6 +771,12 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 	}
 	if(m_group_walk && m_precision == NBH_FP32 && m_data->get_count() >= 64)
 	{
+		if(m_net != nullptr)
+		{
+			check(nbh_bh_fcompute_group_mlp(m_ctx, m_tree, m_net, y->data(), f->data(), m_net_rho, m_net_near),
+			      "bh fcompute (group walk, learned correction)");
+			return;
+		}
 		check(nbh_bh_fcompute_group(m_ctx, m_tree, y->data(), f->data()), "bh fcompute (group walk)");
 		return;
 	}
@@ -727,4 +793,10 @@ void nbody_engine_hip_bh::print_info() const
 	nbody_print("\ttree_layout: %s", tree_layout_name(m_tree_layout));
 	nbody_print("\ttree_build_rate: %zu", m_tree_build_rate);
 	nbody_print("\tbh_walk: %s", m_group_walk ? "group" : "body");
+	nbody_print("\tlearned_correction: %s", m_net_params.empty() ? "none" : m_net_path.c_str());
+	if(!m_net_params.empty())
+	{
+		nbody_print("\tcorrection_rho: %g", static_cast<double>(m_net_rho));
+		nbody_print("\tcorrection_near: %g", static_cast<double>(m_net_near));
+	}
 }

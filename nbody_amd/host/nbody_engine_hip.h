@@ -12,6 +12,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "nbody_engine.h"
 #include "nbody_hip.h"
@@ -115,11 +116,22 @@ class nbody_engine_hip_bh : public nbody_engine_hip
 	nbh_tree*       m_tree;
 	bool            m_tree_built;
 	bool            m_group_walk;
+	std::vector<float> m_net_params;   // learned correction (empty: none)
+	float           m_net_rho;
+	float           m_net_near;
+	std::string     m_net_path;
+	nbh_mlp*        m_net;
 public:
 	nbody_engine_hip_bh(nbcoord_t distance_to_node_radius_ratio, e_traverse_type tt,
 	                    size_t tree_build_rate, e_tree_layout tl);
 	//! Use the shared 64-body group walk (FP32 only) instead of one walk per body.
 	void set_group_walk(bool g) { m_group_walk = g; }
+	//! Read a correction network file ("NBHMLP01", int32 parameter count,
+	//! float32 rho_min, float32 near_k, float32 parameters); rho >= 0 and
+	//! near_k >= 0 override the thresholds stored in it.  False if the file is
+	//! unreadable or malformed.
+	bool load_correction(const std::string& path, double rho, double near_k);
+	const nbh_mlp* correction() const { return m_net; }
 	~nbody_engine_hip_bh() override;
 	const char* type_name() const override;
 	bool init(nbody_data* data) override;

@@ -109,8 +109,32 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("bh_walk=group needs force_precision=fp32");
 			return nullptr;
 		}
+		// learned correction of the group walk (no reference counterpart)
+		std::string net = param.value("learned_correction", "none");
+		double rho = -1.0;
+		if(!param.get_double("correction_rho", -1.0, &rho) || !(rho == -1.0 || rho >= 0.0))
+		{
+			nbody_log("Invalid correction_rho");
+			return nullptr;
+		}
+		double near_k = -1.0;
+		if(!param.get_double("correction_near", -1.0, &near_k) || !(near_k == -1.0 || near_k >= 0.0))
+		{
+			nbody_log("Invalid correction_near");
+			return nullptr;
+		}
+		if(net != "none" && walk != "group")
+		{
+			nbody_log("learned_correction needs bh_walk=group");
+			return nullptr;
+		}
 		nbody_engine_hip_bh* engine = new nbody_engine_hip_bh(ratio, tt, static_cast<size_t>(rate), tl);
 		engine->set_group_walk(walk == "group");
+		if(net != "none" && !engine->load_correction(net, rho, near_k))
+		{
+			delete engine;
+			return nullptr;
+		}
 		if(!common_hip_params(param, engine))
 		{
 			delete engine;

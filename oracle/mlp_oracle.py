@@ -1,0 +1,233 @@
+"""CPU oracle of the Barnes-Hut correction network and of its training data.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ (and nowhere in the product path).
+The network has no counterpart in the reference; what it approximates does:
+the field of a tree cell's bodies (force law of nbody/nbody_engine_simple.cpp:
+26-55, r^2 clamped at 1e-8) minus the monopole the heap walk uses in its place
+(nbody/nbody_space_heap.cpp:94-133, nbody_space_heap_stackless.cpp:3-26).
+
+Contents
+  * unpack / forward / loss_grad: the 16-64-64-9 tanh MLP on a flat parameter
+    vector (W1[16][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16], row-major),
+    in float64, or emulating the device arithmetic (every matrix-core operand
+    rounded to BF16 with round-to-nearest-even, FP32 elsewhere);
+  * cell_features: the 16 input features of (evaluation point, cell);
+  * cell_residual: exact residual field and Jacobian of a cell by brute force;
+  * group_band: the near and far cells of one 64-body group, by a plain stack
+    walk of the heap tree with the group walk's acceptance rule;
+  * quadrupole_field: the textbook quadrupole term of a cell's field.
+
+Pinning (tests/test_oracle_mlp.py)
+  * loss_grad (float64): central finite differences of the loss, every
+    parameter block;  bf16 emulation with rounding disabled equals float64;
+  * bf16 rounding: exhaustive on special patterns, ties to even, vs float32;
+  * cell_residual: Jacobian vs finite differences of the field; at small
+    size/distance the residual tends to the textbook quadrupole field;
+  * cell_features: invariants (unit direction, rotation covariance of q and
+    q.u, rho = radius / distance) rather than the formulas restated;
+  * group_band: a cell set property (every body of the group is covered once by
+    accepted cells and opened cells' descendants), checked on small trees;
+  * quadrupole_field: limit of the exact residual of a small distant cell.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+K_IN, K_HID, K_OUTP, K_OUT = 16, 64, 16, 9
+OFF_W1 = 0
+OFF_B1 = OFF_W1 + K_IN * K_HID
+OFF_W2 = OFF_B1 + K_HID
+OFF_B2 = OFF_W2 + K_HID * K_HID
+OFF_W3 = OFF_B2 + K_HID
+OFF_B3 = OFF_W3 + K_HID * K_OUTP
+N_PARAMS = OFF_B3 + K_OUTP  # 6288
+K_JAC = 0.25                # Jacobian outputs are K_JAC * J * d / scale
+
+
+def bf16(a) -> np.ndarray:
+    """Round float32 values to the nearest BF16 (ties to even), as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def unpack(p):
+    p = np.asarray(p)
+    assert p.size == N_PARAMS
+    W1 = p[OFF_W1:OFF_B1].reshape(K_IN, K_HID)
+    b1 = p[OFF_B1:OFF_W2]
+    W2 = p[OFF_W2:OFF_B2].reshape(K_HID, K_HID)
+    b2 = p[OFF_B2:OFF_W3]
+    W3 = p[OFF_W3:OFF_B3].reshape(K_HID, K_OUTP)
+    b3 = p[OFF_B3:]
+    return W1, b1, W2, b2, W3, b3
+
+
+def glorot(seed: int) -> np.ndarray:
+    """Glorot-uniform weights and zero biases (output columns >= 9 zero)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(N_PARAMS, np.float32)
+    for off, fi, fo, used in ((OFF_W1, K_IN, K_HID, K_HID), (OFF_W2, K_HID, K_HID, K_HID),
+                              (OFF_W3, K_HID, K_OUTP, K_OUT)):
+        a = np.sqrt(6.0 / (fi + used))
+        w = rng.uniform(-a, a, size=(fi, fo))
+        w[:, used:] = 0.0
+        p[off:off + fi * fo] = w.ravel()
+    return p
+
+
+def _ops(mode: str):
+    if mode == "f64":
+        return np.float64, (lambda a: np.asarray(a, np.float64))
+    if mode == "bf16":
+        return np.float32, bf16
+    if mode == "f32":   # bf16 path with rounding disabled
+        return np.float32, (lambda a: np.asarray(a, np.float32))
+    raise ValueError(mode)
+
+
+def forward(p, x, mode: str = "f64"):
+    """Outputs [m][9] (and the activations) of the network on features x[m][16]."""
+    dt, rnd = _ops(mode)
+    W1, b1, W2, b2, W3, b3 = (np.asarray(v, dt) for v in unpack(p))
+    x = np.asarray(x, dt)
+    xb = rnd(x)
+    h1 = np.tanh(rnd(xb) @ rnd(W1) + b1).astype(dt)
+    h2 = np.tanh(rnd(h1) @ rnd(W2) + b2).astype(dt)
+    out = (rnd(h2) @ rnd(W3) + b3).astype(dt)
+    return out[:, :K_OUT], (xb, h1, h2, out)
+
+
+def loss_grad(p, x, t, mode: str = "f64"):
+    """L = sum |out - t|^2 / m and dL/dp (flat), matching nbh_mlp_gradient."""
+    dt, rnd = _ops(mode)
+    W1, b1, W2, b2, W3, b3 = (np.asarray(v, dt) for v in unpack(p))
+    m = x.shape[0]
+    _, (xb, h1, h2, out) = forward(p, x, mode)
+    t = np.asarray(t, dt)
+    diff = np.zeros_like(out)
+    diff[:, :K_OUT] = out[:, :K_OUT] - t
+    loss = float(np.sum(diff.astype(np.float64) ** 2) / m)
+    dout = (2.0 * diff / m).astype(dt)
+    gW3 = rnd(h2).T @ rnd(dout)
+    gb3 = dout.sum(0)
+    dz2 = (rnd(dout) @ rnd(W3).T) * (1 - h2 * h2)
+    gW2 = rnd(h1).T @ rnd(dz2)
+    gb2 = dz2.sum(0)
+    dz1 = (rnd(dz2) @ rnd(W2).T) * (1 - h1 * h1)
+    gW1 = rnd(xb).T @ rnd(dz1)
+    gb1 = dz1.sum(0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3]).astype(np.float64)
+    return loss, g
+
+
+# ---------------------------------------------------------------------------
+# physics side
+# ---------------------------------------------------------------------------
+def cell_radius(bmin, bmax, cm) -> float:
+    """Bounding radius of a cell as in the opening test (nbody_space_heap.cpp:130-133)."""
+    bmin, bmax, cm = (np.asarray(v, np.float64) for v in (bmin, bmax, cm))
+    return 0.5 * np.linalg.norm(bmax - bmin) + np.linalg.norm(0.5 * (bmax + bmin) - cm)
+
+
+def cell_features(p, cm, M, bmin, bmax, Q):
+    """16 features and the target scale M r^2 / d^4 of (point p, cell)."""
+    p, cm, Q = (np.asarray(v, np.float64) for v in (p, cm, Q))
+    u = cm - p
+    d = np.linalg.norm(u)
+    r = cell_radius(bmin, bmax, cm)
+    q = np.array([[Q[0], Q[3], Q[4]], [Q[3], Q[1], Q[5]], [Q[4], Q[5], Q[2]]]) / (M * r * r)
+    h = u / d
+    w = q @ h
+    rho = r / d
+    f = np.array([h[0], h[1], h[2], rho, q[0, 0], q[1, 1], q[2, 2], q[0, 1], q[0, 2], q[1, 2],
+                  w[0], w[1], w[2], h @ w, rho * rho, np.trace(q)])
+    return f, M * rho * rho / (d * d), d
+
+
+def field(p, pos, mass):
+    """Field sum_j m_j (x_j - p) / |x_j - p|^3 (r^2 >= 1e-8) and its Jacobian."""
+    r = np.asarray(pos, np.float64) - np.asarray(p, np.float64)
+    r2 = np.maximum(np.einsum("ij,ij->i", r, r), 1e-8)
+    w = mass / (r2 * np.sqrt(r2))
+    a = (r * w[:, None]).sum(0)
+    J = (3.0 * (w / r2)[:, None, None] * r[:, :, None] * r[:, None, :]).sum(0) - np.eye(3) * w.sum()
+    return a, J
+
+
+def second_moments(pos, mass):
+    cm = (pos * mass[:, None]).sum(0) / mass.sum()
+    d = pos - cm
+    Q = np.einsum("i,ij,ik->jk", mass, d, d)
+    return cm, np.array([Q[0, 0], Q[1, 1], Q[2, 2], Q[0, 1], Q[0, 2], Q[1, 2]])
+
+
+def cell_residual(p, pos, mass):
+    """Exact field of the bodies minus their monopole at p, and its Jacobian."""
+    a, J = field(p, pos, mass)
+    M = mass.sum()
+    cm = (pos * mass[:, None]).sum(0) / M
+    a0, J0 = field(p, cm[None, :], np.array([M]))
+    return a - a0, J - J0
+
+
+def target_vector(p, pos, mass, bmin, bmax):
+    """Features x[16] and target t[9] of the cell holding bodies pos/mass."""
+    cm, Q = second_moments(pos, mass)
+    M = mass.sum()
+    f, scale, d = cell_features(p, cm, M, bmin, bmax, Q)
+    R, J = cell_residual(p, pos, mass)
+    Jv = np.array([J[0, 0], J[1, 1], J[2, 2], J[0, 1], J[0, 2], J[1, 2]])
+    return f, np.concatenate([R / scale, K_JAC * Jv * d / scale])
+
+
+def quadrupole_field(p, cm, Q):
+    """Textbook quadrupole term of the field of second moments Q about cm at p:
+    a = -grad Phi2, Phi2 = -(3 n.Q.n - tr Q) / (2 d^3), n = (p - cm)/d."""
+    s = np.asarray(p, np.float64) - np.asarray(cm, np.float64)
+    d = np.linalg.norm(s)
+    n = s / d
+    Qm = np.array([[Q[0], Q[3], Q[4]], [Q[3], Q[1], Q[5]], [Q[4], Q[5], Q[2]]])
+    qn = Qm @ n
+    nqn = n @ qn
+    tr = np.trace(Qm)
+    # grad of -(3 nQn - tr)/(2 d^3) with respect to p
+    grad = -(3.0 * (2.0 * qn - 2.0 * nqn * n) / d) / (2 * d ** 3) + 3.0 * (3 * nqn - tr) * n / (2 * d ** 4)
+    return -grad
+
+
+def group_band(n, group, cm, crit, bmin, bmax, rho_min, near_k=0.0):
+    """(far, taken, near) heap nodes of group `group` of a heap tree of n bodies.
+
+    cm[2n][3], crit[2n], bmin/bmax[2n][3] as returned by nbh_tree_get.  A node is
+    taken when the distance from the group box to its mass centre squared exceeds
+    crit (leaves always).  A taken internal node whose mass centre lies within
+    near_k half-diagonals of the box centre is near; otherwise it is far when its
+    bounding radius exceeds rho_min times that distance (thresholds rounded to
+    float32 as the kernels receive them)."""
+    g = n // 64 + group
+    lo, hi = bmin[g], bmax[g]
+    c = 0.5 * (lo + hi)
+    half = 0.5 * np.linalg.norm(hi - lo)
+    rho = float(np.float32(rho_min))
+    nk = float(np.float32(near_k))
+    far, taken, near = [], [], []
+    stack = [1]
+    while stack:
+        node = stack.pop()
+        if node >= n:
+            taken.append(node)
+            continue
+        e = np.maximum(np.maximum(lo - cm[node], cm[node] - hi), 0.0)
+        if e @ e > crit[node]:
+            taken.append(node)
+            u = cm[node] - c
+            dc = np.sqrt(u @ u)
+            if dc < nk * half:
+                near.append(node)
+            elif cell_radius(bmin[node], bmax[node], cm[node]) > rho * dc:
+                far.append(node)
+        else:
+            stack += [2 * node + 1, 2 * node]
+    return far, taken, near

@@ -39,6 +39,16 @@ class Ctx:
     def sync(self):
         native.check(self.lib.nbh_ctx_synchronize(self.h), "sync")
 
+    def put_raw(self, a: np.ndarray) -> "DevArray":
+        """Device copy of an array of any dtype (read back with read_raw)."""
+        a = np.ascontiguousarray(a)
+        b = self.alloc((a.nbytes + 7) // 8)
+        native.check(self.lib.nbh_memcpy_h2d(self.h, b.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes), "h2d")
+        return b
+
+    def zeros_raw(self, dtype, count: int) -> "DevArray":
+        return self.put_raw(np.zeros(count, dtype))
+
 
 class DevArray:
     def __init__(self, ctx: Ctx, n: int):
@@ -57,6 +67,13 @@ class DevArray:
     def read(self) -> np.ndarray:
         out = np.empty(self.n)
         native.check(self.ctx.lib.nbh_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, 8 * self.n),
+                     "d2h")
+        return out
+
+    def read_raw(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= 8 * self.n
+        native.check(self.ctx.lib.nbh_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes),
                      "d2h")
         return out
 

@@ -1,0 +1,405 @@
+"""The learned Barnes-Hut correction on the GPU against the CPU oracle
+(oracle/mlp_oracle.py): BF16 matrix-core forward and hand-written backward,
+training, the band-cell sampler, the corrected group walk, and the gain of the
+trained network over the plain walk at the same opening ratio."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from hip_capi import Ctx
+from nbody_amd import native, nbody_data
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Ctx(0)
+    yield c
+    c.close()
+
+
+class Net:
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        native.check(ctx.lib.nbh_mlp_create(ctx.h, seed, C.byref(self.h)), "mlp_create")
+
+    def close(self):
+        native.check(self.ctx.lib.nbh_mlp_destroy(self.h), "mlp_destroy")
+
+    def set(self, p):
+        p = np.ascontiguousarray(p, F32)
+        native.check(self.ctx.lib.nbh_mlp_set_params(self.ctx.h, self.h, p.ctypes.data_as(C.c_void_p)), "set")
+
+    def get(self):
+        p = np.empty(mo.N_PARAMS, F32)
+        native.check(self.ctx.lib.nbh_mlp_get_params(self.ctx.h, self.h, p.ctypes.data_as(C.c_void_p)), "get")
+        return p
+
+    def forward(self, x):
+        m = x.shape[0]
+        dx = self.ctx.put_raw(np.ascontiguousarray(x, F32))
+        do = self.ctx.zeros_raw(F32, m * mo.K_OUT)
+        native.check(self.ctx.lib.nbh_mlp_forward(self.ctx.h, self.h, dx.ptr, m, do.ptr), "forward")
+        self.ctx.sync()
+        return do.read_raw(F32, m * mo.K_OUT).reshape(m, mo.K_OUT)
+
+    def gradient(self, dx, dt, m):
+        g = np.empty(mo.N_PARAMS, F32)
+        loss = C.c_float()
+        native.check(self.ctx.lib.nbh_mlp_gradient(self.ctx.h, self.h, dx.ptr, dt.ptr, m,
+                                                   g.ctypes.data_as(C.c_void_p), C.byref(loss)), "gradient")
+        return g.astype(np.float64), loss.value
+
+    def train(self, dx, dt, m, epochs, batch, lr):
+        el = np.empty(epochs, F32)
+        native.check(self.ctx.lib.nbh_mlp_train(self.ctx.h, self.h, dx.ptr, dt.ptr, m, epochs, batch, lr,
+                                                el.ctypes.data_as(C.c_void_p)), "train")
+        return el
+
+
+def _random_params(seed, bias=0.3):
+    rng = np.random.default_rng(seed)
+    p = mo.glorot(seed)
+    for a, b in ((mo.OFF_B1, mo.OFF_W2), (mo.OFF_B2, mo.OFF_W3), (mo.OFF_B3, mo.OFF_B3 + mo.K_OUT)):
+        p[a:b] = rng.normal(0, bias, b - a)
+    return p.astype(F32)
+
+
+def test_param_count_and_roundtrip(ctx):
+    cnt = C.c_int()
+    native.check(ctx.lib.nbh_mlp_param_count(C.byref(cnt)))
+    assert cnt.value == mo.N_PARAMS
+    net = Net(ctx, 7)
+    p0 = net.get()
+    assert np.all(np.isfinite(p0)) and np.abs(p0[mo.OFF_W1:mo.OFF_B1]).max() > 0
+    W3 = p0[mo.OFF_W3:mo.OFF_B3].reshape(mo.K_HID, mo.K_OUTP)
+    assert np.all(W3[:, mo.K_OUT:] == 0)          # unused output columns start at zero
+    p = _random_params(3)
+    net.set(p)
+    assert np.array_equal(net.get(), p)
+    net.close()
+
+
+@pytest.mark.parametrize("m", [1, 16, 1000])
+def test_forward_matches_oracle(ctx, m):
+    rng = np.random.default_rng(m)
+    net = Net(ctx)
+    p = _random_params(11)
+    net.set(p)
+    x = rng.normal(size=(m, mo.K_IN)).astype(F32)
+    got = net.forward(x)
+    want, _ = mo.forward(p, x, "bf16")
+    exact, _ = mo.forward(p.astype(np.float64), x.astype(np.float64), "f64")
+    scale = np.abs(want).max()
+    # same BF16 operand rounding: differences come from FP32 summation order
+    # and tanh, and rare BF16 rounding flips of hidden activations
+    assert np.abs(got - want).max() <= 4e-3 * scale, np.abs(got - want).max() / scale
+    assert np.abs(got - want).mean() <= 2e-4 * scale
+    # BF16 operands vs float64: a few parts in a thousand
+    assert np.linalg.norm(got - exact) <= 3e-2 * np.linalg.norm(exact)
+    net.close()
+
+
+@pytest.mark.parametrize("m", [16, 333, 4096])
+def test_gradient_matches_oracle(ctx, m):
+    """Gradient of the hand-written backward kernels against the oracle's
+    (float64 and BF16-emulating) backpropagation, itself pinned by finite
+    differences."""
+    rng = np.random.default_rng(100 + m)
+    net = Net(ctx)
+    p = _random_params(12)
+    net.set(p)
+    x = rng.normal(size=(m, mo.K_IN)).astype(F32)
+    t = rng.normal(size=(m, mo.K_OUT)).astype(F32)
+    dx, dt = ctx.put_raw(x), ctx.put_raw(t)
+    g, loss = net.gradient(dx, dt, m)
+    lb, gb = mo.loss_grad(p, x, t, "bf16")
+    l64, g64 = mo.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), "f64")
+    assert abs(loss - lb) <= 2e-3 * lb
+    assert np.linalg.norm(g - gb) <= 1e-2 * np.linalg.norm(gb), np.linalg.norm(g - gb) / np.linalg.norm(gb)
+    assert np.linalg.norm(g - g64) <= 5e-2 * np.linalg.norm(g64)
+    assert np.dot(g, g64) >= 0.995 * np.linalg.norm(g) * np.linalg.norm(g64)
+    # per block as well (a transposed or misplaced block would fail here)
+    for a, b in ((mo.OFF_W1, mo.OFF_B1), (mo.OFF_B1, mo.OFF_W2), (mo.OFF_W2, mo.OFF_B2), (mo.OFF_B2, mo.OFF_W3),
+                 (mo.OFF_W3, mo.OFF_B3), (mo.OFF_B3, mo.N_PARAMS)):
+        assert np.linalg.norm(g[a:b] - gb[a:b]) <= 2e-2 * np.linalg.norm(gb[a:b]) + 1e-7, (a, b)
+    net.close()
+
+
+def test_training_reduces_loss(ctx):
+    rng = np.random.default_rng(5)
+    m = 16384
+    x = rng.normal(size=(m, mo.K_IN)).astype(F32)
+    A = rng.normal(size=(mo.K_IN, mo.K_OUT)) / 4
+    t = (np.tanh(x @ A) + 0.1 * np.sin(x[:, :mo.K_OUT] * 2)).astype(F32)
+    dx, dt = ctx.put_raw(x), ctx.put_raw(t)
+    net = Net(ctx, 3)
+    _, l0 = net.gradient(dx, dt, m)
+    el = net.train(dx, dt, m, 20, 512, 3e-3)
+    _, l1 = net.gradient(dx, dt, m)
+    assert np.all(np.isfinite(el))
+    assert l1 < 0.1 * l0, (l0, l1, el)
+    # the trained parameters mean the same thing to the oracle
+    pout, _ = mo.forward(net.get(), x[:2048], "bf16")
+    assert abs(np.mean((pout - t[:2048]) ** 2) * mo.K_OUT - l1) < 0.25 * l1 + 1e-4
+    net.close()
+
+
+# ---------------------------------------------------------------------------
+# tree side
+# ---------------------------------------------------------------------------
+class Tree:
+    def __init__(self, ctx, y, m, ratio):
+        self.ctx, self.n = ctx, len(m)
+        self.h = C.c_void_p()
+        native.check(ctx.lib.nbh_tree_create(ctx.h, self.n, C.byref(self.h)), "tree_create")
+        self.dy, self.dm = ctx.put(y), ctx.put(m)
+        native.check(ctx.lib.nbh_tree_build(ctx.h, self.h, self.dy.ptr, self.dm.ptr, ratio), "build")
+
+    def close(self):
+        self.ctx.lib.nbh_tree_destroy(self.h)
+
+    def arrays(self):
+        s = 2 * self.n
+        cm = np.empty(3 * s); mass = np.empty(s); crit = np.empty(s); bmin = np.empty(3 * s); bmax = np.empty(3 * s)
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        native.check(self.ctx.lib.nbh_tree_get(self.ctx.h, self.h, None, P(cm), P(mass), P(crit), P(bmin), P(bmax)))
+        return dict(cm=cm.reshape(-1, 3), mass=mass, crit=crit, bmin=bmin.reshape(-1, 3), bmax=bmax.reshape(-1, 3))
+
+    def samples(self, groups, rho, near_k, capacity):
+        g = np.ascontiguousarray(groups, np.uint32)
+        dx = self.ctx.zeros_raw(F32, max(1, capacity) * mo.K_IN)
+        dt = self.ctx.zeros_raw(F32, max(1, capacity) * mo.K_OUT)
+        dc = self.ctx.zeros_raw(np.uint32, max(1, capacity) * 2)
+        cnt = C.c_size_t()
+        native.check(self.ctx.lib.nbh_bh_band_samples(self.ctx.h, self.h, g.ctypes.data_as(C.c_void_p), len(g),
+                                                      rho, near_k, dx.ptr, dt.ptr, dc.ptr, capacity, C.byref(cnt)),
+                     "samples")
+        k = min(cnt.value, capacity)
+        return (cnt.value, dx, dt, dx.read_raw(F32, k * mo.K_IN).reshape(k, mo.K_IN),
+                dt.read_raw(F32, k * mo.K_OUT).reshape(k, mo.K_OUT), dc.read_raw(np.uint32, 2 * k).reshape(k, 2))
+
+    def group(self):
+        df = self.ctx.alloc(6 * self.n)
+        native.check(self.ctx.lib.nbh_bh_fcompute_group(self.ctx.h, self.h, self.dy.ptr, df.ptr), "group")
+        return df.read()
+
+    def group_mlp(self, net, rho, near_k):
+        df = self.ctx.alloc(6 * self.n)
+        native.check(self.ctx.lib.nbh_bh_fcompute_group_mlp(self.ctx.h, self.h, net.h, self.dy.ptr, df.ptr, rho,
+                                                            near_k), "group_mlp")
+        return df.read()
+
+    def counters(self):
+        c = (C.c_uint64 * 5)()
+        native.check(self.ctx.lib.nbh_bh_walk_counters(self.ctx.h, self.h, c))
+        return list(c)
+
+
+def _state(kind, n, seed):
+    d = nbody_data()
+    d.set_quiet(True)
+    d.generate(kind, n, seed)
+    return oracle_lib.state_from_data(d)
+
+
+def _leaf_slice(n, node):
+    depth = int(node).bit_length() - 1
+    span = int(np.log2(n)) - depth
+    return node << span, (node + 1) << span
+
+
+def test_band_samples_match_oracle(ctx):
+    n, ratio, rho, near_k = 4096, 1.25, 0.2, 2.0
+    y, m = _state("plummer", n, 21)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    groups = np.arange(n // 64)
+    total, _, _, x, tg, cells = t.samples(groups, rho, near_k, 1 << 20)
+    assert total == len(cells) > 0
+    want = set()
+    for g in groups:
+        far, _, _ = mo.group_band(n, int(g), a["cm"], a["crit"], a["bmin"], a["bmax"], rho, near_k)
+        want |= {(int(g), int(c)) for c in far}
+    got = {(int(g), int(c)) for g, c in cells}
+    assert got == want and len(got) == len(cells)
+    for k in range(len(cells)):
+        g, c = int(cells[k, 0]), int(cells[k, 1])
+        gn = n // 64 + g
+        p = 0.5 * (a["bmin"][gn] + a["bmax"][gn])
+        l0, l1 = _leaf_slice(n, c)
+        fx, ft = mo.target_vector(p, a["cm"][l0:l1], a["mass"][l0:l1], a["bmin"][c], a["bmax"][c])
+        assert np.allclose(x[k], fx, rtol=1e-5, atol=1e-6 * np.abs(fx).max()), (k, x[k], fx)
+        assert np.allclose(tg[k], ft, rtol=1e-4, atol=1e-4 * np.abs(ft).max()), (k, tg[k], ft)
+    # a smaller capacity: the same count, only `capacity` rows written
+    total2, _, _, x2, _, cells2 = t.samples(groups[:8], rho, near_k, 5)
+    assert len(cells2) == 5 and total2 == sum(1 for g, _ in want if g < 8)
+    assert {(int(g), int(c)) for g, c in cells2} <= want
+    t.close()
+
+
+def test_band_samples_reject_bad_group(ctx):
+    n = 1024
+    y, m = _state("ball", n, 2)
+    t = Tree(ctx, y, m, 1.25)
+    g = np.array([0, n // 64], np.uint32)
+    cnt = C.c_size_t()
+    st = ctx.lib.nbh_bh_band_samples(ctx.h, t.h, g.ctypes.data_as(C.c_void_p), 2, 0.2, 2.0, None, None, None, 0,
+                                     C.byref(cnt))
+    assert st != 0 and b"out of range" in ctx.lib.nbh_last_error()
+    t.close()
+
+
+def test_zero_network_without_near_cells_is_group_walk(ctx):
+    n = 1 << 14
+    y, m = _state("plummer", n, 22)
+    t = Tree(ctx, y, m, 1.25)
+    plain = t.group()
+    c_plain = t.counters()
+    net = Net(ctx)
+    net.set(np.zeros(mo.N_PARAMS, F32))
+    got = t.group_mlp(net, 0.2, 0.0)
+    c = t.counters()
+    assert np.array_equal(got, plain)
+    assert c[:3] == c_plain[:3] and c[3] > 0 and c[4] == 0
+    net.close()
+    t.close()
+
+
+def _body_order(ctx, t):
+    body = np.empty(2 * t.n, np.int32)
+    native.check(ctx.lib.nbh_tree_get(ctx.h, t.h, body.ctypes.data_as(C.c_void_p), None, None, None, None, None))
+    return body
+
+
+def _oracle_correction(n, a, body, p, rho, near_k, use_net=True):
+    """Per-body correction (original body order): quadrupole terms of the near
+    cells plus the network's expansion of the far cells about the group centre."""
+    want = np.zeros((n, 3))
+    for g in range(n // 64):
+        gn = n // 64 + g
+        cen = 0.5 * (a["bmin"][gn] + a["bmax"][gn])
+        far, _, near = mo.group_band(n, g, a["cm"], a["crit"], a["bmin"], a["bmax"], rho, near_k)
+        S = np.zeros(mo.K_OUT)
+        if far and use_net:
+            feats, w0, w1 = [], [], []
+            for c in far:
+                l0, l1 = _leaf_slice(n, c)
+                _, Q = mo.second_moments(a["cm"][l0:l1], a["mass"][l0:l1])
+                f, scale, d = mo.cell_features(cen, a["cm"][c], a["mass"][c], a["bmin"][c], a["bmax"][c], Q)
+                feats.append(f); w0.append(scale); w1.append(scale / (mo.K_JAC * d))
+            out, _ = mo.forward(p, np.array(feats, F32), "bf16")
+            S[:3] = (out[:, :3] * np.array(w0)[:, None]).sum(0)
+            S[3:] = (out[:, 3:] * np.array(w1)[:, None]).sum(0)
+        J = np.array([[S[3], S[6], S[7]], [S[6], S[4], S[8]], [S[7], S[8], S[5]]])
+        Qs = []
+        for c in near:
+            l0, l1 = _leaf_slice(n, c)
+            Qs.append(mo.second_moments(a["cm"][l0:l1], a["mass"][l0:l1])[1])
+        for leaf in range(n + 64 * g, n + 64 * g + 64):
+            x = a["cm"][leaf]
+            v = S[:3] + J @ (x - cen)
+            for c, Q in zip(near, Qs):
+                v = v + mo.quadrupole_field(x, a["cm"][c], Q)
+            want[body[leaf]] = v
+    return want
+
+
+def test_near_quadrupole_matches_oracle(ctx):
+    n, ratio, near_k = 4096, 1.25, 2.0
+    y, m = _state("plummer", n, 24)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    net = Net(ctx)
+    net.set(np.zeros(mo.N_PARAMS, F32))
+    plain = t.group()
+    got = t.group_mlp(net, 0.0, near_k)
+    assert t.counters()[4] > 0
+    delta = (got - plain)[3 * n:].reshape(3, n).T
+    want = _oracle_correction(n, a, _body_order(ctx, t), None, 0.0, near_k, use_net=False)
+    err = np.linalg.norm(delta - want, axis=1) / np.abs(want).max()
+    assert err.max() < 1e-3, err.max()
+    net.close()
+    t.close()
+
+
+def test_corrected_walk_matches_oracle(ctx):
+    """Correction added by the kernel == the oracle's near-cell quadrupoles plus
+    its network evaluated on the oracle's far cells, expanded to first order
+    about each group centre."""
+    n, ratio, rho, near_k = 4096, 1.25, 0.1, 2.0
+    y, m = _state("plummer", n, 23)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    net = Net(ctx)
+    p = _random_params(31, bias=0.1) * F32(0.5)
+    net.set(p)
+    plain = t.group()
+    got = t.group_mlp(net, rho, near_k)
+    c = t.counters()
+    assert c[3] > 0 and c[4] > 0
+    delta = (got - plain)[3 * n:].reshape(3, n).T
+    want = _oracle_correction(n, a, _body_order(ctx, t), p, rho, near_k)
+    scale = np.abs(want).max()
+    assert scale > 0
+    err = np.abs(delta - want).max() / scale
+    assert err < 2e-2, err
+    net.close()
+    t.close()
+
+
+def _sample_accel(ctx, dy, dm, n, idx):
+    di = ctx.put_raw(np.ascontiguousarray(idx, np.uint32))
+    da = ctx.alloc(3 * len(idx))
+    native.check(ctx.lib.nbh_direct_sample(ctx.h, dy.ptr, dm.ptr, n, di.ptr, len(idx), da.ptr), "sample")
+    return da.read().reshape(-1, 3)
+
+
+def _rms_rel(f, a0, n, idx):
+    a = f[3 * n:].reshape(3, n).T[idx]
+    e = np.linalg.norm(a - a0, axis=1) / np.linalg.norm(a0, axis=1)
+    return float(np.sqrt(np.mean(e ** 2))), float(np.percentile(e, 99))
+
+
+def test_trained_correction_gain(ctx, capsys):
+    """Train on the far cells of one Plummer sphere, apply to another: at the
+    same opening ratio, the trained network lowers the error of the walk with
+    the near-cell quadrupoles alone, and both beat the plain walk."""
+    n, ratio, rho, near_k = 1 << 16, 1.25, 0.0, 2.0
+    ytr, mtr = _state("plummer", n, 31)
+    ttr = Tree(ctx, ytr, mtr, ratio)
+    groups = np.random.default_rng(0).choice(n // 64, 512, replace=False)
+    cap = 1 << 21
+    total, dx, dt, _, _, _ = ttr.samples(groups, rho, near_k, cap)
+    k = min(total, cap)
+    net = Net(ctx, 5)
+    el = net.train(dx, dt, k, 30, 1024, 2e-3)
+    ttr.close()
+    y, m = _state("plummer", n, 32)
+    t = Tree(ctx, y, m, ratio)
+    idx = np.random.default_rng(3).choice(n, 4096, replace=False)
+    a0 = _sample_accel(ctx, t.dy, t.dm, n, idx)
+    e_plain = _rms_rel(t.group(), a0, n, idx)
+    zero = Net(ctx)
+    zero.set(np.zeros(mo.N_PARAMS, F32))
+    e_near = _rms_rel(t.group_mlp(zero, rho, near_k), a0, n, idx)
+    e_full = _rms_rel(t.group_mlp(net, rho, near_k), a0, n, idx)
+    t2 = Tree(ctx, y, m, 2.0)
+    e_plain2 = _rms_rel(t2.group(), a0, n, idx)
+    with capsys.disabled():
+        print(f"\n[correction gain] samples {total}  loss {el[0]:.3g}->{el[-1]:.3g}  rms/p99: "
+              f"plain 1.25 {e_plain[0]:.3e}/{e_plain[1]:.3e}  near-quad 1.25 {e_near[0]:.3e}/{e_near[1]:.3e}  "
+              f"near-quad+net 1.25 {e_full[0]:.3e}/{e_full[1]:.3e}  plain 2.0 {e_plain2[0]:.3e}/{e_plain2[1]:.3e}")
+    assert e_near[0] < e_plain[0]
+    assert e_full[0] < 0.85 * e_near[0]
+    net.close()
+    zero.close()
+    t.close()
+    t2.close()

@@ -1,0 +1,217 @@
+"""Pins of the CPU oracle of the correction network (oracle/mlp_oracle.py).
+
+The network and its training data have no reference counterpart; these checks
+fix them by mathematics: finite differences, the quadrupole limit of a cell's
+field, symmetry of the features, and a cover property of the band walk.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+import oracle_lib as ol
+
+
+def _random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = mo.glorot(seed).astype(np.float64)
+    # non-zero biases so that every bias path is exercised
+    for a, b in ((mo.OFF_B1, mo.OFF_W2), (mo.OFF_B2, mo.OFF_W3), (mo.OFF_B3, mo.N_PARAMS)):
+        p[a:b] = rng.normal(0, 0.3, b - a)
+    return p
+
+
+def test_bf16_rounding():
+    one = np.float32(1.0)
+    ulp = np.float32(2.0 ** -7)
+    cases = {
+        1.0: 1.0,
+        1.0 + 2.0 ** -8: 1.0,                       # tie -> even mantissa
+        1.0 + 3 * 2.0 ** -8: 1.0 + 2 * 2.0 ** -7,  # tie -> even mantissa
+        1.0 + 2.0 ** -8 + 2.0 ** -20: 1.0 + 2.0 ** -7,
+        -1.5: -1.5,
+        0.0: 0.0,
+    }
+    for x, want in cases.items():
+        assert mo.bf16(np.float32(x))[()] == np.float32(want), x
+    rng = np.random.default_rng(1)
+    x = (rng.normal(size=20000) * np.exp(rng.uniform(-20, 20, 20000))).astype(np.float32)
+    b = mo.bf16(x)
+    assert np.all(b.view(np.uint32) & 0xFFFF == 0)
+    # nearest of the two bracketing BF16 values
+    lo = (x.view(np.uint32) & 0xFFFF0000).view(np.float32)
+    hi = ((x.view(np.uint32) & 0xFFFF0000) + 0x10000).view(np.float32)
+    best = np.minimum(np.abs(x.astype(np.float64) - lo), np.abs(x.astype(np.float64) - hi))
+    assert np.all(np.abs(x.astype(np.float64) - b) == best)
+    assert one + ulp == np.float32(1.0078125)
+
+
+def test_loss_grad_finite_differences():
+    rng = np.random.default_rng(2)
+    p = _random_params(3)
+    m = 37                                    # not a multiple of the 16-row tile
+    x = rng.normal(size=(m, mo.K_IN))
+    t = rng.normal(size=(m, mo.K_OUT))
+    loss, g = mo.loss_grad(p, x, t, "f64")
+    blocks = [(mo.OFF_W1, mo.OFF_B1), (mo.OFF_B1, mo.OFF_W2), (mo.OFF_W2, mo.OFF_B2), (mo.OFF_B2, mo.OFF_W3),
+              (mo.OFF_W3, mo.OFF_B3), (mo.OFF_B3, mo.N_PARAMS)]
+    h = 1e-6
+    for a, b in blocks:
+        idx = rng.choice(np.arange(a, b), size=min(24, b - a), replace=False)
+        for i in idx:
+            pp = p.copy(); pp[i] += h
+            pm = p.copy(); pm[i] -= h
+            fd = (mo.loss_grad(pp, x, t)[0] - mo.loss_grad(pm, x, t)[0]) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+    # unused output columns (9..15) receive no gradient
+    W3g = g[mo.OFF_W3:mo.OFF_B3].reshape(mo.K_HID, mo.K_OUTP)
+    assert np.all(W3g[:, mo.K_OUT:] == 0) and np.all(g[mo.OFF_B3 + mo.K_OUT:] == 0)
+    assert loss > 0
+
+
+def test_rounding_modes_agree():
+    rng = np.random.default_rng(4)
+    p = _random_params(5)
+    x = rng.normal(size=(64, mo.K_IN))
+    t = rng.normal(size=(64, mo.K_OUT))
+    l64, g64 = mo.loss_grad(p, x, t, "f64")
+    l32, g32 = mo.loss_grad(p, x, t, "f32")
+    assert abs(l32 - l64) < 1e-5 * l64
+    assert np.linalg.norm(g32 - g64) < 1e-5 * np.linalg.norm(g64)
+    lb, gb = mo.loss_grad(p, x, t, "bf16")
+    assert np.linalg.norm(gb - g64) < 5e-2 * np.linalg.norm(g64)
+    assert np.dot(gb, g64) > 0.99 * np.linalg.norm(gb) * np.linalg.norm(g64)
+
+
+def _cluster(rng, k=200, scale=1.0):
+    pos = rng.normal(size=(k, 3)) * scale * np.array([1.0, 0.6, 0.3])
+    mass = rng.uniform(0.5, 1.5, k)
+    return pos, mass
+
+
+def test_field_jacobian_finite_differences():
+    rng = np.random.default_rng(6)
+    pos, mass = _cluster(rng)
+    p = np.array([4.0, -3.0, 2.5])
+    a, J = mo.field(p, pos, mass)
+    h = 1e-5
+    for j in range(3):
+        e = np.zeros(3); e[j] = h
+        fd = (mo.field(p + e, pos, mass)[0] - mo.field(p - e, pos, mass)[0]) / (2 * h)
+        assert np.allclose(fd, J[:, j], rtol=1e-6, atol=1e-9 * np.abs(J).max())
+    assert np.allclose(J, J.T)
+    # traceless away from the sources (Laplace)
+    assert abs(np.trace(J)) < 1e-12 * np.abs(J).max()
+
+
+def test_residual_tends_to_quadrupole():
+    rng = np.random.default_rng(7)
+    pos, mass = _cluster(rng)
+    cm, Q = mo.second_moments(pos, mass)
+    errs = []
+    for D in (20.0, 40.0, 80.0):
+        p = cm + D * np.array([0.48, -0.6, 0.64])
+        R, _ = mo.cell_residual(p, pos, mass)
+        quad = mo.quadrupole_field(p, cm, Q)
+        errs.append(np.linalg.norm(R - quad) / np.linalg.norm(quad))
+    # the remainder is the octupole: relative error ~ 1/D
+    assert errs[0] < 0.1
+    assert errs[1] < 0.6 * errs[0] and errs[2] < 0.6 * errs[1]
+
+
+def _rotation(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_features_rotation_covariance():
+    rng = np.random.default_rng(8)
+    pos, mass = _cluster(rng)
+    p = np.array([5.0, 1.0, -2.0])
+    Rm = _rotation(rng)
+
+    def feats(P, X):
+        cm, Q = mo.second_moments(X, mass)
+        return mo.cell_features(P, cm, mass.sum(), X.min(0), X.max(0), Q)
+
+    f0, s0, d0 = feats(p, pos)
+    assert abs(np.linalg.norm(f0[:3]) - 1) < 1e-12
+    cm = (pos * mass[:, None]).sum(0) / mass.sum()
+    r0 = mo.cell_radius(pos.min(0), pos.max(0), cm)
+    assert np.isclose(f0[3], r0 / np.linalg.norm(cm - p)) and np.isclose(f0[14], f0[3] ** 2)
+    # rotate bodies and point about the origin with the box kept equal in size:
+    # use a rotation that maps the box to itself (axis permutation with signs)
+    P = np.array([[0, 1, 0], [0, 0, -1], [-1, 0, 0]], float)
+    f1, s1, d1 = feats(P @ p, pos @ P.T)
+    assert np.allclose(f1[:3], P @ f0[:3])
+    q0 = np.array([[f0[4], f0[7], f0[8]], [f0[7], f0[5], f0[9]], [f0[8], f0[9], f0[6]]])
+    q1 = np.array([[f1[4], f1[7], f1[8]], [f1[7], f1[5], f1[9]], [f1[8], f1[9], f1[6]]])
+    assert np.allclose(q1, P @ q0 @ P.T)
+    assert np.allclose(f1[10:13], P @ f0[10:13])
+    assert np.allclose([f1[3], f1[13], f1[14], f1[15], s1, d1], [f0[3], f0[13], f0[14], f0[15], s0, d0])
+    # general rotation: the second-moment tensor transforms as a tensor
+    cmr, Qr = mo.second_moments(pos @ Rm.T, mass)
+    Qm = lambda v: np.array([[v[0], v[3], v[4]], [v[3], v[1], v[5]], [v[4], v[5], v[2]]])
+    _, Q0 = mo.second_moments(pos, mass)
+    assert np.allclose(Qm(Qr), Rm @ Qm(Q0) @ Rm.T)
+
+
+def _plummer_like(n, seed):
+    rng = np.random.default_rng(seed)
+    r = 1.0 / np.sqrt(rng.uniform(0.05, 1.0, n) ** (-2.0 / 3.0) - 1.0 + 1e-9)
+    v = rng.normal(size=(n, 3))
+    pos = v / np.linalg.norm(v, axis=1)[:, None] * r[:, None]
+    mass = rng.uniform(0.5, 1.5, n) / n
+    y = np.concatenate([pos[:, 0], pos[:, 1], pos[:, 2], np.zeros(3 * n)])
+    return y, mass
+
+
+@pytest.mark.parametrize("ratio", [1.25, 2.0])
+def test_group_band_covers_every_body_once(ratio):
+    n = 2048
+    y, mass = _plummer_like(n, 9)
+    h = ol.Heap(n)
+    h.build(y, mass, ratio)
+    a = h.arrays()
+    for group in (0, 7, n // 64 - 1):
+        far, taken, near = mo.group_band(n, group, a["cm"], a["crit_r2"], a["bmin"], a["bmax"], 0.2, 2.0)
+        # the leaf slices of the taken nodes partition [n, 2n)
+        cover = np.zeros(2 * n, int)
+        for node in taken:
+            depth = int(node).bit_length() - 1
+            span = int(np.log2(n)) - depth
+            cover[node << span:(node + 1) << span] += 1
+        assert np.all(cover[n:] == 1)
+        assert set(far) <= set(taken) and set(near) <= set(taken) and not set(far) & set(near)
+        assert all(b < n for b in far + near)
+        # near_k = 0: no near cells, the far set only grows
+        far0, taken0, near0 = mo.group_band(n, group, a["cm"], a["crit_r2"], a["bmin"], a["bmax"], 0.2, 0.0)
+        assert taken0 == taken and not near0 and set(far) <= set(far0)
+
+
+def test_quadrupole_sum_improves_monopole():
+    """Monopole + quadrupole of the taken cells is closer to the direct sum than
+    the monopole alone, for every body of a group (small tree, large ratio)."""
+    n = 1024
+    y, mass = _plummer_like(n, 10)
+    h = ol.Heap(n)
+    h.build(y, mass, 3.0)
+    a = h.arrays()
+    pos, ms = a["cm"][n:], a["mass"][n:]
+    far, taken, near = mo.group_band(n, 3, a["cm"], a["crit_r2"], a["bmin"], a["bmax"], 0.0, 1e9)
+    assert not far and near
+    worse = 0
+    for leaf in range(n + 3 * 64, n + 4 * 64):
+        x = a["cm"][leaf]
+        exact = mo.field(x, pos, ms)[0]
+        mono = sum(mo.field(x, a["cm"][c][None], np.array([a["mass"][c]]))[0] for c in taken)
+        quad = mono.copy()
+        for c in near:
+            depth = int(c).bit_length() - 1
+            s = int(np.log2(n)) - depth
+            l0, l1 = (c << s) - n, ((c + 1) << s) - n
+            _, Q = mo.second_moments(pos[l0:l1], ms[l0:l1])
+            quad += mo.quadrupole_field(x, a["cm"][c], Q)
+        worse += np.linalg.norm(quad - exact) > np.linalg.norm(mono - exact)
+    assert worse <= 3
