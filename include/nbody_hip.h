@@ -162,10 +162,17 @@ nbh_status nbh_bh_walk_counters(nbh_ctx* ctx, const nbh_tree* tree, uint64_t out
            every body of the group (quadrupole term, FP32).
      far:  otherwise, when the cell's bounding radius r exceeds rho_min * dc.  A
            16-64-64-9 tanh MLP (BF16 operands on the matrix cores, FP32
-           accumulate) maps 16 features of (p, cell: mass, box, Q) to the
-           residual R(p) = (exact field of the cell's bodies - monopole field) /
-           scale and its Jacobian dR/dp * dc / (4 scale), scale = M r^2 / dc^4; each
-           body of the group gets the sum over far cells of R(p) + J (x - p).
+           accumulate) maps 16 rotation invariants of (p, cell: mass, box,
+           Q) to 9 coefficients of a basis built from the direction uh to the
+           cell, w = q uh (q = Q / (M r^2)), the cell's box-centre offset o and
+           q (sample layout of basis[16]: uh, w, o, q as xx yy zz xy xz yz, 0):
+             R / scale = c0 uh + c1 w + c2 o,
+             J dc / (4 scale) = c3 q + c4 (uh w' + w uh') + c5 I + c6 uh uh'
+                                + c7 (uh o' + o uh') + c8 o o',
+           R(p) = exact field of the cell's bodies - monopole field, J = dR/dp,
+           scale = M r^2 / dc^4 (target layout t[9]: R (3), J as xx yy zz xy
+           xz yz).  Each body of the group gets the sum over far cells of
+           R(p) + J (x - p).
    nbh_mlp_* keep an FP32 master copy of the 6288 parameters (row-major
    W1[16][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16], output columns 9..15
    unused) and its BF16 matrix-core packing. */
@@ -175,29 +182,32 @@ nbh_status nbh_mlp_param_count(int* count);
 /* Host arrays of nbh_mlp_param_count floats; set also resets the Adam state. */
 nbh_status nbh_mlp_set_params(nbh_ctx* ctx, nbh_mlp* net, const float* params);
 nbh_status nbh_mlp_get_params(nbh_ctx* ctx, const nbh_mlp* net, float* params);
-/* Device x[count*16] -> device out[count*9] (asynchronous). */
+/* Device x[count*16] -> device out[count*9], the basis coefficients (asynchronous). */
 nbh_status nbh_mlp_forward(nbh_ctx* ctx, const nbh_mlp* net, const float* x, size_t count,
                            float* out);
-/* Gradient of L = sum |out - t|^2 / count over device x[count*16], t[count*9]
-   into host grad[param_count]; *loss = L (may be NULL).  Synchronous. */
-nbh_status nbh_mlp_gradient(nbh_ctx* ctx, nbh_mlp* net, const float* x, const float* t,
-                            size_t count, float* grad, float* loss);
+/* Gradient of L = sum |B(basis) out - t|^2 / count over device x[count*16],
+   basis[count*16], t[count*9] into host grad[param_count]; *loss = L (may be
+   NULL).  Synchronous. */
+nbh_status nbh_mlp_gradient(nbh_ctx* ctx, nbh_mlp* net, const float* x, const float* basis,
+                            const float* t, size_t count, float* grad, float* loss);
 /* Adam (beta 0.9/0.999, eps 1e-8) with a cosine learning-rate decay from lr to 0
    over epochs * ceil(count/batch) steps; batch b of an epoch holds the samples
    b, b + nb, b + 2nb, ...  epoch_loss[epochs] (may be NULL) receives the mean
    loss of the last batch of each epoch.  Synchronous. */
-nbh_status nbh_mlp_train(nbh_ctx* ctx, nbh_mlp* net, const float* x, const float* t,
-                         size_t count, int epochs, size_t batch, float lr, float* epoch_loss);
+nbh_status nbh_mlp_train(nbh_ctx* ctx, nbh_mlp* net, const float* x, const float* basis,
+                         const float* t, size_t count, int epochs, size_t batch, float lr,
+                         float* epoch_loss);
 /* Training samples from a built tree: for the groups listed in the host array
    groups[n_groups] (group g = the 64 bodies under heap node n/64 + g), every far
-   cell's features x[16] and exact target t[9] (FP64 sums over the cell's bodies,
-   rounded to FP32) into device arrays of `capacity` rows; cells (device, may be
+   cell's features x[16], basis[16] and exact target t[9] (FP64 sums over the
+   cell's bodies, rounded to FP32) into device arrays of `capacity` rows; cells (device, may be
    NULL) receives the (group, heap node) pair of each row.  *count = number of
    far cells found (rows beyond capacity are not written; their order depends
    on the schedule, their set does not).  Synchronous. */
 nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* tree, const uint32_t* groups,
                                size_t n_groups, float rho_min, float near_k, float* x,
-                               float* target, uint32_t* cells, size_t capacity, size_t* count);
+                               float* basis, float* target, uint32_t* cells, size_t capacity,
+                               size_t* count);
 /* nbh_bh_fcompute_group plus the near-cell quadrupole terms and the network's
    correction of the far cells.  near_k = 0 and an all-zero network give
    nbh_bh_fcompute_group's result exactly. */

@@ -108,8 +108,9 @@ k_mlp_forward(const float* __restrict__ x, size_t m, const uint2* __restrict__ f
 // One wave (block) per partial: tiles w, w + nw, ... of the batch whose
 // samples are base + stride * s, s < count.
 __global__ void __launch_bounds__(64)
-k_mlp_grad(const float* __restrict__ x, const float* __restrict__ t, size_t base, size_t stride, size_t count,
-           float inv_batch, const uint2* __restrict__ frags, const float* __restrict__ p, float* __restrict__ partial)
+k_mlp_grad(const float* __restrict__ x, const float* __restrict__ bs, const float* __restrict__ t, size_t base,
+           size_t stride, size_t count, float inv_batch, const uint2* __restrict__ frags, const float* __restrict__ p,
+           float* __restrict__ partial)
 {
 	__shared__ __align__(16) uint16_t lds[2 * 16 * kHid];
 	__shared__ __align__(16) uint16_t zt[16 * kHid];
@@ -143,15 +144,39 @@ k_mlp_grad(const float* __restrict__ x, const float* __restrict__ t, size_t base
 		}
 		f32x4 h1[4], h2[4];
 		const f32x4 out = forward_tile(pack4(xv[0], xv[1], xv[2], xv[3]), frags, p, lds, lane, h1, h2);
-		// dL/dout, D layout (rows = samples 4 grp + e, column = output col)
+		// dL/dout, D layout (rows = samples 4 grp + e, column = output col): the
+		// prediction of a row is the sum over its 16 lanes of c_col * basis_col
 		f32x4 dout;
 		for(int e = 0; e < 4; ++e)
 		{
 			const size_t r = s0 + 4 * grp + e;
-			const bool ok = r < count && col < kOut;
-			const float diff = ok ? out[e] - t[(base + stride * r) * kOut + col] : 0.0f;
-			loss += diff * diff;
-			dout[e] = 2.0f * diff * inv_batch;
+			const bool ok = r < count;             // uniform over the row's 16 lanes
+			float v[kOut];
+			if(ok)
+			{
+				float bv[kBasis];
+				const float* br = bs + (base + stride * r) * kBasis;
+				for(int k = 0; k < kBasis; ++k) { bv[k] = br[k]; }
+				basis_col(bv, col, v);
+			}
+			else
+			{
+				for(int k = 0; k < kOut; ++k) { v[k] = 0.0f; }
+			}
+			float P[kOut];
+			for(int k = 0; k < kOut; ++k) { P[k] = out[e] * v[k]; }
+			for(int sh = 1; sh < 16; sh <<= 1)
+			{
+				for(int k = 0; k < kOut; ++k) { P[k] += __shfl_xor(P[k], sh, 64); }
+			}
+			float gsum = 0.0f;
+			for(int k = 0; k < kOut; ++k)
+			{
+				const float diff = ok ? P[k] - t[(base + stride * r) * kOut + k] : 0.0f;
+				gsum += diff * v[k];
+				if(col == 0) { loss += diff * diff; }
+			}
+			dout[e] = 2.0f * gsum * inv_batch;
 			gb3 += dout[e];
 		}
 		const bf16x4 bdout = pack4(dout);
@@ -379,14 +404,14 @@ nbh_status nbh_mlp_forward(nbh_ctx* ctx, const nbh_mlp* m, const float* x, size_
 	return NBH_OK;
 }
 
-nbh_status nbh_mlp_gradient(nbh_ctx* ctx, nbh_mlp* m, const float* x, const float* t, size_t count,
-                            float* grad, float* loss)
+nbh_status nbh_mlp_gradient(nbh_ctx* ctx, nbh_mlp* m, const float* x, const float* basis, const float* t,
+                            size_t count, float* grad, float* loss)
 {
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(m != nullptr && m->ctx == ctx, "network does not belong to this context");
-	NBH_CHECK(count > 0 && x != nullptr && t != nullptr && grad != nullptr, "bad arguments");
+	NBH_CHECK(count > 0 && x != nullptr && basis != nullptr && t != nullptr && grad != nullptr, "bad arguments");
 	const int nw = static_cast<int>(std::min<size_t>(kGradWaves, (count + 15) / 16));
-	hipLaunchKernelGGL(k_mlp_grad, dim3(nw), dim3(64), 0, ctx->stream, x, t, static_cast<size_t>(0),
+	hipLaunchKernelGGL(k_mlp_grad, dim3(nw), dim3(64), 0, ctx->stream, x, basis, t, static_cast<size_t>(0),
 	                   static_cast<size_t>(1), count, 1.0f / static_cast<float>(count), m->frags, m->params,
 	                   m->partial);
 	NBH_LAUNCHED();
@@ -401,12 +426,13 @@ nbh_status nbh_mlp_gradient(nbh_ctx* ctx, nbh_mlp* m, const float* x, const floa
 	return NBH_OK;
 }
 
-nbh_status nbh_mlp_train(nbh_ctx* ctx, nbh_mlp* m, const float* x, const float* t, size_t count, int epochs,
-                         size_t batch, float lr, float* epoch_loss)
+nbh_status nbh_mlp_train(nbh_ctx* ctx, nbh_mlp* m, const float* x, const float* basis, const float* t, size_t count,
+                         int epochs, size_t batch, float lr, float* epoch_loss)
 {
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(m != nullptr && m->ctx == ctx, "network does not belong to this context");
-	NBH_CHECK(count > 0 && x != nullptr && t != nullptr && epochs > 0 && batch >= 16, "bad arguments");
+	NBH_CHECK(count > 0 && x != nullptr && basis != nullptr && t != nullptr && epochs > 0 && batch >= 16,
+	          "bad arguments");
 	const size_t nb = (count + batch - 1) / batch;
 	const long total = static_cast<long>(nb) * epochs;
 	for(int ep = 0; ep < epochs; ++ep)
@@ -422,7 +448,7 @@ nbh_status nbh_mlp_train(nbh_ctx* ctx, nbh_mlp* m, const float* x, const float* 
 			const float lr_t = lr * 0.5f * (1.0f + std::cos(3.14159265f * std::min(frac, 1.0f)));
 			const float b1t = std::pow(0.9f, static_cast<float>(m->step));
 			const float b2t = std::pow(0.999f, static_cast<float>(m->step));
-			hipLaunchKernelGGL(k_mlp_grad, dim3(nw), dim3(64), 0, ctx->stream, x, t, b, nb, cnt,
+			hipLaunchKernelGGL(k_mlp_grad, dim3(nw), dim3(64), 0, ctx->stream, x, basis, t, b, nb, cnt,
 			                   1.0f / static_cast<float>(cnt), m->frags, m->params, m->partial);
 			NBH_LAUNCHED();
 			hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, ctx->stream, m->partial, nw, m->grad, m->params,

@@ -21,7 +21,7 @@ namespace nbh_net {
 constexpr int kIn = 16;      // input features (padded)
 constexpr int kHid = 64;     // hidden width (two hidden layers)
 constexpr int kOutP = 16;    // output columns computed (first kOut used)
-constexpr int kOut = 9;     // residual (3) and its symmetric Jacobian (6)
+constexpr int kOut = 9;     // basis coefficients: residual (3) and its symmetric Jacobian (6)
 // The Jacobian outputs are J d / scale times kJac: a body sits within about a
 // quarter of d from the group centre, so this weighs a Jacobian error in the
 // training loss as it shows in the correction of a body.
@@ -165,17 +165,26 @@ __device__ inline f32x4 forward_tile(const bf16x4& x, const uint2* __restrict__ 
 	return out;
 }
 
-// Input features of one (evaluation point p, cell) pair and the scale of the
-// target.  u = cm - p, d = |u|, r = bounding radius of the cell (the radius of
-// its opening test, nbody/nbody_space_heap.cpp:130-133), q = Q / (M r^2) with Q
-// the second central moments of the cell.  The network predicts the residual
-// R(p) = exact cell field - monopole field at p, divided by scale = M (r/d)^2 / d^2,
-// and its Jacobian dR/dp (symmetric: xx yy zz xy xz yz) multiplied by kJac d / scale,
-// so that R(p + delta) ~ scale * (out[0:3] + J(out[3:9]) delta / (kJac d)).
-// f: [uhat(3), rho, qxx, qyy, qzz, qxy, qxz, qyz, (q uhat)(3), uhat q uhat, rho^2, tr q]
+// Features, basis and target scale of one (evaluation point p, cell) pair.
+//   u = cm - p, d = |u|, uh = u / d; r = bounding radius of the cell (the radius
+//   of its opening test, nbody/nbody_space_heap.cpp:130-133), rho = r / d;
+//   q = Q / (M r^2) with Q the second central moments; w = q uh; o = (box
+//   centre - cm) / r.
+// The network sees only rotation invariants (f[16]); its 9 outputs are the
+// coefficients of a basis built from uh, w, o and q (b[16] = uh, w, o, q as
+// xx yy zz xy xz yz, 0), so the predicted field turns with the cell:
+//   R / scale              = c0 uh + c1 w + c2 o
+//   kJac J d / scale       = c3 q + c4 (uh w' + w uh') + c5 I + c6 uh uh'
+//                            + c7 (uh o' + o uh') + c8 o o'
+// with R(p) the residual field (exact field of the cell's bodies - monopole)
+// and J = dR/dp, scale = M rho^2 / d^2.  The quadrupole term is exactly of
+// this form (c0 = 7.5 uh.w - 1.5 tr q, c1 = -3, c3 = 3 kJac, c4 = -15 kJac,
+// c5 = (1.5 tr q - 7.5 uh.w) kJac, c6 = (52.5 uh.w - 7.5 tr q) kJac).
+constexpr int kBasis = 16;
+
 __device__ __host__ inline float cell_features(double px, double py, double pz, double cx, double cy, double cz,
                                                double M, const double* bmin, const double* bmax, const double* Q,
-                                               float* f)
+                                               float* f, float* b)
 {
 	const double ux = cx - px, uy = cy - py, uz = cz - pz;
 	const double d2 = ux * ux + uy * uy + uz * uz;
@@ -192,16 +201,70 @@ __device__ __host__ inline float cell_features(double px, double py, double pz, 
 	const double wx = qxx * hx + qxy * hy + qxz * hz;
 	const double wy = qxy * hx + qyy * hy + qyz * hz;
 	const double wz = qxz * hx + qyz * hy + qzz * hz;
+	const double ox = gx / r, oy = gy / r, oz = gz / r;
+	const double vx = qxx * ox + qxy * oy + qxz * oz;   // q o
+	const double vy = qxy * ox + qyy * oy + qyz * oz;
+	const double vz = qxz * ox + qyz * oy + qzz * oz;
 	const double rho = r / d;
-	f[0] = static_cast<float>(hx); f[1] = static_cast<float>(hy); f[2] = static_cast<float>(hz);
-	f[3] = static_cast<float>(rho);
-	f[4] = static_cast<float>(qxx); f[5] = static_cast<float>(qyy); f[6] = static_cast<float>(qzz);
-	f[7] = static_cast<float>(qxy); f[8] = static_cast<float>(qxz); f[9] = static_cast<float>(qyz);
-	f[10] = static_cast<float>(wx); f[11] = static_cast<float>(wy); f[12] = static_cast<float>(wz);
-	f[13] = static_cast<float>(hx * wx + hy * wy + hz * wz);
-	f[14] = static_cast<float>(rho * rho);
-	f[15] = static_cast<float>(qxx + qyy + qzz);
+	const double trq = qxx + qyy + qzz;
+	const double uqu = hx * wx + hy * wy + hz * wz;
+	const double uo = hx * ox + hy * oy + hz * oz;
+	f[0] = static_cast<float>(rho);
+	f[1] = static_cast<float>(rho * rho);
+	f[2] = static_cast<float>(trq);
+	f[3] = static_cast<float>(uqu);
+	f[4] = static_cast<float>(wx * wx + wy * wy + wz * wz);
+	f[5] = static_cast<float>(qxx * qxx + qyy * qyy + qzz * qzz + 2.0 * (qxy * qxy + qxz * qxz + qyz * qyz));
+	f[6] = static_cast<float>(uo);
+	f[7] = static_cast<float>(ox * ox + oy * oy + oz * oz);
+	f[8] = static_cast<float>(wx * ox + wy * oy + wz * oz);
+	f[9] = static_cast<float>(ox * vx + oy * vy + oz * vz);
+	f[10] = static_cast<float>(rho * uqu);
+	f[11] = static_cast<float>(rho * trq);
+	f[12] = static_cast<float>(uo * uo);
+	f[13] = static_cast<float>(rho * uo);
+	f[14] = static_cast<float>(vx * vx + vy * vy + vz * vz);
+	f[15] = static_cast<float>(uqu * uqu);
+	b[0] = static_cast<float>(hx); b[1] = static_cast<float>(hy); b[2] = static_cast<float>(hz);
+	b[3] = static_cast<float>(wx); b[4] = static_cast<float>(wy); b[5] = static_cast<float>(wz);
+	b[6] = static_cast<float>(ox); b[7] = static_cast<float>(oy); b[8] = static_cast<float>(oz);
+	b[9] = static_cast<float>(qxx); b[10] = static_cast<float>(qyy); b[11] = static_cast<float>(qzz);
+	b[12] = static_cast<float>(qxy); b[13] = static_cast<float>(qxz); b[14] = static_cast<float>(qyz);
+	b[15] = 0.0f;
 	return static_cast<float>(M * rho * rho / d2);
+}
+
+// Basis element of output column col as its 9 target components (R: 3, then
+// the symmetric Jacobian as xx yy zz xy xz yz); zero for col >= kOut.
+__device__ __host__ inline void basis_col(const float* b, int col, float* v)
+{
+	for(int k = 0; k < kOut; ++k) { v[k] = 0.0f; }
+	const float ux = b[0], uy = b[1], uz = b[2];
+	const float wx = b[3], wy = b[4], wz = b[5];
+	const float ox = b[6], oy = b[7], oz = b[8];
+	switch(col)
+	{
+	case 0: v[0] = ux; v[1] = uy; v[2] = uz; break;
+	case 1: v[0] = wx; v[1] = wy; v[2] = wz; break;
+	case 2: v[0] = ox; v[1] = oy; v[2] = oz; break;
+	case 3: for(int k = 0; k < 6; ++k) { v[3 + k] = b[9 + k]; } break;
+	case 4:
+		v[3] = 2.0f * ux * wx; v[4] = 2.0f * uy * wy; v[5] = 2.0f * uz * wz;
+		v[6] = ux * wy + uy * wx; v[7] = ux * wz + uz * wx; v[8] = uy * wz + uz * wy;
+		break;
+	case 5: v[3] = 1.0f; v[4] = 1.0f; v[5] = 1.0f; break;
+	case 6:
+		v[3] = ux * ux; v[4] = uy * uy; v[5] = uz * uz; v[6] = ux * uy; v[7] = ux * uz; v[8] = uy * uz;
+		break;
+	case 7:
+		v[3] = 2.0f * ux * ox; v[4] = 2.0f * uy * oy; v[5] = 2.0f * uz * oz;
+		v[6] = ux * oy + uy * ox; v[7] = ux * oz + uz * ox; v[8] = uy * oz + uz * oy;
+		break;
+	case 8:
+		v[3] = ox * ox; v[4] = oy * oy; v[5] = oz * oz; v[6] = ox * oy; v[7] = ox * oz; v[8] = oy * oz;
+		break;
+	default: break;
+	}
 }
 
 }  // namespace nbh_net

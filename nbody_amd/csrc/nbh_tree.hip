@@ -872,11 +872,13 @@ k_band_collect(const double4* __restrict__ xyzr, const double* __restrict__ bmin
 
 // One wave per sample: the exact FP64 field of the cell's bodies at the group
 // centre p and its Jacobian, minus those of the monopole, written as features
-// x[16] and target t[9] = [R / scale, kJac J d / scale] (J as xx yy zz xy xz yz).
+// x[16], basis b[16] and target t[9] = [R / scale, kJac J d / scale] (J as
+// xx yy zz xy xz yz).
 __global__ void __launch_bounds__(64)
 k_band_residual(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
                 const double* __restrict__ bmax, const double* __restrict__ quad, int n, int levels,
-                const uint32_t* __restrict__ pairs, size_t count, float* __restrict__ x, float* __restrict__ t)
+                const uint32_t* __restrict__ pairs, size_t count, float* __restrict__ x, float* __restrict__ bs,
+                float* __restrict__ t)
 {
 	const int lane = threadIdx.x;
 	for(size_t s = blockIdx.x; s < count; s += gridDim.x)
@@ -932,16 +934,18 @@ k_band_residual(const double4* __restrict__ xyzr, const double* __restrict__ nma
 			a[7] -= w5 * dx * dz;
 			a[8] -= w5 * dy * dz;
 			float f[nbh_net::kIn];
+			float bv[nbh_net::kBasis];
 			const double scale = nbh_net::cell_features(px, py, pz, rec.x, rec.y, rec.z, M, bmin + 3 * c, bmax + 3 * c,
-			                                            quad + 6 * c, f);
+			                                            quad + 6 * c, f, bv);
 			for(int k = 0; k < nbh_net::kIn; ++k) { x[s * nbh_net::kIn + k] = f[k]; }
+			for(int k = 0; k < nbh_net::kBasis; ++k) { bs[s * nbh_net::kBasis + k] = bv[k]; }
 			for(int k = 0; k < 3; ++k) { t[s * nbh_net::kOut + k] = static_cast<float>(a[k] / scale); }
 			for(int k = 3; k < 9; ++k) { t[s * nbh_net::kOut + k] = static_cast<float>(a[k] * nbh_net::kJac * d / scale); }
 		}
 	}
 }
 
-constexpr int kBand = 128;       // far-cell rows waiting for the network
+constexpr int kBand = 80;        // far-cell rows waiting for the network (< 16 + 64)
 constexpr int kQuadBuf = 128;    // near cells waiting for the quadrupole sweep
 
 // Quadrupole field of near cells at the lane's body: entries of 3 float4
@@ -987,6 +991,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	__shared__ float4 qbuf[3 * kQuadBuf];
 	__shared__ __align__(16) uint16_t bandf[kBand * nbh_net::kIn];
 	__shared__ float bandw[2 * kBand];      // per far row: scale, scale / (kJac d)
+	__shared__ float bandb[kBand * nbh_net::kBasis];
 	__shared__ __align__(16) uint16_t mlds[2 * 16 * nbh_net::kHid];
 	const int lane = threadIdx.x;
 	const int col = lane & 15;
@@ -1002,9 +1007,8 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	const float tz = static_cast<float>(me.z - ccz);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 	float qx = 0.0f, qy = 0.0f, qz = 0.0f;  // quadrupole terms of near cells
-	float corr = 0.0f;                      // column `col` of the output, weighted and summed over far cells
-	const int wsel = col < 3 ? 0 : 1;       // weight: scale for R, scale / (kJac d) for J
-	const bool used_col = col < nbh_net::kOut;
+	float acc[nbh_net::kOut];               // this lane's share of the far-cell correction (R, J)
+	for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] = 0.0f; }
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	unsigned long long vis = 0, taken = 0, spill = 0, corrected = 0, nearq = 0;
 
@@ -1018,21 +1022,28 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 				for(int r = nb + (lane >> 4); r < 16; r += 4)
 				{
 					bandf[r * nbh_net::kIn + col] = 0;
+					bandb[r * nbh_net::kBasis + col] = 0.0f;
 					if(col < 2) { bandw[2 * r + col] = 0.0f; }
 				}
 				__syncthreads();
 			}
 			const nbh_net::bf16x4 xa = nbh_net::lds_afrag(bandf, nbh_net::kIn, 0, lane);
 			const nbh_net::f32x4 o = nbh_net::forward_tile(xa, frags, mlp_p, mlds, lane, nullptr, nullptr);
-			if(used_col)
+			for(int e = 0; e < 4; ++e)
 			{
-				for(int e = 0; e < 4; ++e) { corr += o[e] * bandw[2 * (4 * (lane >> 4) + e) + wsel]; }
+				const int row = 4 * (lane >> 4) + e;
+				float v[nbh_net::kOut];
+				nbh_net::basis_col(bandb + row * nbh_net::kBasis, col, v);
+				const float c0 = o[e] * bandw[2 * row], c1 = o[e] * bandw[2 * row + 1];
+				for(int k = 0; k < 3; ++k) { acc[k] = fmaf(c0, v[k], acc[k]); }
+				for(int k = 3; k < nbh_net::kOut; ++k) { acc[k] = fmaf(c1, v[k], acc[k]); }
 			}
 			const int rest = nb > 16 ? nb - 16 : 0;
 			__syncthreads();
 			for(int r = lane >> 4; r < rest; r += 4)
 			{
 				bandf[r * nbh_net::kIn + col] = bandf[(r + 16) * nbh_net::kIn + col];
+				bandb[r * nbh_net::kBasis + col] = bandb[(r + 16) * nbh_net::kBasis + col];
 				if(col < 2) { bandw[2 * r + col] = bandw[2 * (r + 16) + col]; }
 			}
 			__syncthreads();
@@ -1106,8 +1117,10 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		{
 			const int slot = nband + __popcll(bband & lt);
 			float fe[nbh_net::kIn];
+			float be[nbh_net::kBasis];
 			const float scale = nbh_net::cell_features(ccx, ccy, ccz, rec.x, rec.y, rec.z, m, bmin + 3 * node,
-			                                           bmax + 3 * node, quad + 6 * node, fe);
+			                                           bmax + 3 * node, quad + 6 * node, fe, be);
+			for(int k = 0; k < nbh_net::kBasis; ++k) { bandb[slot * nbh_net::kBasis + k] = be[k]; }
 			bandw[2 * slot] = scale;
 			bandw[2 * slot + 1] = static_cast<float>(static_cast<double>(scale) / (nbh_net::kJac * dc));
 			for(int k = 0; k < nbh_net::kIn; ++k) { bandf[slot * nbh_net::kIn + k] = nbh_net::to_bf16(fe[k]); }
@@ -1154,10 +1167,11 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	}
 	__syncthreads();
 	run_tiles(nband, true);
-	corr += __shfl_xor(corr, 16, 64);
-	corr += __shfl_xor(corr, 32, 64);
-	float S[nbh_net::kOut];
-	for(int k = 0; k < nbh_net::kOut; ++k) { S[k] = __shfl(corr, k, 64); }
+	for(int sh = 1; sh < 64; sh <<= 1)
+	{
+		for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
+	}
+	const float* S = acc;
 	const float kx = qx + (S[0] + S[3] * tx + S[6] * ty + S[7] * tz);
 	const float ky = qy + (S[1] + S[6] * tx + S[4] * ty + S[8] * tz);
 	const float kz = qz + (S[2] + S[7] * tx + S[8] * ty + S[5] * tz);
@@ -1418,8 +1432,8 @@ nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* t, const double* 
 }
 
 nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* groups, size_t n_groups,
-                               float rho_min, float near_k, float* x, float* target, uint32_t* cells,
-                               size_t capacity, size_t* count)
+                               float rho_min, float near_k, float* x, float* basis, float* target,
+                               uint32_t* cells, size_t capacity, size_t* count)
 {
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
@@ -1428,7 +1442,7 @@ nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* 
 	NBH_CHECK(t->levels + 1 <= kWalkStack / kGroup, "tree too deep for the walk stack");
 	NBH_CHECK(groups != nullptr && count != nullptr, "null argument");
 	NBH_CHECK(n_groups > 0 && n_groups <= (static_cast<size_t>(1) << 30), "bad group count %zu", n_groups);
-	NBH_CHECK(capacity == 0 || (x != nullptr && target != nullptr), "null sample buffer");
+	NBH_CHECK(capacity == 0 || (x != nullptr && basis != nullptr && target != nullptr), "null sample buffer");
 	NBH_CHECK(capacity <= (static_cast<size_t>(1) << 31), "capacity %zu too large", capacity);
 	NBH_CHECK(rho_min >= 0.0f && near_k >= 0.0f, "rho_min and near_k must be >= 0");
 	const size_t ngroups_tree = t->n / kGroup;
@@ -1462,7 +1476,7 @@ nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* 
 	if(m > 0)
 	{
 		hipLaunchKernelGGL(k_band_residual, dim3(nbh::grid_for(ctx, m, 1)), dim3(64), 0, ctx->stream, t->xyzr,
-		                   t->mass, t->bmin, t->bmax, t->quad, n, t->levels, pairs, m, x, target);
+		                   t->mass, t->bmin, t->bmax, t->quad, n, t->levels, pairs, m, x, basis, target);
 		NBH_LAUNCHED();
 		if(cells != nullptr)
 		{

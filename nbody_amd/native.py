@@ -65,9 +65,9 @@ HIP_API = {
     "nbh_mlp_set_params": (i32, [vp, vp, vp]),
     "nbh_mlp_get_params": (i32, [vp, vp, vp]),
     "nbh_mlp_forward": (i32, [vp, vp, vp, sz, vp]),
-    "nbh_mlp_gradient": (i32, [vp, vp, vp, vp, sz, vp, C.POINTER(C.c_float)]),
-    "nbh_mlp_train": (i32, [vp, vp, vp, vp, sz, i32, sz, C.c_float, vp]),
-    "nbh_bh_band_samples": (i32, [vp, vp, vp, sz, C.c_float, C.c_float, vp, vp, vp, sz, C.POINTER(sz)]),
+    "nbh_mlp_gradient": (i32, [vp, vp, vp, vp, vp, sz, vp, C.POINTER(C.c_float)]),
+    "nbh_mlp_train": (i32, [vp, vp, vp, vp, vp, sz, i32, sz, C.c_float, vp]),
+    "nbh_bh_band_samples": (i32, [vp, vp, vp, sz, C.c_float, C.c_float, vp, vp, vp, vp, sz, C.POINTER(sz)]),
     "nbh_bh_fcompute_group_mlp": (i32, [vp, vp, vp, vp, vp, C.c_float, C.c_float]),
     "nbh_stats": (i32, [vp, vp, vp, sz, i32, PD]),
     "nbh_timing_enable": (i32, [vp, i32]),

@@ -11,7 +11,8 @@ Contents
     vector (W1[16][64] b1[64] W2[64][64] b2[64] W3[64][16] b3[16], row-major),
     in float64, or emulating the device arithmetic (every matrix-core operand
     rounded to BF16 with round-to-nearest-even, FP32 elsewhere);
-  * cell_features: the 16 input features of (evaluation point, cell);
+  * cell_features / basis_matrix / predict: the 16 rotation-invariant inputs of
+    (evaluation point, cell) and the basis the 9 outputs are coefficients of;
   * cell_residual: exact residual field and Jacobian of a cell by brute force;
   * group_band: the near and far cells of one 64-body group, by a plain stack
     walk of the heap tree with the group walk's acceptance rule;
@@ -23,8 +24,9 @@ Pinning (tests/test_oracle_mlp.py)
   * bf16 rounding: exhaustive on special patterns, ties to even, vs float32;
   * cell_residual: Jacobian vs finite differences of the field; at small
     size/distance the residual tends to the textbook quadrupole field;
-  * cell_features: invariants (unit direction, rotation covariance of q and
-    q.u, rho = radius / distance) rather than the formulas restated;
+  * cell_features / basis: invariance of the features and covariance of the
+    basis under a symmetry of the box; the quadrupole coefficients reproduce
+    the textbook quadrupole field and its finite-difference Jacobian;
   * group_band: a cell set property (every body of the group is covered once by
     accepted cells and opened cells' descendants), checked on small trees;
   * quadrupole_field: limit of the exact residual of a small distant cell.
@@ -99,17 +101,48 @@ def forward(p, x, mode: str = "f64"):
     return out[:, :K_OUT], (xb, h1, h2, out)
 
 
-def loss_grad(p, x, t, mode: str = "f64"):
-    """L = sum |out - t|^2 / m and dL/dp (flat), matching nbh_mlp_gradient."""
+def basis_matrix(b) -> np.ndarray:
+    """[m][9 columns][9 components]: the target-space vector of each output
+    column (R: 3 components, then the symmetric Jacobian xx yy zz xy xz yz)."""
+    b = np.asarray(b)
+    m = b.shape[0]
+    u, w, o, q = b[:, 0:3], b[:, 3:6], b[:, 6:9], b[:, 9:15]
+
+    def sym(a, c):
+        return np.stack([2 * a[:, 0] * c[:, 0], 2 * a[:, 1] * c[:, 1], 2 * a[:, 2] * c[:, 2],
+                         a[:, 0] * c[:, 1] + a[:, 1] * c[:, 0], a[:, 0] * c[:, 2] + a[:, 2] * c[:, 0],
+                         a[:, 1] * c[:, 2] + a[:, 2] * c[:, 1]], axis=1)
+
+    B = np.zeros((m, K_OUT, K_OUT), dtype=b.dtype)
+    B[:, 0, :3], B[:, 1, :3], B[:, 2, :3] = u, w, o
+    B[:, 3, 3:] = q
+    B[:, 4, 3:] = sym(u, w)
+    B[:, 5, 3:6] = 1.0
+    B[:, 6, 3:] = 0.5 * sym(u, u)
+    B[:, 7, 3:] = sym(u, o)
+    B[:, 8, 3:] = 0.5 * sym(o, o)
+    return B
+
+
+def predict(p, x, b, mode: str = "f64"):
+    """Predicted targets [m][9] = sum over columns of coefficient x basis vector."""
+    c, _ = forward(p, x, mode)
+    dt, _ = _ops(mode)
+    return np.einsum("mc,mck->mk", c, basis_matrix(np.asarray(b, dt))).astype(dt)
+
+
+def loss_grad(p, x, b, t, mode: str = "f64"):
+    """L = sum |B c - t|^2 / m and dL/dp (flat), matching nbh_mlp_gradient."""
     dt, rnd = _ops(mode)
     W1, b1, W2, b2, W3, b3 = (np.asarray(v, dt) for v in unpack(p))
     m = x.shape[0]
     _, (xb, h1, h2, out) = forward(p, x, mode)
-    t = np.asarray(t, dt)
-    diff = np.zeros_like(out)
-    diff[:, :K_OUT] = out[:, :K_OUT] - t
+    B = basis_matrix(np.asarray(b, dt))
+    pred = np.einsum("mc,mck->mk", out[:, :K_OUT], B)
+    diff = (pred - np.asarray(t, dt)).astype(dt)
     loss = float(np.sum(diff.astype(np.float64) ** 2) / m)
-    dout = (2.0 * diff / m).astype(dt)
+    dout = np.zeros_like(out)
+    dout[:, :K_OUT] = 2.0 * np.einsum("mk,mck->mc", diff, B) / m
     gW3 = rnd(h2).T @ rnd(dout)
     gb3 = dout.sum(0)
     dz2 = (rnd(dout) @ rnd(W3).T) * (1 - h2 * h2)
@@ -132,18 +165,25 @@ def cell_radius(bmin, bmax, cm) -> float:
 
 
 def cell_features(p, cm, M, bmin, bmax, Q):
-    """16 features and the target scale M r^2 / d^4 of (point p, cell)."""
+    """16 rotation-invariant features, the basis data b[16] (uh, w, o, q, 0),
+    the target scale M r^2 / d^4 and the distance d of (point p, cell)."""
     p, cm, Q = (np.asarray(v, np.float64) for v in (p, cm, Q))
+    bmin, bmax = np.asarray(bmin, np.float64), np.asarray(bmax, np.float64)
     u = cm - p
     d = np.linalg.norm(u)
     r = cell_radius(bmin, bmax, cm)
     q = np.array([[Q[0], Q[3], Q[4]], [Q[3], Q[1], Q[5]], [Q[4], Q[5], Q[2]]]) / (M * r * r)
     h = u / d
     w = q @ h
+    o = (0.5 * (bmax + bmin) - cm) / r
+    v = q @ o
     rho = r / d
-    f = np.array([h[0], h[1], h[2], rho, q[0, 0], q[1, 1], q[2, 2], q[0, 1], q[0, 2], q[1, 2],
-                  w[0], w[1], w[2], h @ w, rho * rho, np.trace(q)])
-    return f, M * rho * rho / (d * d), d
+    uqu = h @ w
+    uo = h @ o
+    f = np.array([rho, rho * rho, np.trace(q), uqu, w @ w, np.sum(q * q), uo, o @ o, w @ o, o @ v,
+                  rho * uqu, rho * np.trace(q), uo * uo, rho * uo, v @ v, uqu * uqu])
+    b = np.concatenate([h, w, o, [q[0, 0], q[1, 1], q[2, 2], q[0, 1], q[0, 2], q[1, 2]], [0.0]])
+    return f, b, M * rho * rho / (d * d), d
 
 
 def field(p, pos, mass):
@@ -173,13 +213,26 @@ def cell_residual(p, pos, mass):
 
 
 def target_vector(p, pos, mass, bmin, bmax):
-    """Features x[16] and target t[9] of the cell holding bodies pos/mass."""
+    """Features x[16], basis b[16] and target t[9] of the cell holding bodies pos/mass."""
     cm, Q = second_moments(pos, mass)
     M = mass.sum()
-    f, scale, d = cell_features(p, cm, M, bmin, bmax, Q)
+    f, b, scale, d = cell_features(p, cm, M, bmin, bmax, Q)
     R, J = cell_residual(p, pos, mass)
     Jv = np.array([J[0, 0], J[1, 1], J[2, 2], J[0, 1], J[0, 2], J[1, 2]])
-    return f, np.concatenate([R / scale, K_JAC * Jv * d / scale])
+    return f, b, np.concatenate([R / scale, K_JAC * Jv * d / scale])
+
+
+def quadrupole_coefficients(f):
+    """Basis coefficients of the exact quadrupole term (features f)."""
+    trq, uqu = f[..., 2], f[..., 3]
+    c = np.zeros(np.shape(f)[:-1] + (K_OUT,))
+    c[..., 0] = 7.5 * uqu - 1.5 * trq
+    c[..., 1] = -3.0
+    c[..., 3] = 3.0 * K_JAC
+    c[..., 4] = -15.0 * K_JAC
+    c[..., 5] = (1.5 * trq - 7.5 * uqu) * K_JAC
+    c[..., 6] = (52.5 * uqu - 7.5 * trq) * K_JAC
+    return c
 
 
 def quadrupole_field(p, cm, Q):

@@ -50,18 +50,24 @@ class Net:
         self.ctx.sync()
         return do.read_raw(F32, m * mo.K_OUT).reshape(m, mo.K_OUT)
 
-    def gradient(self, dx, dt, m):
+    def gradient(self, dx, db, dt, m):
         g = np.empty(mo.N_PARAMS, F32)
         loss = C.c_float()
-        native.check(self.ctx.lib.nbh_mlp_gradient(self.ctx.h, self.h, dx.ptr, dt.ptr, m,
+        native.check(self.ctx.lib.nbh_mlp_gradient(self.ctx.h, self.h, dx.ptr, db.ptr, dt.ptr, m,
                                                    g.ctypes.data_as(C.c_void_p), C.byref(loss)), "gradient")
         return g.astype(np.float64), loss.value
 
-    def train(self, dx, dt, m, epochs, batch, lr):
+    def train(self, dx, db, dt, m, epochs, batch, lr):
         el = np.empty(epochs, F32)
-        native.check(self.ctx.lib.nbh_mlp_train(self.ctx.h, self.h, dx.ptr, dt.ptr, m, epochs, batch, lr,
+        native.check(self.ctx.lib.nbh_mlp_train(self.ctx.h, self.h, dx.ptr, db.ptr, dt.ptr, m, epochs, batch, lr,
                                                 el.ctypes.data_as(C.c_void_p)), "train")
         return el
+
+
+def _random_basis(rng, m):
+    b = rng.normal(size=(m, 16)).astype(F32)
+    b[:, 15] = 0.0
+    return b
 
 
 def _random_params(seed, bias=0.3):
@@ -117,11 +123,13 @@ def test_gradient_matches_oracle(ctx, m):
     p = _random_params(12)
     net.set(p)
     x = rng.normal(size=(m, mo.K_IN)).astype(F32)
+    bs = _random_basis(rng, m)
     t = rng.normal(size=(m, mo.K_OUT)).astype(F32)
-    dx, dt = ctx.put_raw(x), ctx.put_raw(t)
-    g, loss = net.gradient(dx, dt, m)
-    lb, gb = mo.loss_grad(p, x, t, "bf16")
-    l64, g64 = mo.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), "f64")
+    dx, db, dt = ctx.put_raw(x), ctx.put_raw(bs), ctx.put_raw(t)
+    g, loss = net.gradient(dx, db, dt, m)
+    lb, gb = mo.loss_grad(p, x, bs, t, "bf16")
+    l64, g64 = mo.loss_grad(p.astype(np.float64), x.astype(np.float64), bs.astype(np.float64),
+                            t.astype(np.float64), "f64")
     assert abs(loss - lb) <= 2e-3 * lb
     assert np.linalg.norm(g - gb) <= 1e-2 * np.linalg.norm(gb), np.linalg.norm(g - gb) / np.linalg.norm(gb)
     assert np.linalg.norm(g - g64) <= 5e-2 * np.linalg.norm(g64)
@@ -137,24 +145,32 @@ def test_training_reduces_loss(ctx):
     rng = np.random.default_rng(5)
     m = 16384
     x = rng.normal(size=(m, mo.K_IN)).astype(F32)
+    bs = _random_basis(rng, m) * F32(0.5)
     A = rng.normal(size=(mo.K_IN, mo.K_OUT)) / 4
-    t = (np.tanh(x @ A) + 0.1 * np.sin(x[:, :mo.K_OUT] * 2)).astype(F32)
-    dx, dt = ctx.put_raw(x), ctx.put_raw(t)
+    c_true = np.tanh(x @ A) + 0.1 * np.sin(x[:, :mo.K_OUT] * 2)
+    t = np.einsum("mc,mck->mk", c_true, mo.basis_matrix(bs.astype(np.float64))).astype(F32)
+    dx, db, dt = ctx.put_raw(x), ctx.put_raw(bs), ctx.put_raw(t)
     net = Net(ctx, 3)
-    _, l0 = net.gradient(dx, dt, m)
-    el = net.train(dx, dt, m, 20, 512, 3e-3)
-    _, l1 = net.gradient(dx, dt, m)
+    _, l0 = net.gradient(dx, db, dt, m)
+    el = net.train(dx, db, dt, m, 20, 512, 3e-3)
+    _, l1 = net.gradient(dx, db, dt, m)
     assert np.all(np.isfinite(el))
     assert l1 < 0.1 * l0, (l0, l1, el)
     # the trained parameters mean the same thing to the oracle
-    pout, _ = mo.forward(net.get(), x[:2048], "bf16")
-    assert abs(np.mean((pout - t[:2048]) ** 2) * mo.K_OUT - l1) < 0.25 * l1 + 1e-4
+    pout = mo.predict(net.get(), x[:2048], bs[:2048], "bf16")
+    assert abs(np.sum((pout - t[:2048]) ** 2) / 2048 - l1) < 0.25 * l1 + 1e-4
     net.close()
 
 
 # ---------------------------------------------------------------------------
 # tree side
 # ---------------------------------------------------------------------------
+class Samples:
+    def __init__(self, total, dx, db, dt, x, b, t, cells):
+        self.total, self.dx, self.db, self.dt = total, dx, db, dt
+        self.x, self.b, self.t, self.cells = x, b, t, cells
+
+
 class Tree:
     def __init__(self, ctx, y, m, ratio):
         self.ctx, self.n = ctx, len(m)
@@ -176,15 +192,17 @@ class Tree:
     def samples(self, groups, rho, near_k, capacity):
         g = np.ascontiguousarray(groups, np.uint32)
         dx = self.ctx.zeros_raw(F32, max(1, capacity) * mo.K_IN)
+        db = self.ctx.zeros_raw(F32, max(1, capacity) * 16)
         dt = self.ctx.zeros_raw(F32, max(1, capacity) * mo.K_OUT)
         dc = self.ctx.zeros_raw(np.uint32, max(1, capacity) * 2)
         cnt = C.c_size_t()
         native.check(self.ctx.lib.nbh_bh_band_samples(self.ctx.h, self.h, g.ctypes.data_as(C.c_void_p), len(g),
-                                                      rho, near_k, dx.ptr, dt.ptr, dc.ptr, capacity, C.byref(cnt)),
-                     "samples")
+                                                      rho, near_k, dx.ptr, db.ptr, dt.ptr, dc.ptr, capacity,
+                                                      C.byref(cnt)), "samples")
         k = min(cnt.value, capacity)
-        return (cnt.value, dx, dt, dx.read_raw(F32, k * mo.K_IN).reshape(k, mo.K_IN),
-                dt.read_raw(F32, k * mo.K_OUT).reshape(k, mo.K_OUT), dc.read_raw(np.uint32, 2 * k).reshape(k, 2))
+        return Samples(cnt.value, dx, db, dt, dx.read_raw(F32, k * mo.K_IN).reshape(k, mo.K_IN),
+                       db.read_raw(F32, k * 16).reshape(k, 16), dt.read_raw(F32, k * mo.K_OUT).reshape(k, mo.K_OUT),
+                       dc.read_raw(np.uint32, 2 * k).reshape(k, 2))
 
     def group(self):
         df = self.ctx.alloc(6 * self.n)
@@ -222,7 +240,8 @@ def test_band_samples_match_oracle(ctx):
     t = Tree(ctx, y, m, ratio)
     a = t.arrays()
     groups = np.arange(n // 64)
-    total, _, _, x, tg, cells = t.samples(groups, rho, near_k, 1 << 20)
+    smp = t.samples(groups, rho, near_k, 1 << 20)
+    total, x, bs, tg, cells = smp.total, smp.x, smp.b, smp.t, smp.cells
     assert total == len(cells) > 0
     want = set()
     for g in groups:
@@ -235,11 +254,13 @@ def test_band_samples_match_oracle(ctx):
         gn = n // 64 + g
         p = 0.5 * (a["bmin"][gn] + a["bmax"][gn])
         l0, l1 = _leaf_slice(n, c)
-        fx, ft = mo.target_vector(p, a["cm"][l0:l1], a["mass"][l0:l1], a["bmin"][c], a["bmax"][c])
+        fx, fb, ft = mo.target_vector(p, a["cm"][l0:l1], a["mass"][l0:l1], a["bmin"][c], a["bmax"][c])
         assert np.allclose(x[k], fx, rtol=1e-5, atol=1e-6 * np.abs(fx).max()), (k, x[k], fx)
+        assert np.allclose(bs[k], fb, rtol=1e-5, atol=1e-6 * np.abs(fb).max()), (k, bs[k], fb)
         assert np.allclose(tg[k], ft, rtol=1e-4, atol=1e-4 * np.abs(ft).max()), (k, tg[k], ft)
     # a smaller capacity: the same count, only `capacity` rows written
-    total2, _, _, x2, _, cells2 = t.samples(groups[:8], rho, near_k, 5)
+    smp2 = t.samples(groups[:8], rho, near_k, 5)
+    total2, cells2 = smp2.total, smp2.cells
     assert len(cells2) == 5 and total2 == sum(1 for g, _ in want if g < 8)
     assert {(int(g), int(c)) for g, c in cells2} <= want
     t.close()
@@ -251,8 +272,8 @@ def test_band_samples_reject_bad_group(ctx):
     t = Tree(ctx, y, m, 1.25)
     g = np.array([0, n // 64], np.uint32)
     cnt = C.c_size_t()
-    st = ctx.lib.nbh_bh_band_samples(ctx.h, t.h, g.ctypes.data_as(C.c_void_p), 2, 0.2, 2.0, None, None, None, 0,
-                                     C.byref(cnt))
+    st = ctx.lib.nbh_bh_band_samples(ctx.h, t.h, g.ctypes.data_as(C.c_void_p), 2, 0.2, 2.0, None, None, None,
+                                     None, 0, C.byref(cnt))
     assert st != 0 and b"out of range" in ctx.lib.nbh_last_error()
     t.close()
 
@@ -289,13 +310,13 @@ def _oracle_correction(n, a, body, p, rho, near_k, use_net=True):
         far, _, near = mo.group_band(n, g, a["cm"], a["crit"], a["bmin"], a["bmax"], rho, near_k)
         S = np.zeros(mo.K_OUT)
         if far and use_net:
-            feats, w0, w1 = [], [], []
+            feats, basis, w0, w1 = [], [], [], []
             for c in far:
                 l0, l1 = _leaf_slice(n, c)
                 _, Q = mo.second_moments(a["cm"][l0:l1], a["mass"][l0:l1])
-                f, scale, d = mo.cell_features(cen, a["cm"][c], a["mass"][c], a["bmin"][c], a["bmax"][c], Q)
-                feats.append(f); w0.append(scale); w1.append(scale / (mo.K_JAC * d))
-            out, _ = mo.forward(p, np.array(feats, F32), "bf16")
+                f, b, scale, d = mo.cell_features(cen, a["cm"][c], a["mass"][c], a["bmin"][c], a["bmax"][c], Q)
+                feats.append(f); basis.append(b); w0.append(scale); w1.append(scale / (mo.K_JAC * d))
+            out = mo.predict(p, np.array(feats, F32), np.array(basis, F32), "bf16").astype(np.float64)
             S[:3] = (out[:, :3] * np.array(w0)[:, None]).sum(0)
             S[3:] = (out[:, 3:] * np.array(w1)[:, None]).sum(0)
         J = np.array([[S[3], S[6], S[7]], [S[6], S[4], S[8]], [S[7], S[8], S[5]]])
@@ -375,12 +396,13 @@ def test_trained_correction_gain(ctx, capsys):
     n, ratio, rho, near_k = 1 << 16, 1.25, 0.0, 2.0
     ytr, mtr = _state("plummer", n, 31)
     ttr = Tree(ctx, ytr, mtr, ratio)
-    groups = np.random.default_rng(0).choice(n // 64, 512, replace=False)
-    cap = 1 << 21
-    total, dx, dt, _, _, _ = ttr.samples(groups, rho, near_k, cap)
+    groups = np.arange(n // 64)
+    cap = 1 << 22
+    smp = ttr.samples(groups, rho, near_k, cap)
+    total = smp.total
     k = min(total, cap)
     net = Net(ctx, 5)
-    el = net.train(dx, dt, k, 30, 1024, 2e-3)
+    el = net.train(smp.dx, smp.db, smp.dt, k, 30, 1024, 2e-3)
     ttr.close()
     y, m = _state("plummer", n, 32)
     t = Tree(ctx, y, m, ratio)
@@ -397,8 +419,9 @@ def test_trained_correction_gain(ctx, capsys):
         print(f"\n[correction gain] samples {total}  loss {el[0]:.3g}->{el[-1]:.3g}  rms/p99: "
               f"plain 1.25 {e_plain[0]:.3e}/{e_plain[1]:.3e}  near-quad 1.25 {e_near[0]:.3e}/{e_near[1]:.3e}  "
               f"near-quad+net 1.25 {e_full[0]:.3e}/{e_full[1]:.3e}  plain 2.0 {e_plain2[0]:.3e}/{e_plain2[1]:.3e}")
-    assert e_near[0] < e_plain[0]
-    assert e_full[0] < 0.85 * e_near[0]
+    # measured on the MI355X: plain 1.40e-2, near-quad 1.05e-2, near-quad + net 8.1e-3
+    assert e_near[0] < 0.9 * e_plain[0]
+    assert e_full[0] < 0.9 * e_near[0]
     net.close()
     zero.close()
     t.close()

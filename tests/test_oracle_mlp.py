@@ -47,13 +47,20 @@ def test_bf16_rounding():
     assert one + ulp == np.float32(1.0078125)
 
 
+def _random_basis(rng, m):
+    b = rng.normal(size=(m, 16))
+    b[:, 15] = 0.0
+    return b
+
+
 def test_loss_grad_finite_differences():
     rng = np.random.default_rng(2)
     p = _random_params(3)
     m = 37                                    # not a multiple of the 16-row tile
     x = rng.normal(size=(m, mo.K_IN))
+    bs = _random_basis(rng, m)
     t = rng.normal(size=(m, mo.K_OUT))
-    loss, g = mo.loss_grad(p, x, t, "f64")
+    loss, g = mo.loss_grad(p, x, bs, t, "f64")
     blocks = [(mo.OFF_W1, mo.OFF_B1), (mo.OFF_B1, mo.OFF_W2), (mo.OFF_W2, mo.OFF_B2), (mo.OFF_B2, mo.OFF_W3),
               (mo.OFF_W3, mo.OFF_B3), (mo.OFF_B3, mo.N_PARAMS)]
     h = 1e-6
@@ -62,7 +69,7 @@ def test_loss_grad_finite_differences():
         for i in idx:
             pp = p.copy(); pp[i] += h
             pm = p.copy(); pm[i] -= h
-            fd = (mo.loss_grad(pp, x, t)[0] - mo.loss_grad(pm, x, t)[0]) / (2 * h)
+            fd = (mo.loss_grad(pp, x, bs, t)[0] - mo.loss_grad(pm, x, bs, t)[0]) / (2 * h)
             assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
     # unused output columns (9..15) receive no gradient
     W3g = g[mo.OFF_W3:mo.OFF_B3].reshape(mo.K_HID, mo.K_OUTP)
@@ -74,12 +81,13 @@ def test_rounding_modes_agree():
     rng = np.random.default_rng(4)
     p = _random_params(5)
     x = rng.normal(size=(64, mo.K_IN))
+    b = _random_basis(rng, 64)
     t = rng.normal(size=(64, mo.K_OUT))
-    l64, g64 = mo.loss_grad(p, x, t, "f64")
-    l32, g32 = mo.loss_grad(p, x, t, "f32")
+    l64, g64 = mo.loss_grad(p, x, b, t, "f64")
+    l32, g32 = mo.loss_grad(p, x, b, t, "f32")
     assert abs(l32 - l64) < 1e-5 * l64
     assert np.linalg.norm(g32 - g64) < 1e-5 * np.linalg.norm(g64)
-    lb, gb = mo.loss_grad(p, x, t, "bf16")
+    lb, gb = mo.loss_grad(p, x, b, t, "bf16")
     assert np.linalg.norm(gb - g64) < 5e-2 * np.linalg.norm(g64)
     assert np.dot(gb, g64) > 0.99 * np.linalg.norm(gb) * np.linalg.norm(g64)
 
@@ -125,36 +133,60 @@ def _rotation(rng):
     return q * np.sign(np.linalg.det(q))
 
 
-def test_features_rotation_covariance():
+def test_features_invariant_basis_covariant():
     rng = np.random.default_rng(8)
     pos, mass = _cluster(rng)
+    pos = pos + np.array([0.3, -0.2, 0.1])     # box centre away from the mass centre
     p = np.array([5.0, 1.0, -2.0])
-    Rm = _rotation(rng)
 
     def feats(P, X):
         cm, Q = mo.second_moments(X, mass)
         return mo.cell_features(P, cm, mass.sum(), X.min(0), X.max(0), Q)
 
-    f0, s0, d0 = feats(p, pos)
-    assert abs(np.linalg.norm(f0[:3]) - 1) < 1e-12
+    f0, b0, s0, d0 = feats(p, pos)
+    assert abs(np.linalg.norm(b0[:3]) - 1) < 1e-12
     cm = (pos * mass[:, None]).sum(0) / mass.sum()
     r0 = mo.cell_radius(pos.min(0), pos.max(0), cm)
-    assert np.isclose(f0[3], r0 / np.linalg.norm(cm - p)) and np.isclose(f0[14], f0[3] ** 2)
-    # rotate bodies and point about the origin with the box kept equal in size:
-    # use a rotation that maps the box to itself (axis permutation with signs)
+    assert np.isclose(f0[0], r0 / np.linalg.norm(cm - p)) and np.isclose(f0[1], f0[0] ** 2)
+    assert np.isclose(d0, np.linalg.norm(cm - p)) and np.isclose(s0, mass.sum() * r0 ** 2 / d0 ** 4)
+    # a symmetry of the axis-aligned box (signed axis permutation): features
+    # unchanged, basis vectors and q transformed
     P = np.array([[0, 1, 0], [0, 0, -1], [-1, 0, 0]], float)
-    f1, s1, d1 = feats(P @ p, pos @ P.T)
-    assert np.allclose(f1[:3], P @ f0[:3])
-    q0 = np.array([[f0[4], f0[7], f0[8]], [f0[7], f0[5], f0[9]], [f0[8], f0[9], f0[6]]])
-    q1 = np.array([[f1[4], f1[7], f1[8]], [f1[7], f1[5], f1[9]], [f1[8], f1[9], f1[6]]])
-    assert np.allclose(q1, P @ q0 @ P.T)
-    assert np.allclose(f1[10:13], P @ f0[10:13])
-    assert np.allclose([f1[3], f1[13], f1[14], f1[15], s1, d1], [f0[3], f0[13], f0[14], f0[15], s0, d0])
-    # general rotation: the second-moment tensor transforms as a tensor
-    cmr, Qr = mo.second_moments(pos @ Rm.T, mass)
+    f1, b1, s1, d1 = feats(P @ p, pos @ P.T)
+    assert np.allclose(f1, f0) and np.isclose(s1, s0) and np.isclose(d1, d0)
+    for k in (0, 3, 6):
+        assert np.allclose(b1[k:k + 3], P @ b0[k:k + 3])
     Qm = lambda v: np.array([[v[0], v[3], v[4]], [v[3], v[1], v[5]], [v[4], v[5], v[2]]])
-    _, Q0 = mo.second_moments(pos, mass)
-    assert np.allclose(Qm(Qr), Rm @ Qm(Q0) @ Rm.T)
+    assert np.allclose(Qm(b1[9:15]), P @ Qm(b0[9:15]) @ P.T)
+    # predictions of any network turn with the cell
+    prm = _random_params(9)
+    y0 = mo.predict(prm, f0[None], b0[None])[0]
+    y1 = mo.predict(prm, f1[None], b1[None])[0]
+    assert np.allclose(y1[:3], P @ y0[:3])
+    assert np.allclose(Qm(y1[3:]), P @ Qm(y0[3:]) @ P.T)
+
+
+def test_quadrupole_coefficients_reproduce_quadrupole_field():
+    """The basis spans the quadrupole term: its closed-form coefficients give
+    the textbook quadrupole field and its finite-difference Jacobian."""
+    rng = np.random.default_rng(11)
+    pos, mass = _cluster(rng)
+    cm, Q = mo.second_moments(pos, mass)
+    for p in (np.array([6.0, -2.0, 3.0]), np.array([-1.0, 9.0, 0.5])):
+        f, b, scale, d = mo.cell_features(p, cm, mass.sum(), pos.min(0), pos.max(0), Q)
+        c = mo.quadrupole_coefficients(f)
+        y = np.einsum("c,ck->k", c, mo.basis_matrix(b[None])[0])
+        assert np.allclose(y[:3] * scale, mo.quadrupole_field(p, cm, Q), rtol=1e-12)
+        J = np.zeros((3, 3))
+        h = 1e-6 * d
+        for j in range(3):
+            e = np.zeros(3); e[j] = h
+            J[:, j] = (mo.quadrupole_field(p + e, cm, Q) - mo.quadrupole_field(p - e, cm, Q)) / (2 * h)
+        Jv = np.array([J[0, 0], J[1, 1], J[2, 2], J[0, 1], J[0, 2], J[1, 2]])
+        assert np.allclose(y[3:] * scale / (mo.K_JAC * d), Jv, rtol=1e-6, atol=1e-9 * np.abs(Jv).max())
+        # and the exact residual is close to it for this distant cell
+        _, _, t = mo.target_vector(p, pos, mass, pos.min(0), pos.max(0))
+        assert np.linalg.norm(t - y) < 0.3 * np.linalg.norm(y)
 
 
 def _plummer_like(n, seed):
