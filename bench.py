@@ -61,13 +61,17 @@ def cpu_baseline(y, m, n, budget_s: float, threads: int) -> dict:
                       f"{t:.1f} s on {threads} host threads"}
 
 
-def bh_bench(n: int, steps: int, warmup: int, ratios, samples: int = 8192) -> dict:
+def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> dict:
     """Barnes-Hut euler steps/s (one fcompute = tree build + group walk, plus one
     fmadd_inplace, as the reference bench's step) on a Plummer sphere of n bodies,
-    with the relative force error on `samples` random bodies against FP64 direct sums."""
+    with the relative force error on `samples` random bodies against FP64 direct
+    sums.  variants: (label, ratio, learned) -- learned runs train the correction
+    first on another Plummer realisation of the same size (seed 5)."""
+    import tempfile
+
     import numpy as np
 
-    from nbody_amd import native, nbody_create_engine, nbody_create_solver, nbody_data
+    from nbody_amd import correction, native, nbody_create_engine, nbody_create_solver, nbody_data
     hip = native.hip_lib()
     data = nbody_data()
     data.set_quiet(True)
@@ -77,9 +81,26 @@ def bh_bench(n: int, steps: int, warmup: int, ratios, samples: int = 8192) -> di
     out = {"workload": f"Barnes-Hut heap kd-tree, group walk, FP32 forces / FP64 state, Plummer N={n} "
                        f"(a=1, M=1, seed 4), euler dt=1e-4, tree rebuilt every step", "n": n, "runs": []}
     idx = np.sort(np.random.default_rng(0).choice(n, samples, replace=False)).astype(np.uint32)
-    for ratio in ratios:
-        e = nbody_create_engine({"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
-                                 "distance_to_node_radius_ratio": ratio, "tree_layout": "heap_stackless"})
+    tmpdir = tempfile.mkdtemp(prefix="nbh_corr_")
+    for label, ratio, learned in variants:
+        params = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
+                  "distance_to_node_radius_ratio": ratio, "tree_layout": "heap_stackless"}
+        train_info = None
+        if learned:
+            tdata = nbody_data()
+            tdata.set_quiet(True)
+            tdata.generate("plummer", n, 5)
+            tp, tv, tm, _ = tdata.bodies()
+            ty = np.concatenate([tp[:, 0], tp[:, 1], tp[:, 2], tv[:, 0], tv[:, 1], tv[:, 2]])
+            log(f"BH: training the correction at ratio {ratio}")
+            t0 = time.perf_counter()
+            corr = correction.train(ty, tm, ratio, rho_min=0.0, near_k=2.0, max_samples=2_000_000, epochs=20)
+            train_info = dict(corr.stats, seconds=time.perf_counter() - t0, rho_min=corr.rho_min,
+                              near_k=corr.near_k, train_state="Plummer seed 5")
+            path = os.path.join(tmpdir, f"corr_{ratio:g}.nbhmlp")
+            correction.save(path, corr)
+            params["learned_correction"] = path
+        e = nbody_create_engine(params)
         data.set_time(0.0)
         data.set_step(0)
         if e is None or not e.init(data):
@@ -101,8 +122,8 @@ def bh_bench(n: int, steps: int, warmup: int, ratios, samples: int = 8192) -> di
         native.check(hip.nbh_timing_read(ctx, b"tree_build", C.byref(tb), C.byref(cb)))
         native.check(hip.nbh_timing_read(ctx, b"bh_traverse", C.byref(tw), C.byref(cw)))
         native.check(hip.nbh_timing_enable(ctx, 0))
-        inter = C.c_uint64()
-        native.check(hip.nbh_bh_counters(ctx, e.hip_tree(), C.byref(inter), None))
+        cnt = (C.c_uint64 * 5)()
+        native.check(hip.nbh_bh_walk_counters(ctx, e.hip_tree(), cnt))
         # error of the last force evaluation's state: recompute f, compare sampled bodies
         y = e.get_y()
         f = e.create_buffer(48 * n)
@@ -120,17 +141,86 @@ def bh_bench(n: int, steps: int, warmup: int, ratios, samples: int = 8192) -> di
         a0 = e.read_buffer(da).reshape(-1, 3)
         rel = np.linalg.norm(fa - a0, axis=1) / np.linalg.norm(a0, axis=1)
         e.free_buffers([f, dm, di, da])
-        out["runs"].append({
-            "ratio": ratio, "theta": 1.0 / ratio, "steps_per_s": steps / wall, "ms_per_step": wall / steps * 1e3,
-            "tree_build_ms": tb.value / max(1, cb.value), "walk_ms": tw.value / max(1, cw.value),
-            "interactions_per_body": inter.value / n,
+        run = {
+            "variant": label, "ratio": ratio, "theta": 1.0 / ratio, "steps_per_s": steps / wall,
+            "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
+            "walk_ms": tw.value / max(1, cw.value), "interactions_per_body": cnt[0] / n,
             "force_rel_err": {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
                               "max": float(rel.max()), "samples": samples},
-        })
-        log(f"BH ratio {ratio}: {steps / wall:.2f} steps/s, rms err {out['runs'][-1]['force_rel_err']['rms']:.2e}")
+        }
+        if learned:
+            run["far_cells_per_group"] = cnt[3] / (n // 64)
+            run["near_cells_per_group"] = cnt[4] / (n // 64)
+            run["training"] = train_info
+        out["runs"].append(run)
+        log(f"BH {label}: {steps / wall:.2f} steps/s, rms err {run['force_rel_err']['rms']:.2e}")
         s.destroy()
         e.destroy()
+    for name in os.listdir(tmpdir):
+        os.remove(os.path.join(tmpdir, name))
+    os.rmdir(tmpdir)
     return out
+
+
+def pmc_probe(n: int) -> int:
+    """Launches for the hardware-counter runs: the FP32 direct kernel three times
+    on the benchmark state, then a plain streaming kernel of known traffic
+    (nbh_fmaddn a = b + c over 2^26 doubles: 1 GiB read, 512 MiB written) three
+    times, so FETCH_SIZE / WRITE_SIZE of the first can be scaled by what the
+    counters read for the second."""
+    import numpy as np
+
+    from nbody_amd import native, nbody_data
+    hip = native.hip_lib()
+    data = nbody_data()
+    data.set_quiet(True)
+    data.generate("plummer", n, 2)
+    pos, vel, mass, _ = data.bodies()
+    y = np.ascontiguousarray(np.concatenate([pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1], vel[:, 2]]))
+    ctx = C.c_void_p()
+    native.check(hip.nbh_ctx_create(0, C.byref(ctx)))
+    bufs = []
+
+    def dev(nbytes):
+        ptr = C.c_void_p()
+        native.check(hip.nbh_malloc(ctx, nbytes, C.byref(ptr)))
+        bufs.append(ptr)
+        return ptr
+
+    dy, dm, df = dev(y.nbytes), dev(mass.nbytes), dev(y.nbytes)
+    native.check(hip.nbh_memcpy_h2d(ctx, dy, y.ctypes.data_as(C.c_void_p), y.nbytes))
+    native.check(hip.nbh_memcpy_h2d(ctx, dm, mass.ctypes.data_as(C.c_void_p), mass.nbytes))
+    for _ in range(3):
+        native.check(hip.nbh_fcompute_direct(ctx, dy, dm, df, n, 1, 0))
+    native.check(hip.nbh_ctx_synchronize(ctx))
+    log("direct launches done")
+    m = 1 << 26
+    a, b, c = dev(8 * m), dev(8 * m), dev(8 * m)
+    native.check(hip.nbh_fill(ctx, b, 1.0, m))
+    native.check(hip.nbh_fill(ctx, c, 2.0, m))
+    srcs = (C.c_void_p * 1)(c.value)
+    coef = (C.c_double * 1)(1.0)
+    for _ in range(3):
+        native.check(hip.nbh_fmaddn(ctx, a, b, srcs, coef, 1, m))
+    native.check(hip.nbh_ctx_synchronize(ctx))
+    log("copy launches done")
+    for ptr in bufs:
+        native.check(hip.nbh_free(ctx, ptr))
+    native.check(hip.nbh_ctx_destroy(ctx))
+    return 0
+
+
+def traffic_from_profile(n: int):
+    """HBM bytes per launch of the FP32 direct kernel at this N, from the counter
+    summary in profiles/ (line 'traffic_bytes_per_launch N=<n>: <bytes>'), or None."""
+    path = ROOT / "profiles" / f"direct_fp32_plummer_{n}_traffic.md"
+    if not path.exists():
+        return None
+    key = f"traffic_bytes_per_launch N={n}:"
+    for line in path.read_text().splitlines():
+        if line.strip().startswith(key):
+            return float(line.split(":", 1)[1].strip())
+    return None
 
 
 def main(argv=None) -> int:
@@ -146,6 +236,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-bh", action="store_true", help="skip the Barnes-Hut steps/s block")
     ap.add_argument("--bh-n", type=int, default=1 << 22)
     ap.add_argument("--bh-steps", type=int, default=5)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction Barnes-Hut run")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only launch the kernels measured by the hardware-counter runs, then exit")
     a = ap.parse_args(argv)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -161,6 +254,8 @@ def main(argv=None) -> int:
     if nbody_amd.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 2
+    if a.pmc_probe:
+        return pmc_probe(a.n)
     hip = native.hip_lib()
     n = a.n
     log(f"generating Plummer N={n}")
@@ -255,7 +350,7 @@ def main(argv=None) -> int:
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": traffic_from_profile(n) if a.precision == "fp32" else None,
             "avg_kernel_ms": avg_ms,
             "launches": cnt.value,
             "pairs_per_s_kernel": n * n / (avg_ms * 1e-3),
@@ -270,7 +365,10 @@ def main(argv=None) -> int:
         log(f"cpu baseline ({threads} threads, ~{a.cpu_seconds:g} s)")
         out["cpu_baseline"] = cpu_baseline(yh, mass, n, a.cpu_seconds, threads)
     if not a.no_bh:
-        out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, (2.0, 1.25))
+        variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False)]
+        if not a.no_learned:
+            variants.append(("learned theta=0.8", 1.25, True))
+        out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)
     print(json.dumps(out), flush=True)
     return 0
 
