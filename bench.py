@@ -65,7 +65,8 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
     """Barnes-Hut euler steps/s (one fcompute = tree build + group walk, plus one
     fmadd_inplace, as the reference bench's step) on a Plummer sphere of n bodies,
     with the relative force error on `samples` random bodies against FP64 direct
-    sums.  variants: (label, ratio, learned) -- learned runs train the correction
+    sums.  variants: (label, ratio, kind) -- kind False: monopole walk;
+    "quadrupole": monopole + quadrupole walk; True: learned correction, trained
     first on another Plummer realisation of the same size (seed 5)."""
     import tempfile
 
@@ -85,6 +86,9 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
     for label, ratio, learned in variants:
         params = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
                   "distance_to_node_radius_ratio": ratio, "tree_layout": "heap_stackless"}
+        if learned == "quadrupole":
+            params["bh_multipole"] = "quadrupole"
+            learned = False
         train_info = None
         if learned:
             tdata = nbody_data()
@@ -142,7 +146,8 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
         rel = np.linalg.norm(fa - a0, axis=1) / np.linalg.norm(a0, axis=1)
         e.free_buffers([f, dm, di, da])
         run = {
-            "variant": label, "ratio": ratio, "theta": 1.0 / ratio, "steps_per_s": steps / wall,
+            "variant": label, "ratio": ratio, "theta": 1.0 / ratio,
+            "multipole": params.get("bh_multipole", "monopole"), "steps_per_s": steps / wall,
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
             "walk_ms": tw.value / max(1, cw.value), "interactions_per_body": cnt[0] / n,
             "force_rel_err": {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
@@ -365,7 +370,9 @@ def main(argv=None) -> int:
         log(f"cpu baseline ({threads} threads, ~{a.cpu_seconds:g} s)")
         out["cpu_baseline"] = cpu_baseline(yh, mass, n, a.cpu_seconds, threads)
     if not a.no_bh:
-        variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False)]
+        variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False),
+                    ("plain theta=0.25", 4.0, False), ("quadrupole theta=0.5", 2.0, "quadrupole"),
+                    ("quadrupole theta=0.8", 1.25, "quadrupole")]
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))
         out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)

@@ -137,6 +137,12 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* tree, const double* y,
    kfcompute_heap_bh_stackless (nbody_engine_cuda_impl.cu:372-451) on the fast
    path; n >= 64. */
 nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f);
+/* nbh_bh_fcompute_group plus the quadrupole term of every internal cell taken as
+   a monopole (second central moments of the cell about its mass centre, kept
+   per node by nbh_tree_build / nbh_tree_refit); FP32.  Not in the reference,
+   whose walks are monopole only: an option that lowers the error at a given
+   opening ratio. */
+nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f);
 /* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may
    be NULL.  cm/bmin/bmax are xyz-interleaved [2n*3]; body_n holds -1 on
    internal nodes.  Used to check the layout against the reference heap

@@ -37,7 +37,8 @@ constexpr int kSortTile = kSortThreads * kSortItems;   // 1024 keys per work-gro
 constexpr int kPartThreads = 256;
 constexpr int kPartItems = 4;
 constexpr int kPartTile = kPartThreads * kPartItems;   // 1024 entries per work-group
-constexpr int kScanThreads = 1024;
+constexpr int kScanThreads = 256;
+constexpr int kScanParts = 1024;                     // chunks of one scan
 
 }  // namespace
 
@@ -52,6 +53,7 @@ struct nbh_tree {
 	uint64_t* keys = nullptr;      // [2][n] radix sort ping-pong
 	uint32_t* vals = nullptr;      // [2][n]
 	uint32_t* counts = nullptr;    // radix / partition block counts
+	uint32_t* scan_part = nullptr; // [kScanParts] chunk sums of a scan
 	// nodes, heap index 0..2n-1 (0 unused)
 	double4*  xyzr = nullptr;      // cm.x, cm.y, cm.z, crit_r2
 	double*   mass = nullptr;
@@ -106,31 +108,82 @@ k_radix_hist(const uint64_t* __restrict__ keys, size_t n, int shift, uint32_t* _
 	counts[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan in place of `m` uint32 counts by one work-group.
+// Exclusive scan in place of `m` uint32 counts: chunk sums, a scan of the (at
+// most kScanParts) chunk sums, then a scan of every chunk from its offset.
 __global__ void __launch_bounds__(kScanThreads)
-k_scan_single(uint32_t* __restrict__ c, size_t m)
+k_scan_reduce(const uint32_t* __restrict__ c, size_t m, size_t chunk, uint32_t* __restrict__ part)
 {
-	__shared__ uint32_t sh[kScanThreads];
-	size_t per = (m + kScanThreads - 1) / kScanThreads;
-	size_t b = threadIdx.x * per;
-	size_t e = b + per < m ? b + per : m;
+	__shared__ uint32_t ws[kScanThreads / 64];
+	const size_t b0 = static_cast<size_t>(blockIdx.x) * chunk;
+	const size_t b1 = b0 + chunk < m ? b0 + chunk : m;
 	uint32_t s = 0;
-	for(size_t i = b; i < e; ++i) { s += c[i]; }
-	sh[threadIdx.x] = s;
+	for(size_t i = b0 + threadIdx.x; i < b1; i += kScanThreads) { s += c[i]; }
+	for(int off = 32; off > 0; off >>= 1) { s += __shfl_down(s, off, 64); }
+	if((threadIdx.x & 63) == 0) { ws[threadIdx.x >> 6] = s; }
 	__syncthreads();
-	for(int off = 1; off < kScanThreads; off <<= 1)
+	if(threadIdx.x == 0)
 	{
-		uint32_t v = threadIdx.x >= static_cast<unsigned>(off) ? sh[threadIdx.x - off] : 0;
+		uint32_t t = 0;
+		for(int w = 0; w < kScanThreads / 64; ++w) { t += ws[w]; }
+		part[blockIdx.x] = t;
+	}
+}
+
+__global__ void __launch_bounds__(kScanParts)
+k_scan_parts(uint32_t* __restrict__ part, int g)
+{
+	__shared__ uint32_t sh[kScanParts];
+	const uint32_t v = static_cast<int>(threadIdx.x) < g ? part[threadIdx.x] : 0;
+	sh[threadIdx.x] = v;
+	__syncthreads();
+	for(int off = 1; off < kScanParts; off <<= 1)
+	{
+		const uint32_t u = threadIdx.x >= static_cast<unsigned>(off) ? sh[threadIdx.x - off] : 0;
 		__syncthreads();
-		sh[threadIdx.x] += v;
+		sh[threadIdx.x] += u;
 		__syncthreads();
 	}
-	uint32_t run = threadIdx.x == 0 ? 0 : sh[threadIdx.x - 1];
-	for(size_t i = b; i < e; ++i)
+	if(static_cast<int>(threadIdx.x) < g) { part[threadIdx.x] = sh[threadIdx.x] - v; }
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_apply(uint32_t* __restrict__ c, size_t m, size_t chunk, const uint32_t* __restrict__ part)
+{
+	constexpr int kWaves = kScanThreads / 64;
+	__shared__ uint32_t ws[kWaves];
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	uint32_t carry = part[blockIdx.x];
+	const size_t b0 = static_cast<size_t>(blockIdx.x) * chunk;
+	const size_t b1 = b0 + chunk < m ? b0 + chunk : m;
+	for(size_t t0 = b0; t0 < b1; t0 += 4 * kScanThreads)
 	{
-		uint32_t v = c[i];
-		c[i] = run;
-		run += v;
+		const size_t i0 = t0 + 4 * static_cast<size_t>(threadIdx.x);
+		uint32_t v[4];
+		for(int k = 0; k < 4; ++k) { v[k] = i0 + k < b1 ? c[i0 + k] : 0; }
+		const uint32_t tsum = v[0] + v[1] + v[2] + v[3];
+		uint32_t x = tsum;
+		for(int off = 1; off < 64; off <<= 1)
+		{
+			const uint32_t u = __shfl_up(x, off, 64);
+			if(lane >= off) { x += u; }
+		}
+		if(lane == 63) { ws[wave] = x; }
+		__syncthreads();
+		uint32_t wbase = 0, total = 0;
+		for(int w = 0; w < kWaves; ++w)
+		{
+			wbase += w < wave ? ws[w] : 0;
+			total += ws[w];
+		}
+		uint32_t run = carry + wbase + x - tsum;
+		for(int k = 0; k < 4; ++k)
+		{
+			if(i0 + k < b1) { c[i0 + k] = run; }
+			run += v[k];
+		}
+		carry += total;
+		__syncthreads();
 	}
 }
 
@@ -224,28 +277,12 @@ k_part_count(const uint32_t* __restrict__ list, const uint8_t* __restrict__ side
 	}
 }
 
-// Exclusive scan of the tile counts, restarting every `per_seg` tiles.
-__global__ void __launch_bounds__(256)
-k_part_scan(uint32_t* __restrict__ rights, size_t ntiles, size_t per_seg)
-{
-	size_t nseg = ntiles / per_seg;
-	for(size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < nseg;
-	    s += static_cast<size_t>(gridDim.x) * blockDim.x)
-	{
-		uint32_t run = 0;
-		for(size_t b = s * per_seg; b < (s + 1) * per_seg; ++b)
-		{
-			uint32_t v = rights[b];
-			rights[b] = run;
-			run += v;
-		}
-	}
-}
-
 // Stable partition of every segment of `list` into lower-half then upper-half
 // bodies (flags in `side`), written to `out`.  `tile_off` holds, per tile, the
-// number of upper-half bodies before the tile inside its segment (segments of
-// at least one tile); smaller segments are handled inside the tile.
+// number of upper-half bodies in all tiles before it (exclusive scan over the
+// whole list); for segments of at least one tile the count inside the segment
+// is that minus the value at the segment's first tile.  Smaller segments are
+// handled inside the tile.
 __global__ void __launch_bounds__(kPartThreads)
 k_part_scatter(const uint32_t* __restrict__ list, const uint8_t* __restrict__ side, size_t n, size_t seg,
                const uint32_t* __restrict__ tile_off, uint32_t* __restrict__ out)
@@ -286,7 +323,7 @@ k_part_scatter(const uint32_t* __restrict__ list, const uint8_t* __restrict__ si
 		uint32_t before;
 		if(seg >= static_cast<size_t>(kPartTile))
 		{
-			before = tile_off[blockIdx.x] + excl[tb + k];
+			before = tile_off[blockIdx.x] - tile_off[seg_start / kPartTile] + excl[tb + k];
 		}
 		else
 		{
@@ -652,13 +689,54 @@ __device__ inline void sweep32(const float4* __restrict__ buf, int count, float 
 	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
 }
 
+constexpr int kQuadBuf = 128;    // cells waiting for the quadrupole sweep
+
+// Quadrupole field of cells at the lane's body: entries of 3 float4
+// (cm offset from the group centre, tr Q), (Qxx Qyy Qzz Qxy), (Qxz Qyz - -):
+// a = -3 Q r / d^5 + (7.5 r.Q.r / d^2 - 1.5 tr Q) r / d^5, r = cm - x.
+__device__ inline void sweep_quad(const float4* __restrict__ qb, int count, float tx, float ty, float tz,
+                                  float& qx, float& qy, float& qz)
+{
+	for(int j = 0; j < count; ++j)
+	{
+		const float4 e0 = qb[3 * j], e1 = qb[3 * j + 1], e2 = qb[3 * j + 2];
+		const float rx = e0.x - tx, ry = e0.y - ty, rz = e0.z - tz;
+		const float r2 = fmaxf(fmaf(rz, rz, fmaf(ry, ry, rx * rx)), 1e-8f);
+		const float ri = __builtin_amdgcn_rsqf(r2);
+		const float ri2 = ri * ri;
+		const float ri5 = ri2 * ri2 * ri;
+		const float Qx = fmaf(e1.x, rx, fmaf(e1.w, ry, e2.x * rz));
+		const float Qy = fmaf(e1.w, rx, fmaf(e1.y, ry, e2.y * rz));
+		const float Qz = fmaf(e2.x, rx, fmaf(e2.y, ry, e1.z * rz));
+		const float rqr = fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz));
+		const float c = fmaf(7.5f * rqr, ri2, -1.5f * e0.w);
+		qx = fmaf(fmaf(-3.0f, Qx, c * rx), ri5, qx);
+		qy = fmaf(fmaf(-3.0f, Qy, c * ry), ri5, qy);
+		qz = fmaf(fmaf(-3.0f, Qz, c * rz), ri5, qz);
+	}
+}
+
+// Quadrupole entry of node `node` (mass centre offset ent.xyz from the group
+// centre) at slot `slot` of qb.
+__device__ inline void put_quad(float4* qb, int slot, const float4& ent, const double* __restrict__ Q)
+{
+	qb[3 * slot] = make_float4(ent.x, ent.y, ent.z, static_cast<float>(Q[0] + Q[1] + Q[2]));
+	qb[3 * slot + 1] = make_float4(static_cast<float>(Q[0]), static_cast<float>(Q[1]), static_cast<float>(Q[2]),
+	                               static_cast<float>(Q[3]));
+	qb[3 * slot + 2] = make_float4(static_cast<float>(Q[4]), static_cast<float>(Q[5]), 0.0f, 0.0f);
+}
+
+// kQuad: internal cells taken as monopoles also contribute their quadrupole
+// term (second central moments `quad`) to every body of the group.
+template <bool kQuad>
 __global__ void __launch_bounds__(kGroup)
 k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
-             const double* __restrict__ bmax, const int32_t* __restrict__ body_n, const double* __restrict__ y,
-             double* __restrict__ f, int n, unsigned long long* __restrict__ counters)
+             const double* __restrict__ bmax, const double* __restrict__ quad, const int32_t* __restrict__ body_n,
+             const double* __restrict__ y, double* __restrict__ f, int n, unsigned long long* __restrict__ counters)
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
+	__shared__ float4 qbuf[kQuad ? 3 * kQuadBuf : 1];
 	const int lane = threadIdx.x;
 	const int g = n / kGroup + blockIdx.x;              // heap node of this group
 	const double gx0 = bmin[3 * g], gy0 = bmin[3 * g + 1], gz0 = bmin[3 * g + 2];
@@ -670,11 +748,13 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	const float ty = static_cast<float>(me.y - ccy);
 	const float tz = static_cast<float>(me.z - ccz);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+	float qx = 0.0f, qy = 0.0f, qz = 0.0f;
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	unsigned long long vis = 0, taken = 0, spill = 0;
 
 	int sp = 1;
 	int nbuf = 0;
+	int nq = 0;
 	if(lane == 0) { stack[0] = kRoot; }
 	__syncthreads();
 	while(sp > 0)
@@ -721,6 +801,13 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		{
 			buf[nbuf + __popcll(bacc & lt)] = ent;
 		}
+		if constexpr(kQuad)
+		{
+			const bool cell = accept && node < n;
+			const uint64_t bcell = __ballot(cell);
+			if(cell) { put_quad(qbuf, nq + __popcll(bcell & lt), ent, quad + 6 * node); }
+			nq += __popcll(bcell);
+		}
 		if(open)
 		{
 			const int r = __popcll(bopen2 & lt);
@@ -738,18 +825,31 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			nbuf = 0;
 			__syncthreads();
 		}
+		if constexpr(kQuad)
+		{
+			if(nq > kQuadBuf - kGroup)
+			{
+				sweep_quad(qbuf, nq, tx, ty, tz, qx, qy, qz);
+				nq = 0;
+				__syncthreads();
+			}
+		}
 	}
 	if(nbuf > 0)
 	{
 		sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
 	}
+	if constexpr(kQuad)
+	{
+		if(nq > 0) { sweep_quad(qbuf, nq, tx, ty, tz, qx, qy, qz); }
+	}
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
 	f[n + b] = y[4 * n + b];
 	f[2 * n + b] = y[5 * n + b];
-	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx);
-	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy);
-	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz);
+	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx) + static_cast<double>(qx);
+	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy) + static_cast<double>(qy);
+	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz) + static_cast<double>(qz);
 	// counters: [0] pair interactions (entries x 64 targets), [1] node visits, [2] spills
 	for(int off = 32; off > 0; off >>= 1)
 	{
@@ -946,33 +1046,6 @@ k_band_residual(const double4* __restrict__ xyzr, const double* __restrict__ nma
 }
 
 constexpr int kBand = 80;        // far-cell rows waiting for the network (< 16 + 64)
-constexpr int kQuadBuf = 128;    // near cells waiting for the quadrupole sweep
-
-// Quadrupole field of near cells at the lane's body: entries of 3 float4
-// (cm offset from the group centre, tr Q), (Qxx Qyy Qzz Qxy), (Qxz Qyz - -):
-// a = -3 Q r / d^5 + (7.5 r.Q.r / d^2 - 1.5 tr Q) r / d^5, r = cm - x.
-__device__ inline void sweep_quad(const float4* __restrict__ qb, int count, float tx, float ty, float tz,
-                                  float& qx, float& qy, float& qz)
-{
-	for(int j = 0; j < count; ++j)
-	{
-		const float4 e0 = qb[3 * j], e1 = qb[3 * j + 1], e2 = qb[3 * j + 2];
-		const float rx = e0.x - tx, ry = e0.y - ty, rz = e0.z - tz;
-		const float r2 = fmaxf(fmaf(rz, rz, fmaf(ry, ry, rx * rx)), 1e-8f);
-		const float ri = __builtin_amdgcn_rsqf(r2);
-		const float ri2 = ri * ri;
-		const float ri5 = ri2 * ri2 * ri;
-		const float Qx = fmaf(e1.x, rx, fmaf(e1.w, ry, e2.x * rz));
-		const float Qy = fmaf(e1.w, rx, fmaf(e1.y, ry, e2.y * rz));
-		const float Qz = fmaf(e2.x, rx, fmaf(e2.y, ry, e1.z * rz));
-		const float rqr = fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz));
-		const float c = fmaf(7.5f * rqr, ri2, -1.5f * e0.w);
-		qx = fmaf(fmaf(-3.0f, Qx, c * rx), ri5, qx);
-		qy = fmaf(fmaf(-3.0f, Qy, c * ry), ri5, qy);
-		qz = fmaf(fmaf(-3.0f, Qz, c * rz), ri5, qz);
-	}
-}
-
 // k_bh_group32 plus the correction.  Every accepted cell still enters the
 // monopole sweep.  Near cells also enter the quadrupole sweep; far cells are
 // fed to the network in tiles of 16, and the predicted residual field of each,
@@ -1127,12 +1200,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		}
 		if(isnear)
 		{
-			const int slot = nq + __popcll(bnear & lt);
-			const double* Q = quad + 6 * node;
-			qbuf[3 * slot] = make_float4(ent.x, ent.y, ent.z, static_cast<float>(Q[0] + Q[1] + Q[2]));
-			qbuf[3 * slot + 1] = make_float4(static_cast<float>(Q[0]), static_cast<float>(Q[1]),
-			                                 static_cast<float>(Q[2]), static_cast<float>(Q[3]));
-			qbuf[3 * slot + 2] = make_float4(static_cast<float>(Q[4]), static_cast<float>(Q[5]), 0.0f, 0.0f);
+			put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node);
 		}
 		nbuf += __popcll(bacc);
 		sp += 2 * __popcll(bopen);
@@ -1203,6 +1271,20 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
+nbh_status scan_exclusive(nbh_ctx* ctx, nbh_tree* t, uint32_t* c, size_t m)
+{
+	size_t chunk = (m + kScanParts - 1) / kScanParts;
+	chunk = (chunk + 4 * kScanThreads - 1) / (4 * kScanThreads) * (4 * kScanThreads);
+	const unsigned g = static_cast<unsigned>((m + chunk - 1) / chunk);
+	hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(kScanThreads), 0, ctx->stream, c, m, chunk, t->scan_part);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanParts), 0, ctx->stream, t->scan_part, static_cast<int>(g));
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(kScanThreads), 0, ctx->stream, c, m, chunk, t->scan_part);
+	NBH_LAUNCHED();
+	return NBH_OK;
+}
+
 nbh_status radix_sort(nbh_ctx* ctx, nbh_tree* t, const double* coord, uint32_t* out_vals)
 {
 	const size_t n = t->n;
@@ -1218,9 +1300,7 @@ nbh_status radix_sort(nbh_ctx* ctx, nbh_tree* t, const double* coord, uint32_t* 
 		hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, n, shift,
 		                   t->counts, nblocks);
 		NBH_LAUNCHED();
-		hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, ctx->stream, t->counts,
-		                   static_cast<size_t>(nblocks) * 256);
-		NBH_LAUNCHED();
+		NBH_TRY(scan_exclusive(ctx, t, t->counts, static_cast<size_t>(nblocks) * 256));
 		hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, v0, k1, v1, n,
 		                   shift, t->counts, nblocks);
 		NBH_LAUNCHED();
@@ -1282,6 +1362,7 @@ nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 	alloc(reinterpret_cast<void**>(&t->keys), 2 * n * sizeof(uint64_t));
 	alloc(reinterpret_cast<void**>(&t->vals), 2 * n * sizeof(uint32_t));
 	alloc(reinterpret_cast<void**>(&t->counts), ncounts * sizeof(uint32_t));
+	alloc(reinterpret_cast<void**>(&t->scan_part), kScanParts * sizeof(uint32_t));
 	alloc(reinterpret_cast<void**>(&t->xyzr), 2 * n * sizeof(double4));
 	alloc(reinterpret_cast<void**>(&t->mass), 2 * n * sizeof(double));
 	alloc(reinterpret_cast<void**>(&t->bmin), 2 * n * 3 * sizeof(double));
@@ -1315,7 +1396,7 @@ nbh_status nbh_tree_destroy(nbh_tree* t)
 		(void)hipSetDevice(t->ctx->device);
 		(void)hipStreamSynchronize(t->ctx->stream);
 	}
-	void* ptrs[] = { t->list[0], t->list[1], t->list[2], t->tmp, t->side, t->keys, t->vals, t->counts,
+	void* ptrs[] = { t->list[0], t->list[1], t->list[2], t->tmp, t->side, t->keys, t->vals, t->counts, t->scan_part,
 	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->body_n, t->counters };
 	for(void* p : ptrs)
 	{
@@ -1352,11 +1433,7 @@ nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* t, const double* y, const doub
 				hipLaunchKernelGGL(k_part_count, dim3(static_cast<unsigned>(ntiles)), dim3(kPartThreads), 0,
 				                   ctx->stream, t->list[b], t->side, n, t->counts);
 				NBH_LAUNCHED();
-				size_t per_seg = seg / kPartTile;
-				size_t nseg = ntiles / per_seg;
-				hipLaunchKernelGGL(k_part_scan, dim3(nbh::grid_for(ctx, nseg, 256)), dim3(256), 0, ctx->stream,
-				                   t->counts, ntiles, per_seg);
-				NBH_LAUNCHED();
+				NBH_TRY(scan_exclusive(ctx, t, t->counts, ntiles));
 			}
 			hipLaunchKernelGGL(k_part_scatter, dim3(static_cast<unsigned>(ntiles)), dim3(kPartThreads), 0,
 			                   ctx->stream, t->list[b], t->side, n, seg, t->counts, t->tmp);
@@ -1413,7 +1490,7 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* t, const double* y, con
 	return NBH_OK;
 }
 
-nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f)
+static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, bool quadrupole)
 {
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
@@ -1425,10 +1502,28 @@ nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* t, const double* 
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
 	NBH_LAUNCHED();
 	nbh::scoped_timer timer(ctx, "bh_traverse");
-	hipLaunchKernelGGL(k_bh_group32, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass, t->bmin,
-	                   t->bmax, t->body_n, y, f, n, t->counters);
+	if(quadrupole)
+	{
+		hipLaunchKernelGGL(k_bh_group32<true>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass,
+		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, t->counters);
+	}
+	else
+	{
+		hipLaunchKernelGGL(k_bh_group32<false>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass,
+		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, t->counters);
+	}
 	NBH_LAUNCHED();
 	return NBH_OK;
+}
+
+nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f)
+{
+	return group_walk(ctx, t, y, f, false);
+}
+
+nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f)
+{
+	return group_walk(ctx, t, y, f, true);
 }
 
 nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* groups, size_t n_groups,

@@ -641,6 +641,7 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_tree(nullptr),
 	m_tree_built(false),
 	m_group_walk(false),
+	m_quadrupole(false),
 	m_net_rho(0.0f),
 	m_net_near(0.0f),
 	m_net(nullptr)
@@ -777,6 +778,11 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 			      "bh fcompute (group walk, learned correction)");
 			return;
 		}
+		if(m_quadrupole)
+		{
+			check(nbh_bh_fcompute_group_quad(m_ctx, m_tree, y->data(), f->data()), "bh fcompute (quadrupole walk)");
+			return;
+		}
 		check(nbh_bh_fcompute_group(m_ctx, m_tree, y->data(), f->data()), "bh fcompute (group walk)");
 		return;
 	}
@@ -793,6 +799,7 @@ void nbody_engine_hip_bh::print_info() const
 	nbody_print("\ttree_layout: %s", tree_layout_name(m_tree_layout));
 	nbody_print("\ttree_build_rate: %zu", m_tree_build_rate);
 	nbody_print("\tbh_walk: %s", m_group_walk ? "group" : "body");
+	nbody_print("\tbh_multipole: %s", m_quadrupole ? "quadrupole" : "monopole");
 	nbody_print("\tlearned_correction: %s", m_net_params.empty() ? "none" : m_net_path.c_str());
 	if(!m_net_params.empty())
 	{

@@ -116,6 +116,7 @@ class nbody_engine_hip_bh : public nbody_engine_hip
 	nbh_tree*       m_tree;
 	bool            m_tree_built;
 	bool            m_group_walk;
+	bool            m_quadrupole;
 	std::vector<float> m_net_params;   // learned correction (empty: none)
 	float           m_net_rho;
 	float           m_net_near;
@@ -126,6 +127,8 @@ public:
 	                    size_t tree_build_rate, e_tree_layout tl);
 	//! Use the shared 64-body group walk (FP32 only) instead of one walk per body.
 	void set_group_walk(bool g) { m_group_walk = g; }
+	//! Add the quadrupole term of taken cells to the group walk.
+	void set_quadrupole(bool q) { m_quadrupole = q; }
 	//! Read a correction network file ("NBHMLP01", int32 parameter count,
 	//! float32 rho_min, float32 near_k, float32 parameters); rho >= 0 and
 	//! near_k >= 0 override the thresholds stored in it.  False if the file is

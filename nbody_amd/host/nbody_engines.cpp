@@ -109,6 +109,17 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("bh_walk=group needs force_precision=fp32");
 			return nullptr;
 		}
+		std::string multipole = param.value("bh_multipole", "monopole");
+		if(multipole != "monopole" && multipole != "quadrupole")
+		{
+			nbody_log("Invalid bh_multipole. Allowed values are 'monopole' or 'quadrupole'");
+			return nullptr;
+		}
+		if(multipole == "quadrupole" && walk != "group")
+		{
+			nbody_log("bh_multipole=quadrupole needs bh_walk=group");
+			return nullptr;
+		}
 		// learned correction of the group walk (no reference counterpart)
 		std::string net = param.value("learned_correction", "none");
 		double rho = -1.0;
@@ -123,13 +134,14 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("Invalid correction_near");
 			return nullptr;
 		}
-		if(net != "none" && walk != "group")
+		if(net != "none" && (walk != "group" || multipole != "monopole"))
 		{
-			nbody_log("learned_correction needs bh_walk=group");
+			nbody_log("learned_correction needs bh_walk=group and bh_multipole=monopole");
 			return nullptr;
 		}
 		nbody_engine_hip_bh* engine = new nbody_engine_hip_bh(ratio, tt, static_cast<size_t>(rate), tl);
 		engine->set_group_walk(walk == "group");
+		engine->set_quadrupole(multipole == "quadrupole");
 		if(net != "none" && !engine->load_correction(net, rho, near_k))
 		{
 			delete engine;

@@ -351,6 +351,29 @@ def test_near_quadrupole_matches_oracle(ctx):
     t.close()
 
 
+def test_quadrupole_walk_equals_all_near_cells(ctx):
+    """nbh_bh_fcompute_group_quad == the corrected walk with every taken cell
+    near and a zero network (same cells, same order, same arithmetic), and it
+    is more accurate than the monopole walk at the same ratio."""
+    n, ratio = 1 << 14, 2.0
+    y, m = _state("plummer", n, 25)
+    t = Tree(ctx, y, m, ratio)
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group_quad(ctx.h, t.h, t.dy.ptr, df.ptr), "quad")
+    quad = df.read()
+    assert t.counters()[2] == 0                 # no stack spills: identical cell sets
+    net = Net(ctx)
+    net.set(np.zeros(mo.N_PARAMS, F32))
+    ref = t.group_mlp(net, 0.0, 1e30)
+    assert np.array_equal(quad, ref)
+    idx = np.random.default_rng(4).choice(n, 1024, replace=False)
+    a0 = _sample_accel(ctx, t.dy, t.dm, n, idx)
+    e_mono, e_quad = _rms_rel(t.group(), a0, n, idx), _rms_rel(quad, a0, n, idx)
+    assert e_quad[0] < 0.5 * e_mono[0], (e_quad, e_mono)
+    net.close()
+    t.close()
+
+
 def test_corrected_walk_matches_oracle(ctx):
     """Correction added by the kernel == the oracle's near-cell quadrupoles plus
     its network evaluated on the oracle's far cells, expanded to first order
