@@ -9,6 +9,7 @@ libnbody.so; buffers are numpy float64 arrays on the host side.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -183,10 +184,15 @@ class nbody_engine:
         self._lib = native.host_lib()
         self.handle = handle
         self._data = None
+        self._solvers = weakref.WeakSet()
 
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:
+            # a solver frees its buffers through its engine: destroy attached
+            # solvers first (garbage collection may finalise them in any order)
+            for s in list(getattr(self, "_solvers", ())):
+                s.destroy()
             self._lib.nbs_engine_destroy(h)
             self.handle = None
 
@@ -342,6 +348,7 @@ class nbody_solver:
 
     def set_engine(self, e: nbody_engine) -> None:
         self._engine = e
+        e._solvers.add(self)
         self._lib.nbs_solver_set_engine(self.handle, e.handle)
 
     def set_time_step(self, min_step: float, max_step: float) -> None:

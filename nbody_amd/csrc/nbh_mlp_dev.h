@@ -182,39 +182,39 @@ __device__ inline f32x4 forward_tile(const bf16x4& x, const uint2* __restrict__ 
 // c5 = (1.5 tr q - 7.5 uh.w) kJac, c6 = (52.5 uh.w - 7.5 tr q) kJac).
 constexpr int kBasis = 16;
 
-__device__ __host__ inline float cell_features(double px, double py, double pz, double cx, double cy, double cz,
-                                               double M, const double* bmin, const double* bmax, const double* Q,
-                                               float* f, float* b)
+// Core of cell_features in arithmetic T: u = cm - p, e = box extents,
+// g = box centre - cm, M, Q.
+template <typename T>
+__device__ __host__ inline float cell_features_t(T ux, T uy, T uz, T M, T ex, T ey, T ez, T gx, T gy, T gz,
+                                                 const T* Q, float* f, float* b)
 {
-	const double ux = cx - px, uy = cy - py, uz = cz - pz;
-	const double d2 = ux * ux + uy * uy + uz * uz;
-	const double d = sqrt(d2);
-	const double ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
-	const double gx = 0.5 * (bmax[0] + bmin[0]) - cx, gy = 0.5 * (bmax[1] + bmin[1]) - cy,
-	             gz = 0.5 * (bmax[2] + bmin[2]) - cz;
-	double r = 0.5 * sqrt(ex * ex + ey * ey + ez * ez) + sqrt(gx * gx + gy * gy + gz * gz);
-	if(!(r > 0.0)) { r = 1e-30; }
-	const double inv = 1.0 / (M * r * r);
-	const double qxx = Q[0] * inv, qyy = Q[1] * inv, qzz = Q[2] * inv;
-	const double qxy = Q[3] * inv, qxz = Q[4] * inv, qyz = Q[5] * inv;
-	const double hx = ux / d, hy = uy / d, hz = uz / d;
-	const double wx = qxx * hx + qxy * hy + qxz * hz;
-	const double wy = qxy * hx + qyy * hy + qyz * hz;
-	const double wz = qxz * hx + qyz * hy + qzz * hz;
-	const double ox = gx / r, oy = gy / r, oz = gz / r;
-	const double vx = qxx * ox + qxy * oy + qxz * oz;   // q o
-	const double vy = qxy * ox + qyy * oy + qyz * oz;
-	const double vz = qxz * ox + qyz * oy + qzz * oz;
-	const double rho = r / d;
-	const double trq = qxx + qyy + qzz;
-	const double uqu = hx * wx + hy * wy + hz * wz;
-	const double uo = hx * ox + hy * oy + hz * oz;
+	const T d2 = ux * ux + uy * uy + uz * uz;
+	const T d = sqrt(d2);
+	T r = T(0.5) * sqrt(ex * ex + ey * ey + ez * ez) + sqrt(gx * gx + gy * gy + gz * gz);
+	if(!(r > T(0))) { r = T(1e-30); }
+	const T inv = T(1) / (M * r * r);
+	const T qxx = Q[0] * inv, qyy = Q[1] * inv, qzz = Q[2] * inv;
+	const T qxy = Q[3] * inv, qxz = Q[4] * inv, qyz = Q[5] * inv;
+	const T id = T(1) / d;
+	const T hx = ux * id, hy = uy * id, hz = uz * id;
+	const T wx = qxx * hx + qxy * hy + qxz * hz;
+	const T wy = qxy * hx + qyy * hy + qyz * hz;
+	const T wz = qxz * hx + qyz * hy + qzz * hz;
+	const T ir = T(1) / r;
+	const T ox = gx * ir, oy = gy * ir, oz = gz * ir;
+	const T vx = qxx * ox + qxy * oy + qxz * oz;   // q o
+	const T vy = qxy * ox + qyy * oy + qyz * oz;
+	const T vz = qxz * ox + qyz * oy + qzz * oz;
+	const T rho = r * id;
+	const T trq = qxx + qyy + qzz;
+	const T uqu = hx * wx + hy * wy + hz * wz;
+	const T uo = hx * ox + hy * oy + hz * oz;
 	f[0] = static_cast<float>(rho);
 	f[1] = static_cast<float>(rho * rho);
 	f[2] = static_cast<float>(trq);
 	f[3] = static_cast<float>(uqu);
 	f[4] = static_cast<float>(wx * wx + wy * wy + wz * wz);
-	f[5] = static_cast<float>(qxx * qxx + qyy * qyy + qzz * qzz + 2.0 * (qxy * qxy + qxz * qxz + qyz * qyz));
+	f[5] = static_cast<float>(qxx * qxx + qyy * qyy + qzz * qzz + T(2) * (qxy * qxy + qxz * qxz + qyz * qyz));
 	f[6] = static_cast<float>(uo);
 	f[7] = static_cast<float>(ox * ox + oy * oy + oz * oz);
 	f[8] = static_cast<float>(wx * ox + wy * oy + wz * oz);
@@ -232,6 +232,31 @@ __device__ __host__ inline float cell_features(double px, double py, double pz, 
 	b[12] = static_cast<float>(qxy); b[13] = static_cast<float>(qxz); b[14] = static_cast<float>(qyz);
 	b[15] = 0.0f;
 	return static_cast<float>(M * rho * rho / d2);
+}
+
+// FP64 features of (p, cell): used for the training samples.
+__device__ __host__ inline float cell_features(double px, double py, double pz, double cx, double cy, double cz,
+                                               double M, const double* bmin, const double* bmax, const double* Q,
+                                               float* f, float* b)
+{
+	return cell_features_t<double>(cx - px, cy - py, cz - pz, M, bmax[0] - bmin[0], bmax[1] - bmin[1],
+	                               bmax[2] - bmin[2], 0.5 * (bmax[0] + bmin[0]) - cx, 0.5 * (bmax[1] + bmin[1]) - cy,
+	                               0.5 * (bmax[2] + bmin[2]) - cz, Q, f, b);
+}
+
+// FP32 features for the walk: the differences are taken in FP64 and rounded
+// once, the rest in FP32 (the inputs are rounded to BF16 anyway).
+__device__ inline float cell_features_fast(float ux, float uy, float uz, double cx, double cy, double cz, double M,
+                                           const double* bmin, const double* bmax, const double* Q, float* f,
+                                           float* b)
+{
+	const float q[6] = {static_cast<float>(Q[0]), static_cast<float>(Q[1]), static_cast<float>(Q[2]),
+	                    static_cast<float>(Q[3]), static_cast<float>(Q[4]), static_cast<float>(Q[5])};
+	return cell_features_t<float>(ux, uy, uz, static_cast<float>(M), static_cast<float>(bmax[0] - bmin[0]),
+	                              static_cast<float>(bmax[1] - bmin[1]), static_cast<float>(bmax[2] - bmin[2]),
+	                              static_cast<float>(0.5 * (bmax[0] + bmin[0]) - cx),
+	                              static_cast<float>(0.5 * (bmax[1] + bmin[1]) - cy),
+	                              static_cast<float>(0.5 * (bmax[2] + bmin[2]) - cz), q, f, b);
 }
 
 // Basis element of output column col as its 9 target components (R: 3, then

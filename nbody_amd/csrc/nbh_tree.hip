@@ -1191,11 +1191,11 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 			const int slot = nband + __popcll(bband & lt);
 			float fe[nbh_net::kIn];
 			float be[nbh_net::kBasis];
-			const float scale = nbh_net::cell_features(ccx, ccy, ccz, rec.x, rec.y, rec.z, m, bmin + 3 * node,
-			                                           bmax + 3 * node, quad + 6 * node, fe, be);
+			const float scale = nbh_net::cell_features_fast(ent.x, ent.y, ent.z, rec.x, rec.y, rec.z, m,
+			                                                bmin + 3 * node, bmax + 3 * node, quad + 6 * node, fe, be);
 			for(int k = 0; k < nbh_net::kBasis; ++k) { bandb[slot * nbh_net::kBasis + k] = be[k]; }
 			bandw[2 * slot] = scale;
-			bandw[2 * slot + 1] = static_cast<float>(static_cast<double>(scale) / (nbh_net::kJac * dc));
+			bandw[2 * slot + 1] = scale / (static_cast<float>(nbh_net::kJac) * static_cast<float>(dc));
 			for(int k = 0; k < nbh_net::kIn; ++k) { bandf[slot * nbh_net::kIn + k] = nbh_net::to_bf16(fe[k]); }
 		}
 		if(isnear)

@@ -100,6 +100,11 @@ void nbody_data::compute_statistics(nbody_engine* engine, double out[12])
 	const bool want_e = contains(m_check_list, 'E');
 	if(engine != nullptr && engine->compute_statistics(want_e, out))
 	{
+		if(!want_e)
+		{
+			out[6] = 0.0;
+			out[7] = 0.0;
+		}
 		return;
 	}
 	std::memset(out, 0, 12 * sizeof(double));

@@ -37,6 +37,8 @@ def _stats_rel(a, b):
 def test_unit_ball_rk4_plev_every_10_steps():
     n, dt = 1024, 1e-3
     dh, do = _data("ball", n, 1), _data("ball", n, 1)
+    dh.set_check_list("PLEV")
+    do.set_check_list("PLEV")
     _, _, m, _ = dh.bodies()
     assert np.allclose(m, 1.0 / n)
     eh = nbody_create_engine({"engine": "hip"})
@@ -72,6 +74,7 @@ def test_unit_ball_rk4_plev_every_10_steps():
 
 
 def _energy(d, e):
+    d.set_check_list("PLEV")
     st = d.statistics(e)
     return st[6] + st[7]
 
