@@ -9,7 +9,6 @@ libnbody.so; buffers are numpy float64 arrays on the host side.
 from __future__ import annotations
 
 import ctypes as C
-import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -184,7 +183,9 @@ class nbody_engine:
         self._lib = native.host_lib()
         self.handle = handle
         self._data = None
-        self._solvers = weakref.WeakSet()
+        # strong references: the collector clears weak references before it
+        # runs finalisers, and a solver must be finalised before its engine
+        self._solvers = []
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -193,6 +194,7 @@ class nbody_engine:
             # solvers first (garbage collection may finalise them in any order)
             for s in list(getattr(self, "_solvers", ())):
                 s.destroy()
+            self._solvers = []
             self._lib.nbs_engine_destroy(h)
             self.handle = None
 
@@ -339,6 +341,10 @@ class nbody_solver:
         if h:
             self._lib.nbs_solver_destroy(h)
             self.handle = None
+        e = getattr(self, "_engine", None)
+        if e is not None:
+            e._solvers = [s for s in e._solvers if s is not self]
+            self._engine = None
 
     def destroy(self) -> None:
         self.__del__()
@@ -347,8 +353,11 @@ class nbody_solver:
         return self._lib.nbs_solver_type_name(self.handle).decode()
 
     def set_engine(self, e: nbody_engine) -> None:
+        if self._engine is not None and self._engine is not e:
+            self._engine._solvers = [s for s in self._engine._solvers if s is not self]
         self._engine = e
-        e._solvers.add(self)
+        if all(s is not self for s in e._solvers):
+            e._solvers.append(self)
         self._lib.nbs_solver_set_engine(self.handle, e.handle)
 
     def set_time_step(self, min_step: float, max_step: float) -> None:
