@@ -164,6 +164,11 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
     for name in os.listdir(tmpdir):
         os.remove(os.path.join(tmpdir, name))
     os.rmdir(tmpdir)
+    ok = [r for r in out["runs"] if r["force_rel_err"]["rms"] <= 1e-3]
+    if ok:
+        best = max(ok, key=lambda r: r["steps_per_s"])
+        out["best_at_rms_1e-3"] = {"variant": best["variant"], "steps_per_s": best["steps_per_s"],
+                                   "rms": best["force_rel_err"]["rms"]}
     return out
 
 
@@ -372,6 +377,7 @@ def main(argv=None) -> int:
     if not a.no_bh:
         variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False),
                     ("plain theta=0.25", 4.0, False), ("quadrupole theta=0.5", 2.0, "quadrupole"),
+                    ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
                     ("quadrupole theta=0.8", 1.25, "quadrupole")]
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))

@@ -689,41 +689,49 @@ __device__ inline void sweep32(const float4* __restrict__ buf, int count, float 
 	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
 }
 
-constexpr int kQuadBuf = 128;    // cells waiting for the quadrupole sweep
-
-// Quadrupole field of cells at the lane's body: entries of 3 float4
-// (cm offset from the group centre, tr Q), (Qxx Qyy Qzz Qxy), (Qxz Qyz - -):
-// a = -3 Q r / d^5 + (7.5 r.Q.r / d^2 - 1.5 tr Q) r / d^5, r = cm - x.
-__device__ inline void sweep_quad(const float4* __restrict__ qb, int count, float tx, float ty, float tz,
-                                  float& qx, float& qy, float& qz)
+// Monopole + quadrupole field of the cells in qb at the lane's body.  Entries
+// of 3 float4: (mass-centre offset from the group centre, M), (Q'xx Q'yy Q'zz
+// Q'xy), (Q'xz Q'yz - -), with Q' = 3 Q - tr(Q) I the traceless second moment
+// about the mass centre; r = cm - x, d = |r|:
+//   a = M r / d^3 + (2.5 (r.Q'.r) / d^2 r - Q' r) / d^5.
+// The monopole part is summed like sweep32 (same operations, Kahan per sweep).
+__device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float tx, float ty, float tz,
+                                float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
 {
+	float px = 0.0f, py = 0.0f, pz = 0.0f;
 	for(int j = 0; j < count; ++j)
 	{
 		const float4 e0 = qb[3 * j], e1 = qb[3 * j + 1], e2 = qb[3 * j + 2];
 		const float rx = e0.x - tx, ry = e0.y - ty, rz = e0.z - tz;
-		const float r2 = fmaxf(fmaf(rz, rz, fmaf(ry, ry, rx * rx)), 1e-8f);
+		float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
+		r2 = fmaxf(r2, 1e-8f);
 		const float ri = __builtin_amdgcn_rsqf(r2);
 		const float ri2 = ri * ri;
-		const float ri5 = ri2 * ri2 * ri;
+		const float w = (e0.w * ri) * ri2;
+		const float ri5 = (ri2 * ri2) * ri;
 		const float Qx = fmaf(e1.x, rx, fmaf(e1.w, ry, e2.x * rz));
 		const float Qy = fmaf(e1.w, rx, fmaf(e1.y, ry, e2.y * rz));
 		const float Qz = fmaf(e2.x, rx, fmaf(e2.y, ry, e1.z * rz));
-		const float rqr = fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz));
-		const float c = fmaf(7.5f * rqr, ri2, -1.5f * e0.w);
-		qx = fmaf(fmaf(-3.0f, Qx, c * rx), ri5, qx);
-		qy = fmaf(fmaf(-3.0f, Qy, c * ry), ri5, qy);
-		qz = fmaf(fmaf(-3.0f, Qz, c * rz), ri5, qz);
+		const float c = (2.5f * fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz))) * ri2;
+		px = fmaf(fmaf(c, rx, -Qx), ri5, fmaf(rx, w, px));
+		py = fmaf(fmaf(c, ry, -Qy), ri5, fmaf(ry, w, py));
+		pz = fmaf(fmaf(c, rz, -Qz), ri5, fmaf(rz, w, pz));
 	}
+	float v = px - cx; float t = ax + v; cx = (t - ax) - v; ax = t;
+	v = py - cy; t = ay + v; cy = (t - ay) - v; ay = t;
+	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
 }
 
-// Quadrupole entry of node `node` (mass centre offset ent.xyz from the group
-// centre) at slot `slot` of qb.
+constexpr int kQuadBuf = 128;    // cells waiting for the monopole + quadrupole sweep
+
+// Entry of node `node` (mass-centre offset and mass in ent) at slot `slot` of qb.
 __device__ inline void put_quad(float4* qb, int slot, const float4& ent, const double* __restrict__ Q)
 {
-	qb[3 * slot] = make_float4(ent.x, ent.y, ent.z, static_cast<float>(Q[0] + Q[1] + Q[2]));
-	qb[3 * slot + 1] = make_float4(static_cast<float>(Q[0]), static_cast<float>(Q[1]), static_cast<float>(Q[2]),
-	                               static_cast<float>(Q[3]));
-	qb[3 * slot + 2] = make_float4(static_cast<float>(Q[4]), static_cast<float>(Q[5]), 0.0f, 0.0f);
+	const double tr = Q[0] + Q[1] + Q[2];
+	qb[3 * slot] = ent;
+	qb[3 * slot + 1] = make_float4(static_cast<float>(3.0 * Q[0] - tr), static_cast<float>(3.0 * Q[1] - tr),
+	                               static_cast<float>(3.0 * Q[2] - tr), static_cast<float>(3.0 * Q[3]));
+	qb[3 * slot + 2] = make_float4(static_cast<float>(3.0 * Q[4]), static_cast<float>(3.0 * Q[5]), 0.0f, 0.0f);
 }
 
 // kQuad: internal cells taken as monopoles also contribute their quadrupole
@@ -748,7 +756,6 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	const float ty = static_cast<float>(me.y - ccy);
 	const float tz = static_cast<float>(me.z - ccz);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-	float qx = 0.0f, qy = 0.0f, qz = 0.0f;
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	unsigned long long vis = 0, taken = 0, spill = 0;
 
@@ -764,7 +771,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		const int node = valid ? stack[sp - cnt + lane] : kRoot;
 		sp -= cnt;
 		__syncthreads();
-		bool accept = false, open = false;
+		bool accept = false, open = false, cell = false;
 		float4 ent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		if(valid)
 		{
@@ -784,6 +791,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 				const double ez = fmax(fmax(gz0 - rec.z, rec.z - gz1), 0.0);
 				accept = ex * ex + ey * ey + ez * ez > rec.w;
 				open = !accept;
+				cell = kQuad && accept;     // taken internal cell (stack spills stay monopoles)
 			}
 		}
 		const uint64_t bopen = __ballot(open);
@@ -797,16 +805,20 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		}
 		const uint64_t bacc = __ballot(accept);
 		const uint64_t bopen2 = __ballot(open);
-		if(accept)
-		{
-			buf[nbuf + __popcll(bacc & lt)] = ent;
-		}
 		if constexpr(kQuad)
 		{
-			const bool cell = accept && node < n;
+			// taken internal cells: monopole + quadrupole sweep; the rest: monopole sweep
 			const uint64_t bcell = __ballot(cell);
+			const uint64_t bmono = bacc & ~bcell;
 			if(cell) { put_quad(qbuf, nq + __popcll(bcell & lt), ent, quad + 6 * node); }
+			else if(accept) { buf[nbuf + __popcll(bmono & lt)] = ent; }
 			nq += __popcll(bcell);
+			nbuf += __popcll(bmono);
+		}
+		else
+		{
+			if(accept) { buf[nbuf + __popcll(bacc & lt)] = ent; }
+			nbuf += __popcll(bacc);
 		}
 		if(open)
 		{
@@ -814,7 +826,6 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			stack[sp + 2 * r] = 2 * node + 1;
 			stack[sp + 2 * r + 1] = 2 * node;
 		}
-		nbuf += __popcll(bacc);
 		sp += 2 * __popcll(bopen2);
 		vis += valid ? 1 : 0;
 		taken += accept ? 1 : 0;
@@ -829,7 +840,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		{
 			if(nq > kQuadBuf - kGroup)
 			{
-				sweep_quad(qbuf, nq, tx, ty, tz, qx, qy, qz);
+				sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
 				nq = 0;
 				__syncthreads();
 			}
@@ -841,15 +852,15 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	}
 	if constexpr(kQuad)
 	{
-		if(nq > 0) { sweep_quad(qbuf, nq, tx, ty, tz, qx, qy, qz); }
+		if(nq > 0) { sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz); }
 	}
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
 	f[n + b] = y[4 * n + b];
 	f[2 * n + b] = y[5 * n + b];
-	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx) + static_cast<double>(qx);
-	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy) + static_cast<double>(qy);
-	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz) + static_cast<double>(qz);
+	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx);
+	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy);
+	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz);
 	// counters: [0] pair interactions (entries x 64 targets), [1] node visits, [2] spills
 	for(int off = 32; off > 0; off >>= 1)
 	{
@@ -1046,8 +1057,8 @@ k_band_residual(const double4* __restrict__ xyzr, const double* __restrict__ nma
 }
 
 constexpr int kBand = 80;        // far-cell rows waiting for the network (< 16 + 64)
-// k_bh_group32 plus the correction.  Every accepted cell still enters the
-// monopole sweep.  Near cells also enter the quadrupole sweep; far cells are
+// k_bh_group32 plus the correction.  Near cells enter the monopole +
+// quadrupole sweep, every other taken node the monopole sweep; far cells are
 // fed to the network in tiles of 16, and the predicted residual field of each,
 // expanded to first order about the group centre, is summed over the group's
 // far cells into C (3) and J (6), the correction of a body at offset delta from
@@ -1079,7 +1090,6 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	const float ty = static_cast<float>(me.y - ccy);
 	const float tz = static_cast<float>(me.z - ccz);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-	float qx = 0.0f, qy = 0.0f, qz = 0.0f;  // quadrupole terms of near cells
 	float acc[nbh_net::kOut];               // this lane's share of the far-cell correction (R, J)
 	for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] = 0.0f; }
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -1179,7 +1189,8 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		const uint64_t bopen = __ballot(open);
 		const uint64_t bband = __ballot(band);
 		const uint64_t bnear = __ballot(isnear);
-		if(accept) { buf[nbuf + __popcll(bacc & lt)] = ent; }
+		const uint64_t bmono = bacc & ~bnear;     // near cells go to the monopole + quadrupole sweep
+		if(accept && !isnear) { buf[nbuf + __popcll(bmono & lt)] = ent; }
 		if(open)
 		{
 			const int r = __popcll(bopen & lt);
@@ -1202,7 +1213,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		{
 			put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node);
 		}
-		nbuf += __popcll(bacc);
+		nbuf += __popcll(bmono);
 		sp += 2 * __popcll(bopen);
 		nband += __popcll(bband);
 		nq += __popcll(bnear);
@@ -1219,7 +1230,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		}
 		if(nq > kQuadBuf - kGroup)
 		{
-			sweep_quad(qbuf, nq, tx, ty, tz, qx, qy, qz);
+			sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
 			nq = 0;
 			__syncthreads();
 		}
@@ -1231,7 +1242,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	}
 	if(nq > 0)
 	{
-		sweep_quad(qbuf, nq, tx, ty, tz, qx, qy, qz);
+		sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
 	}
 	__syncthreads();
 	run_tiles(nband, true);
@@ -1240,9 +1251,9 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
 	}
 	const float* S = acc;
-	const float kx = qx + (S[0] + S[3] * tx + S[6] * ty + S[7] * tz);
-	const float ky = qy + (S[1] + S[6] * tx + S[4] * ty + S[8] * tz);
-	const float kz = qz + (S[2] + S[7] * tx + S[8] * ty + S[5] * tz);
+	const float kx = S[0] + S[3] * tx + S[6] * ty + S[7] * tz;
+	const float ky = S[1] + S[6] * tx + S[4] * ty + S[8] * tz;
+	const float kz = S[2] + S[7] * tx + S[8] * ty + S[5] * tz;
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
 	f[n + b] = y[4 * n + b];
