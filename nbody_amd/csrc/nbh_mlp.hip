@@ -5,8 +5,9 @@
 // (nbh_mlp_dev.h).  A training step over a batch:
 //   k_mlp_grad   each wave takes every nw-th tile of the batch, runs forward and
 //                backward and writes its partial gradient (FP32) and loss;
-//   k_mlp_adam   one work-group sums the partials in a fixed order, applies Adam
-//                to the FP32 master weights and re-packs the BF16 fragments.
+//   k_mlp_adam   sums the partials of each parameter in a fixed order and
+//                applies Adam to the FP32 master weights;
+//   k_mlp_pack   re-packs the BF16 fragments.
 // Batches are strided over the sample array (batch b holds samples b, b+nb,
 // b+2nb, ...), so samples generated group by group are mixed without a shuffle.
 #include <cmath>
@@ -66,7 +67,7 @@ __device__ inline float frag_value(const float* __restrict__ p, int f, int lane,
 
 __device__ inline void pack_all(const float* __restrict__ p, uint2* __restrict__ frags)
 {
-	for(int i = threadIdx.x; i < kFrags * 64; i += blockDim.x)
+	for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < kFrags * 64; i += gridDim.x * blockDim.x)
 	{
 		const int f = i / 64, lane = i % 64;
 		uint32_t h[4];
@@ -257,36 +258,33 @@ k_mlp_grad(const float* __restrict__ x, const float* __restrict__ bs, const floa
 	if(lane == 0) { g[kParams] = loss; }
 }
 
-// Sum nw partials in order; optionally apply Adam and re-pack.
+// Sum the nw partial gradients of each parameter in order (and the loss);
+// with apply, take an Adam step on the FP32 master weights.  The BF16
+// fragments are re-packed by k_mlp_pack afterwards.
 __global__ void __launch_bounds__(256)
 k_mlp_adam(const float* __restrict__ partial, int nw, float* __restrict__ grad, float* __restrict__ p,
-           float* __restrict__ am, float* __restrict__ av, uint2* __restrict__ frags, float lr, float b1t, float b2t,
-           int apply)
+           float* __restrict__ am, float* __restrict__ av, float lr, float b1t, float b2t, int apply)
 {
 	const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
-	for(int i = threadIdx.x; i <= kParams; i += blockDim.x)
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i > kParams) { return; }
+	float s = 0.0f;
+	for(int w = 0; w < nw; ++w) { s += partial[static_cast<size_t>(w) * (kParams + 1) + i]; }
+	grad[i] = s;
+	if(apply && i < kParams)
 	{
-		float s = 0.0f;
-		for(int w = 0; w < nw; ++w) { s += partial[static_cast<size_t>(w) * (kParams + 1) + i]; }
-		grad[i] = s;
-		if(apply && i < kParams)
-		{
-			const float m = beta1 * am[i] + (1.0f - beta1) * s;
-			const float v = beta2 * av[i] + (1.0f - beta2) * s * s;
-			am[i] = m;
-			av[i] = v;
-			const float mh = m / (1.0f - b1t);
-			const float vh = v / (1.0f - b2t);
-			p[i] -= lr * mh / (sqrtf(vh) + eps);
-		}
-	}
-	if(apply)
-	{
-		__syncthreads();
-		__threadfence_block();
-		pack_all(p, frags);
+		const float m = beta1 * am[i] + (1.0f - beta1) * s;
+		const float v = beta2 * av[i] + (1.0f - beta2) * s * s;
+		am[i] = m;
+		av[i] = v;
+		const float mh = m / (1.0f - b1t);
+		const float vh = v / (1.0f - b2t);
+		p[i] -= lr * mh / (sqrtf(vh) + eps);
 	}
 }
+
+constexpr int kAdamBlocks = (kParams + 1 + 255) / 256;
+constexpr int kPackBlocks = (kFrags * 64 + 255) / 256;
 
 }  // namespace
 
@@ -376,7 +374,7 @@ nbh_status nbh_mlp_set_params(nbh_ctx* ctx, nbh_mlp* m, const float* params)
 	NBH_HIP(hipMemcpyAsync(m->params, params, kParams * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
 	NBH_HIP(hipMemsetAsync(m->adam_m, 0, kParams * sizeof(float), ctx->stream));
 	NBH_HIP(hipMemsetAsync(m->adam_v, 0, kParams * sizeof(float), ctx->stream));
-	hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, ctx->stream, m->params, m->frags);
+	hipLaunchKernelGGL(k_mlp_pack, dim3(kPackBlocks), dim3(256), 0, ctx->stream, m->params, m->frags);
 	NBH_LAUNCHED();
 	NBH_HIP(hipStreamSynchronize(ctx->stream));
 	m->step = 0;
@@ -415,8 +413,8 @@ nbh_status nbh_mlp_gradient(nbh_ctx* ctx, nbh_mlp* m, const float* x, const floa
 	                   static_cast<size_t>(1), count, 1.0f / static_cast<float>(count), m->frags, m->params,
 	                   m->partial);
 	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, ctx->stream, m->partial, nw, m->grad, m->params,
-	                   m->adam_m, m->adam_v, m->frags, 0.0f, 0.0f, 0.0f, 0);
+	hipLaunchKernelGGL(k_mlp_adam, dim3(kAdamBlocks), dim3(256), 0, ctx->stream, m->partial, nw, m->grad,
+	                   m->params, m->adam_m, m->adam_v, 0.0f, 0.0f, 0.0f, 0);
 	NBH_LAUNCHED();
 	std::vector<float> g(kParams + 1);
 	NBH_HIP(hipMemcpyAsync(g.data(), m->grad, (kParams + 1) * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -451,8 +449,10 @@ nbh_status nbh_mlp_train(nbh_ctx* ctx, nbh_mlp* m, const float* x, const float* 
 			hipLaunchKernelGGL(k_mlp_grad, dim3(nw), dim3(64), 0, ctx->stream, x, basis, t, b, nb, cnt,
 			                   1.0f / static_cast<float>(cnt), m->frags, m->params, m->partial);
 			NBH_LAUNCHED();
-			hipLaunchKernelGGL(k_mlp_adam, dim3(1), dim3(256), 0, ctx->stream, m->partial, nw, m->grad, m->params,
-			                   m->adam_m, m->adam_v, m->frags, lr_t, b1t, b2t, 1);
+			hipLaunchKernelGGL(k_mlp_adam, dim3(kAdamBlocks), dim3(256), 0, ctx->stream, m->partial, nw, m->grad,
+			                   m->params, m->adam_m, m->adam_v, lr_t, b1t, b2t, 1);
+			NBH_LAUNCHED();
+			hipLaunchKernelGGL(k_mlp_pack, dim3(kPackBlocks), dim3(256), 0, ctx->stream, m->params, m->frags);
 			NBH_LAUNCHED();
 			if(epoch_loss != nullptr && b + 1 == nb)
 			{
