@@ -33,6 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "pair-interactions/s (direct); BH steps/s at N=4M, force err ≤1e-3 vs FP64"
 FLOP_PER_PAIR = 20.0
+VALU_PER_PAIR = 15           # inner loop of k_direct_fp32 per pair (14 VALU + v_rsq_f32)
 PEAK_FP32_TFLOPS = 157.3
 
 
@@ -364,6 +365,10 @@ def main(argv=None) -> int:
             "avg_kernel_ms": avg_ms,
             "launches": cnt.value,
             "pairs_per_s_kernel": n * n / (avg_ms * 1e-3),
+            # issued instructions: 14 VALU + 1 v_rsq_f32 per pair against the
+            # 157.3e12 / 2 lane-FMA/s behind the FP32 peak
+            "valu_instr_per_pair": VALU_PER_PAIR,
+            "valu_issue_frac": VALU_PER_PAIR * n * n / (avg_ms * 1e-3) / (peak * 1e12 / 2),
         },
         "force_error": {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "max_rel": float(rel.max()),
                         "vs": "FP64 direct kernel (bit-exact with the CPU oracle)"},
