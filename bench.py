@@ -173,6 +173,57 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
     return out
 
 
+def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dict:
+    """Cold cube [-1,1]^3, v = 0, masses lognormal(0, 0.5) normalised to 1 (seed 5):
+    `steps` steps of max_step 1e-3 (rkfeagin10 with one level of 8 adaptive
+    substeps, bs with max_level 8) with FP32 force tiles (+ the compensated FP64
+    Butcher combination for rkfeagin10) against all-FP64; energy drift and the
+    absolute total momentum / angular momentum of both runs (P0 = L0 = 0, so the
+    reference's percentage errors are undefined for this state)."""
+    import numpy as np
+
+    from nbody_amd import nbody_create_engine, nbody_create_solver, nbody_data
+    out = {"workload": f"cold cube N={n}, lognormal(0.5) masses, {steps} steps of 1e-3", "runs": []}
+    for solver in solvers:
+        res = {"solver": solver}
+        for prec in ("fp64", "fp32"):
+            d = nbody_data()
+            d.set_quiet(True)
+            d.generate("cube", n, 5)
+            d.set_check_list("PLEV")
+            e = nbody_create_engine({"engine": "hip", "force_precision": prec})
+            if e is None or not e.init(d):
+                raise RuntimeError("hip engine init failed")
+            params = {"solver": solver, "max_step": 1e-3}
+            if solver == "bs":
+                params["max_level"] = 8
+            else:
+                # one level of 8 substeps: with the default 8 levels the adaptive rule
+                # can subdivide without bound on close pairs of this cold start
+                params["max_recursion"] = 1
+            if solver == "rkfeagin10" and prec == "fp32":
+                params["correction"] = True
+            s = nbody_create_solver(params)
+            s.set_engine(e)
+            st0 = d.statistics(e)
+            t0 = time.perf_counter()
+            for i in range(steps):
+                s.advise(1e-3)
+                if (i + 1) % max(1, steps // 20) == 0:
+                    log(f"split precision {solver} {prec}: step {i + 1}/{steps}, "
+                        f"{e.get_compute_count()} force evaluations, {time.perf_counter() - t0:.1f} s")
+            e.get_data(d)
+            st1 = d.statistics(e)
+            e0, e1 = st0[6] + st0[7], st1[6] + st1[7]
+            res[prec] = {"energy_drift_rel": (e1 - e0) / abs(e0), "abs_P": float(np.linalg.norm(st1[:3])),
+                         "abs_L": float(np.linalg.norm(st1[3:6])), "seconds": time.perf_counter() - t0,
+                         "force_evaluations": e.get_compute_count()}
+            s.destroy()
+            e.destroy()
+        out["runs"].append(res)
+    return out
+
+
 def pmc_probe(n: int) -> int:
     """Launches for the hardware-counter runs: the FP32 direct kernel three times
     on the benchmark state, then a plain streaming kernel of known traffic
@@ -248,6 +299,8 @@ def main(argv=None) -> int:
     ap.add_argument("--bh-n", type=int, default=1 << 22)
     ap.add_argument("--bh-steps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction Barnes-Hut run")
+    ap.add_argument("--split-precision", type=int, default=0, metavar="STEPS",
+                    help="also run the cold-cube split-precision workload (N=65536) for STEPS steps")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch the kernels measured by the hardware-counter runs, then exit")
     a = ap.parse_args(argv)
@@ -387,6 +440,8 @@ def main(argv=None) -> int:
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))
         out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)
+    if a.split_precision > 0:
+        out["split_precision"] = split_precision_run(1 << 16, a.split_precision)
     print(json.dumps(out), flush=True)
     return 0
 
