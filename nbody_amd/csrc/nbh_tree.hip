@@ -663,7 +663,7 @@ k_bh_stackless32(const double4* __restrict__ xyzr, const double* __restrict__ nm
 // takes the node as a monopole instead (counted in counters[2]).
 constexpr int kGroup = 64;
 constexpr int kWalkStack = 2048;
-constexpr int kWalkBuf = 512;
+constexpr int kWalkBuf = 256;
 
 __device__ inline void sweep32(const float4* __restrict__ buf, int count, float tx, float ty, float tz,
                                float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
@@ -722,7 +722,7 @@ __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float 
 	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
 }
 
-constexpr int kQuadBuf = 128;    // cells waiting for the monopole + quadrupole sweep
+constexpr int kQuadBuf = 96;     // cells waiting for the monopole + quadrupole sweep
 
 // Entry of node `node` (mass-centre offset and mass in ent) at slot `slot` of qb.
 __device__ inline void put_quad(float4* qb, int slot, const float4& ent, const double* __restrict__ Q)
