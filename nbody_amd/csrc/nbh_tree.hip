@@ -662,8 +662,30 @@ k_bh_stackless32(const double4* __restrict__ xyzr, const double* __restrict__ nm
 // FP32 kernel.  No node is dropped: a child push that would overflow the stack
 // takes the node as a monopole instead (counted in counters[2]).
 constexpr int kGroup = 64;
-constexpr int kWalkStack = 2048;
+constexpr int kWalkStack = 1024;   // entries of sibling pairs; (levels + 2) * 32 suffice
 constexpr int kWalkBuf = 256;
+
+// The walk stack holds sibling pairs: an even entry e stands for the nodes e and
+// e + 1, the odd entry kRoot for the root alone.  A step pops up to 32 entries,
+// one node per lane (lane 2i takes the left node of entry i, lane 2i + 1 the
+// right one); every node the step opens pushes one entry, its two children.
+// Popped in this order, the stack never holds more than (levels + 2) * 32
+// entries.
+__device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid)
+{
+	const int cnt = sp < kGroup / 2 ? sp : kGroup / 2;
+	const int i = lane >> 1;
+	int node = kRoot;
+	valid = false;
+	if(i < cnt)
+	{
+		const int e = stack[sp - cnt + i];
+		node = e == kRoot ? kRoot : e + (lane & 1);
+		valid = e != kRoot || (lane & 1) == 0;
+	}
+	sp -= cnt;
+	return node;
+}
 
 __device__ inline void sweep32(const float4* __restrict__ buf, int count, float tx, float ty, float tz,
                                float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
@@ -766,10 +788,8 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	__syncthreads();
 	while(sp > 0)
 	{
-		const int cnt = sp < kGroup ? sp : kGroup;
-		const bool valid = lane < cnt;
-		const int node = valid ? stack[sp - cnt + lane] : kRoot;
-		sp -= cnt;
+		bool valid;
+		const int node = walk_pop(stack, sp, lane, valid);
 		__syncthreads();
 		bool accept = false, open = false, cell = false;
 		float4 ent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -796,7 +816,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		}
 		const uint64_t bopen = __ballot(open);
 		const int nopen = __popcll(bopen);
-		if(sp + 2 * nopen > kWalkStack)
+		if(sp + nopen > kWalkStack)
 		{
 			// no room for the children: take these nodes as monopoles
 			spill += open ? 1 : 0;
@@ -820,13 +840,8 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			if(accept) { buf[nbuf + __popcll(bacc & lt)] = ent; }
 			nbuf += __popcll(bacc);
 		}
-		if(open)
-		{
-			const int r = __popcll(bopen2 & lt);
-			stack[sp + 2 * r] = 2 * node + 1;
-			stack[sp + 2 * r + 1] = 2 * node;
-		}
-		sp += 2 * __popcll(bopen2);
+		if(open) { stack[sp + __popcll(bopen2 & lt)] = 2 * node; }
+		sp += __popcll(bopen2);
 		vis += valid ? 1 : 0;
 		taken += accept ? 1 : 0;
 		__syncthreads();
@@ -933,10 +948,8 @@ k_band_collect(const double4* __restrict__ xyzr, const double* __restrict__ bmin
 	__syncthreads();
 	while(sp > 0)
 	{
-		const int cnt = sp < kGroup ? sp : kGroup;
-		const bool valid = lane < cnt;
-		const int node = valid ? stack[sp - cnt + lane] : kRoot;
-		sp -= cnt;
+		bool valid;
+		const int node = walk_pop(stack, sp, lane, valid);
 		__syncthreads();
 		bool open = false, band = false;
 		if(valid && node < n)
@@ -954,15 +967,10 @@ k_band_collect(const double4* __restrict__ xyzr, const double* __restrict__ bmin
 			}
 		}
 		const int nopen = __popcll(__ballot(open));
-		if(sp + 2 * nopen > kWalkStack) { open = false; }
+		if(sp + nopen > kWalkStack) { open = false; }
 		const uint64_t bopen = __ballot(open);
-		if(open)
-		{
-			const int r = __popcll(bopen & lt);
-			stack[sp + 2 * r] = 2 * node + 1;
-			stack[sp + 2 * r + 1] = 2 * node;
-		}
-		sp += 2 * __popcll(bopen);
+		if(open) { stack[sp + __popcll(bopen & lt)] = 2 * node; }
+		sp += __popcll(bopen);
 		const uint64_t bband = __ballot(band);
 		const int nband = __popcll(bband);
 		if(nband > 0)
@@ -1142,10 +1150,8 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	__syncthreads();
 	while(sp > 0)
 	{
-		const int cnt = sp < kGroup ? sp : kGroup;
-		const bool valid = lane < cnt;
-		const int node = valid ? stack[sp - cnt + lane] : kRoot;
-		sp -= cnt;
+		bool valid;
+		const int node = walk_pop(stack, sp, lane, valid);
 		__syncthreads();
 		bool accept = false, open = false;
 		int cls = kCellNone;
@@ -1177,7 +1183,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 			}
 		}
 		const int nopen = __popcll(__ballot(open));
-		if(sp + 2 * nopen > kWalkStack)
+		if(sp + nopen > kWalkStack)
 		{
 			spill += open ? 1 : 0;
 			accept = accept || open;
@@ -1191,12 +1197,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		const uint64_t bnear = __ballot(isnear);
 		const uint64_t bmono = bacc & ~bnear;     // near cells go to the monopole + quadrupole sweep
 		if(accept && !isnear) { buf[nbuf + __popcll(bmono & lt)] = ent; }
-		if(open)
-		{
-			const int r = __popcll(bopen & lt);
-			stack[sp + 2 * r] = 2 * node + 1;
-			stack[sp + 2 * r + 1] = 2 * node;
-		}
+		if(open) { stack[sp + __popcll(bopen & lt)] = 2 * node; }
 		if(band)
 		{
 			const int slot = nband + __popcll(bband & lt);
@@ -1214,7 +1215,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 			put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node);
 		}
 		nbuf += __popcll(bmono);
-		sp += 2 * __popcll(bopen);
+		sp += __popcll(bopen);
 		nband += __popcll(bband);
 		nq += __popcll(bnear);
 		corrected += band ? 1 : 0;
@@ -1508,7 +1509,7 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 	NBH_CHECK(t->built, "tree has not been built");
 	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
 	NBH_CHECK(t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
-	NBH_CHECK(t->levels + 1 <= kWalkStack / kGroup, "tree too deep for the walk stack");
+	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
 	const int n = static_cast<int>(t->n);
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
 	NBH_LAUNCHED();
@@ -1545,7 +1546,7 @@ nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* 
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(t->built, "tree has not been built");
 	NBH_CHECK(t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
-	NBH_CHECK(t->levels + 1 <= kWalkStack / kGroup, "tree too deep for the walk stack");
+	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
 	NBH_CHECK(groups != nullptr && count != nullptr, "null argument");
 	NBH_CHECK(n_groups > 0 && n_groups <= (static_cast<size_t>(1) << 30), "bad group count %zu", n_groups);
 	NBH_CHECK(capacity == 0 || (x != nullptr && basis != nullptr && target != nullptr), "null sample buffer");
@@ -1602,7 +1603,7 @@ nbh_status nbh_bh_fcompute_group_mlp(nbh_ctx* ctx, const nbh_tree* t, const nbh_
 	NBH_CHECK(t->built, "tree has not been built");
 	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
 	NBH_CHECK(t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
-	NBH_CHECK(t->levels + 1 <= kWalkStack / kGroup, "tree too deep for the walk stack");
+	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
 	NBH_CHECK(rho_min >= 0.0f && near_k >= 0.0f, "rho_min and near_k must be >= 0");
 	const int n = static_cast<int>(t->n);
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
