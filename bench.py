@@ -224,6 +224,62 @@ def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dic
     return out
 
 
+def vector_ops_bench(n: int, reps: int = 20) -> dict:
+    """HBM bandwidth of the engine's fused vector ops on a state of n bodies
+    (6n doubles per buffer): nbh_fmaddn with k = 1 and k = 12 terms (the
+    rkfeagin10 stage combinations), nbh_fmaddn_corr with k = 12, nbh_maxabs.
+    Bytes are the algorithmic streams: fmaddn reads init + k sources and writes
+    the result; fmaddn_corr also reads and writes the compensation; maxabs reads
+    one buffer.  Device time from HIP events on the context's stream."""
+    from nbody_amd import native
+    hip = native.hip_lib()
+    m = 6 * n
+    ctx = C.c_void_p()
+    native.check(hip.nbh_ctx_create(0, C.byref(ctx)))
+    bufs = []
+
+    def dev():
+        ptr = C.c_void_p()
+        native.check(hip.nbh_malloc(ctx, 8 * m, C.byref(ptr)))
+        native.check(hip.nbh_fill(ctx, ptr, 1.0, m))
+        bufs.append(ptr)
+        return ptr
+
+    kmax = 12
+    a, init, corr = dev(), dev(), dev()
+    srcs = [dev() for _ in range(kmax)]
+    src_arr = (C.c_void_p * kmax)(*[p.value for p in srcs])
+    coef = (C.c_double * kmax)(*([1e-3] * kmax))
+    out = {"n": n, "doubles_per_buffer": m, "runs": []}
+    cases = [("fmaddn k=1", lambda: hip.nbh_fmaddn(ctx, a, init, src_arr, coef, 1, m), 8.0 * m * 3, "fmaddn"),
+             ("fmaddn k=12", lambda: hip.nbh_fmaddn(ctx, a, init, src_arr, coef, 12, m), 8.0 * m * 14, "fmaddn"),
+             ("fmaddn_corr k=12", lambda: hip.nbh_fmaddn_corr(ctx, a, corr, src_arr, coef, 12, m), 8.0 * m * 16,
+              "fmaddn")]
+    for label, fn, nbytes, cls in cases:
+        native.check(fn())
+        native.check(hip.nbh_timing_enable(ctx, 1))
+        native.check(hip.nbh_timing_reset(ctx))
+        for _ in range(reps):
+            native.check(fn())
+        tot, cnt = C.c_double(), C.c_uint64()
+        native.check(hip.nbh_timing_read(ctx, cls.encode(), C.byref(tot), C.byref(cnt)))
+        native.check(hip.nbh_timing_enable(ctx, 0))
+        ms = tot.value / max(1, cnt.value)
+        out["runs"].append({"op": label, "bytes": nbytes, "ms": ms, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+                            "frac_of_8TBps": nbytes / (ms * 1e-3) / 8e12})
+    val = C.c_double()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        native.check(hip.nbh_maxabs(ctx, a, m, C.byref(val)))
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    out["runs"].append({"op": "maxabs (incl. the 8-byte read-back)", "bytes": 8.0 * m, "ms": ms,
+                        "GB_per_s": 8.0 * m / (ms * 1e-3) / 1e9, "frac_of_8TBps": 8.0 * m / (ms * 1e-3) / 8e12})
+    for ptr in bufs:
+        native.check(hip.nbh_free(ctx, ptr))
+    native.check(hip.nbh_ctx_destroy(ctx))
+    return out
+
+
 def pmc_probe(n: int) -> int:
     """Launches for the hardware-counter runs: the FP32 direct kernel three times
     on the benchmark state, then a plain streaming kernel of known traffic
@@ -299,6 +355,7 @@ def main(argv=None) -> int:
     ap.add_argument("--bh-n", type=int, default=1 << 22)
     ap.add_argument("--bh-steps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction Barnes-Hut run")
+    ap.add_argument("--no-vector-ops", action="store_true", help="skip the vector-op bandwidth block")
     ap.add_argument("--split-precision", type=int, default=0, metavar="STEPS",
                     help="also run the cold-cube split-precision workload (N=65536) for STEPS steps")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -440,6 +497,9 @@ def main(argv=None) -> int:
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))
         out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)
+    if not a.no_vector_ops:
+        log("vector ops")
+        out["vector_ops"] = vector_ops_bench(1 << 22)
     if a.split_precision > 0:
         out["split_precision"] = split_precision_run(1 << 16, a.split_precision)
     print(json.dumps(out), flush=True)
