@@ -70,6 +70,15 @@ const uint2* mlp_frags(const nbh_mlp* m);
 const float* mlp_params(const nbh_mlp* m);
 const nbh_ctx* mlp_ctx(const nbh_mlp* m);
 
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the
+// keys, 8 bits per pass, ping-ponging between (k0, v0) and (k1, v1); *sorted_vals
+// points at the buffer holding the result.  counts: sort_counts_size(n) words,
+// scan_part: sort_scan_size() words of device scratch.
+size_t sort_counts_size(size_t n);
+size_t sort_scan_size();
+nbh_status sort_pairs(nbh_ctx* ctx, uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
+                      uint32_t* counts, uint32_t* scan_part, uint32_t** sorted_vals);
+
 // Number of work-groups for a grid-stride kernel over `work` items with
 // `per_group` items per work-group pass: enough to fill every CU several times.
 inline unsigned grid_for(const nbh_ctx* ctx, size_t work, size_t per_group)

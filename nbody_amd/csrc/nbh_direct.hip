@@ -162,17 +162,21 @@ __device__ inline void kahan_f32(float& s, float& c, float v)
 	s = ns;
 }
 
-// part: [nsplit][3][n] doubles, slice blockIdx.y of the sources.
+// part: [nsplit][3][n] doubles, slice blockIdx.y of the sources.  Targets are
+// taken in the order `perm` (bodies sorted along a space-filling curve), so a
+// work-group's 1024 targets are close together and their offsets from its
+// origin -- and those of the sources near them -- are small in FP32.
 __global__ void __launch_bounds__(kF32Threads)
-k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass,
+k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, const uint32_t* __restrict__ perm,
               double* __restrict__ part, size_t n, size_t jchunk)
 {
 	__shared__ float4 tile[kF32Tile];
 
 	const size_t tbase = static_cast<size_t>(blockIdx.x) * kF32TgtPerGroup;
-	const double ox = y[tbase];
-	const double oy = y[n + tbase];
-	const double oz = y[2 * n + tbase];
+	const size_t o = perm[tbase];
+	const double ox = y[o];
+	const double oy = y[n + o];
+	const double oz = y[2 * n + o];
 
 	float xi[kF32Tgt], yi[kF32Tgt], zi[kF32Tgt];
 	float sx[kF32Tgt], sy[kF32Tgt], sz[kF32Tgt];
@@ -182,9 +186,10 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass,
 	{
 		size_t i = tbase + static_cast<size_t>(k) * kF32Threads + threadIdx.x;
 		bool ok = i < n;
-		xi[k] = ok ? static_cast<float>(y[i] - ox) : 0.0f;
-		yi[k] = ok ? static_cast<float>(y[n + i] - oy) : 0.0f;
-		zi[k] = ok ? static_cast<float>(y[2 * n + i] - oz) : 0.0f;
+		const size_t b = ok ? perm[i] : 0;
+		xi[k] = ok ? static_cast<float>(y[b] - ox) : 0.0f;
+		yi[k] = ok ? static_cast<float>(y[n + b] - oy) : 0.0f;
+		zi[k] = ok ? static_cast<float>(y[2 * n + b] - oz) : 0.0f;
 		sx[k] = sy[k] = sz[k] = 0.0f;
 		cx[k] = cy[k] = cz[k] = 0.0f;
 	}
@@ -258,8 +263,8 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass,
 
 // f[0:3n] = v, f[3n:6n] = sum over slices (fixed order).
 __global__ void __launch_bounds__(256)
-k_direct_fp32_finish(const double* __restrict__ y, const double* __restrict__ part,
-                     double* __restrict__ f, size_t n, int nsplit)
+k_direct_fp32_finish(const double* __restrict__ y, const uint32_t* __restrict__ perm,
+                     const double* __restrict__ part, double* __restrict__ f, size_t n, int nsplit)
 {
 	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
 	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -275,9 +280,111 @@ k_direct_fp32_finish(const double* __restrict__ y, const double* __restrict__ pa
 		f[i] = y[3 * n + i];
 		f[n + i] = y[4 * n + i];
 		f[2 * n + i] = y[5 * n + i];
-		f[3 * n + i] = ax;
-		f[4 * n + i] = ay;
-		f[5 * n + i] = az;
+		const size_t b = perm[i];
+		f[3 * n + b] = ax;
+		f[4 * n + b] = ay;
+		f[5 * n + b] = az;
+	}
+}
+
+// Bounding box of the positions: per work-group minima and maxima ...
+__global__ void __launch_bounds__(256)
+k_bbox_partial(const double* __restrict__ y, size_t n, double* __restrict__ part)
+{
+	__shared__ double sh[6][256];
+	double lo[3] = {INFINITY, INFINITY, INFINITY};
+	double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		for(int d = 0; d < 3; ++d)
+		{
+			const double v = y[d * n + i];
+			lo[d] = fmin(lo[d], v);
+			hi[d] = fmax(hi[d], v);
+		}
+	}
+	for(int d = 0; d < 3; ++d) { sh[d][threadIdx.x] = lo[d]; sh[3 + d][threadIdx.x] = hi[d]; }
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1)
+	{
+		if(static_cast<int>(threadIdx.x) < off)
+		{
+			for(int d = 0; d < 3; ++d)
+			{
+				sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + off]);
+				sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + off]);
+			}
+		}
+		__syncthreads();
+	}
+	if(threadIdx.x < 6) { part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0]; }
+}
+
+// ... folded by one work-group into box[6] = min xyz, max xyz.
+__global__ void __launch_bounds__(256)
+k_bbox_final(const double* __restrict__ part, int g, double* __restrict__ box)
+{
+	__shared__ double sh[6][256];
+	double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+	for(int b = threadIdx.x; b < g; b += blockDim.x)
+	{
+		for(int d = 0; d < 3; ++d)
+		{
+			v[d] = fmin(v[d], part[6 * b + d]);
+			v[3 + d] = fmax(v[3 + d], part[6 * b + 3 + d]);
+		}
+	}
+	for(int d = 0; d < 6; ++d) { sh[d][threadIdx.x] = v[d]; }
+	__syncthreads();
+	for(int off = 128; off > 0; off >>= 1)
+	{
+		if(static_cast<int>(threadIdx.x) < off)
+		{
+			for(int d = 0; d < 3; ++d)
+			{
+				sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + off]);
+				sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + off]);
+			}
+		}
+		__syncthreads();
+	}
+	if(threadIdx.x < 6) { box[threadIdx.x] = sh[threadIdx.x][0]; }
+}
+
+__device__ inline uint32_t spread10(uint32_t x)
+{
+	x = (x | (x << 16)) & 0x030000FFu;
+	x = (x | (x << 8)) & 0x0300F00Fu;
+	x = (x | (x << 4)) & 0x030C30C3u;
+	x = (x | (x << 2)) & 0x09249249u;
+	return x;
+}
+
+// 30-bit Morton keys of the positions on a 1024^3 grid over the box (values:
+// body indices).  Non-finite coordinates map to cell 0.
+__global__ void __launch_bounds__(256)
+k_morton_keys(const double* __restrict__ y, size_t n, const double* __restrict__ box, uint64_t* __restrict__ keys,
+              uint32_t* __restrict__ vals)
+{
+	double lo[3], sc[3];
+	for(int d = 0; d < 3; ++d)
+	{
+		lo[d] = box[d];
+		const double ext = box[3 + d] - box[d];
+		sc[d] = ext > 0.0 ? 1024.0 / ext : 0.0;
+	}
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		uint32_t q[3];
+		for(int d = 0; d < 3; ++d)
+		{
+			const double t = (y[d * n + i] - lo[d]) * sc[d];
+			q[d] = t >= 1023.0 ? 1023u : (t > 0.0 ? static_cast<uint32_t>(t) : 0u);
+		}
+		keys[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+		vals[i] = static_cast<uint32_t>(i);
 	}
 }
 
@@ -328,17 +435,42 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	size_t jchunk = (n + nsplit - 1) / nsplit;
 	jchunk = (jchunk + kF32Tile - 1) / kF32Tile * kF32Tile;
 	nsplit = (n + jchunk - 1) / jchunk;
+	// scratch: partial sums, sort keys / values (two halves each), sort counts,
+	// scan chunk sums, bounding-box partials and the box
+	const unsigned gbox = nbh::grid_for(ctx, n, 256);
+	auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+	const size_t off_keys = align(nsplit * 3 * n * sizeof(double));
+	const size_t off_vals = off_keys + align(2 * n * sizeof(uint64_t));
+	const size_t off_counts = off_vals + align(2 * n * sizeof(uint32_t));
+	const size_t off_scan = off_counts + align(nbh::sort_counts_size(n) * sizeof(uint32_t));
+	const size_t off_bpart = off_scan + align(nbh::sort_scan_size() * sizeof(uint32_t));
+	const size_t off_box = off_bpart + align(6 * static_cast<size_t>(gbox) * sizeof(double));
 	void* sp = nullptr;
-	NBH_TRY(nbh::scratch(ctx, nsplit * 3 * n * sizeof(double), &sp));
-	double* part = static_cast<double*>(sp);
+	NBH_TRY(nbh::scratch(ctx, off_box + align(6 * sizeof(double)), &sp));
+	char* base = static_cast<char*>(sp);
+	double* part = reinterpret_cast<double*>(base);
+	uint64_t* keys = reinterpret_cast<uint64_t*>(base + off_keys);
+	uint32_t* vals = reinterpret_cast<uint32_t*>(base + off_vals);
+	uint32_t* counts = reinterpret_cast<uint32_t*>(base + off_counts);
+	uint32_t* scanp = reinterpret_cast<uint32_t*>(base + off_scan);
+	double* bpart = reinterpret_cast<double*>(base + off_bpart);
+	double* box = reinterpret_cast<double*>(base + off_box);
+	hipLaunchKernelGGL(k_bbox_partial, dim3(gbox), dim3(256), 0, ctx->stream, y, n, bpart);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, ctx->stream, bpart, static_cast<int>(gbox), box);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_morton_keys, dim3(gbox), dim3(256), 0, ctx->stream, y, n, box, keys, vals);
+	NBH_LAUNCHED();
+	uint32_t* perm = nullptr;
+	NBH_TRY(nbh::sort_pairs(ctx, keys, vals, keys + n, vals + n, n, 30, counts, scanp, &perm));
 	{
 		nbh::scoped_timer timer(ctx, "direct_fp32");
 		hipLaunchKernelGGL(k_direct_fp32, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(nsplit)),
-		                   dim3(kF32Threads), 0, ctx->stream, y, mass, part, n, jchunk);
+		                   dim3(kF32Threads), 0, ctx->stream, y, mass, perm, part, n, jchunk);
 		NBH_LAUNCHED();
 	}
 	hipLaunchKernelGGL(k_direct_fp32_finish, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0,
-	                   ctx->stream, y, part, f, n, static_cast<int>(nsplit));
+	                   ctx->stream, y, perm, part, f, n, static_cast<int>(nsplit));
 	NBH_LAUNCHED();
 	return NBH_OK;
 }

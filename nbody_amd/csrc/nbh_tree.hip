@@ -1283,16 +1283,16 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
-nbh_status scan_exclusive(nbh_ctx* ctx, nbh_tree* t, uint32_t* c, size_t m)
+nbh_status scan_exclusive(nbh_ctx* ctx, uint32_t* c, size_t m, uint32_t* scan_part)
 {
 	size_t chunk = (m + kScanParts - 1) / kScanParts;
 	chunk = (chunk + 4 * kScanThreads - 1) / (4 * kScanThreads) * (4 * kScanThreads);
 	const unsigned g = static_cast<unsigned>((m + chunk - 1) / chunk);
-	hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(kScanThreads), 0, ctx->stream, c, m, chunk, t->scan_part);
+	hipLaunchKernelGGL(k_scan_reduce, dim3(g), dim3(kScanThreads), 0, ctx->stream, c, m, chunk, scan_part);
 	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanParts), 0, ctx->stream, t->scan_part, static_cast<int>(g));
+	hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanParts), 0, ctx->stream, scan_part, static_cast<int>(g));
 	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(kScanThreads), 0, ctx->stream, c, m, chunk, t->scan_part);
+	hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(kScanThreads), 0, ctx->stream, c, m, chunk, scan_part);
 	NBH_LAUNCHED();
 	return NBH_OK;
 }
@@ -1300,27 +1300,13 @@ nbh_status scan_exclusive(nbh_ctx* ctx, nbh_tree* t, uint32_t* c, size_t m)
 nbh_status radix_sort(nbh_ctx* ctx, nbh_tree* t, const double* coord, uint32_t* out_vals)
 {
 	const size_t n = t->n;
-	const int nblocks = static_cast<int>((n + kSortTile - 1) / kSortTile);
-	uint64_t* k0 = t->keys;
-	uint64_t* k1 = t->keys + n;
-	uint32_t* v0 = t->vals;
-	uint32_t* v1 = t->vals + n;
-	hipLaunchKernelGGL(k_make_keys, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, coord, k0, v0, n);
+	hipLaunchKernelGGL(k_make_keys, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, coord, t->keys,
+	                   t->vals, n);
 	NBH_LAUNCHED();
-	for(int shift = 0; shift < 64; shift += 8)
-	{
-		hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, n, shift,
-		                   t->counts, nblocks);
-		NBH_LAUNCHED();
-		NBH_TRY(scan_exclusive(ctx, t, t->counts, static_cast<size_t>(nblocks) * 256));
-		hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, v0, k1, v1, n,
-		                   shift, t->counts, nblocks);
-		NBH_LAUNCHED();
-		std::swap(k0, k1);
-		std::swap(v0, v1);
-	}
-	// 8 passes: the result is back in the first half
-	NBH_HIP(hipMemcpyAsync(out_vals, v0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+	uint32_t* sorted = nullptr;
+	NBH_TRY(nbh::sort_pairs(ctx, t->keys, t->vals, t->keys + n, t->vals + n, n, 64, t->counts, t->scan_part,
+	                        &sorted));
+	NBH_HIP(hipMemcpyAsync(out_vals, sorted, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
 	return NBH_OK;
 }
 
@@ -1348,6 +1334,37 @@ nbh_status moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mas
 }
 
 }  // namespace
+
+namespace nbh {
+
+size_t sort_counts_size(size_t n)
+{
+	return ((n + kSortTile - 1) / kSortTile) * 256 + 1;
+}
+
+size_t sort_scan_size() { return kScanParts; }
+
+nbh_status sort_pairs(nbh_ctx* ctx, uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
+                      uint32_t* counts, uint32_t* scan_part, uint32_t** sorted_vals)
+{
+	const int nblocks = static_cast<int>((n + kSortTile - 1) / kSortTile);
+	for(int shift = 0; shift < bits; shift += 8)
+	{
+		hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, n, shift, counts,
+		                   nblocks);
+		NBH_LAUNCHED();
+		NBH_TRY(scan_exclusive(ctx, counts, static_cast<size_t>(nblocks) * 256, scan_part));
+		hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, ctx->stream, k0, v0, k1, v1, n,
+		                   shift, counts, nblocks);
+		NBH_LAUNCHED();
+		std::swap(k0, k1);
+		std::swap(v0, v1);
+	}
+	*sorted_vals = v0;
+	return NBH_OK;
+}
+
+}  // namespace nbh
 
 extern "C" {
 
@@ -1445,7 +1462,7 @@ nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* t, const double* y, const doub
 				hipLaunchKernelGGL(k_part_count, dim3(static_cast<unsigned>(ntiles)), dim3(kPartThreads), 0,
 				                   ctx->stream, t->list[b], t->side, n, t->counts);
 				NBH_LAUNCHED();
-				NBH_TRY(scan_exclusive(ctx, t, t->counts, ntiles));
+				NBH_TRY(scan_exclusive(ctx, t->counts, ntiles, t->scan_part));
 			}
 			hipLaunchKernelGGL(k_part_scatter, dim3(static_cast<unsigned>(ntiles)), dim3(kPartThreads), 0,
 			                   ctx->stream, t->list[b], t->side, n, seg, t->counts, t->tmp);

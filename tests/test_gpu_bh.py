@@ -303,3 +303,37 @@ def test_two_plummer_1m_layout_bit_exact(ctx):
     h.build(y, m, 2.0)
     _compare_trees(t.arrays(), h.arrays(), n)
     t.close()
+
+
+def test_engine_quadrupole_walk_under_rk4():
+    """The hip_bh engine with the quadrupole group walk drives rk4 (tree rebuilt
+    every force evaluation) and its forces are closer to the FP64 direct sum
+    than the monopole walk's at the same opening ratio."""
+    n = 1 << 14
+    errs = {}
+    for mp in ("monopole", "quadrupole"):
+        d = nbody_data()
+        d.set_quiet(True)
+        d.generate("plummer", n, 16)
+        e = nbody_create_engine({"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
+                                 "bh_multipole": mp, "distance_to_node_radius_ratio": 1.6})
+        assert e.init(d)
+        from nbody_amd import nbody_create_solver
+        s = nbody_create_solver({"solver": "rk4", "max_step": 1e-3})
+        s.set_engine(e)
+        for _ in range(3):
+            s.advise(1e-3)
+        assert not e.has_error() and e.get_compute_count() == 12
+        e.get_data(d)
+        y, m = oracle_lib.state_from_data(d)
+        f = e.create_buffer(48 * n)
+        e.fcompute(0.0, e.get_y(), f)
+        a = e.read_buffer(f)[3 * n:].reshape(3, n).T
+        idx = np.random.default_rng(2).choice(n, 512, replace=False)
+        a0 = oracle_lib.accel_targets(y, m, idx.astype(np.uint64), 0)
+        errs[mp] = np.sqrt(np.mean((np.linalg.norm(a[idx] - a0, axis=1) / np.linalg.norm(a0, axis=1)) ** 2))
+        e.free_buffer(f)
+        s.destroy()
+        e.destroy()
+    assert errs["quadrupole"] < 0.3 * errs["monopole"], errs
+
