@@ -84,9 +84,11 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
                        f"(a=1, M=1, seed 4), euler dt=1e-4, tree rebuilt every step", "n": n, "runs": []}
     idx = np.sort(np.random.default_rng(0).choice(n, samples, replace=False)).astype(np.uint32)
     tmpdir = tempfile.mkdtemp(prefix="nbh_corr_")
-    for label, ratio, learned in variants:
+    for label, ratio, learned, *extra in variants:
         params = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
                   "distance_to_node_radius_ratio": ratio, "tree_layout": "heap_stackless"}
+        if extra:
+            params.update(extra[0])
         if learned == "quadrupole":
             params["bh_multipole"] = "quadrupole"
             learned = False
@@ -151,15 +153,18 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
         e.free_buffers([f, dm, di, da])
         run = {
             "variant": label, "ratio": ratio, "theta": 1.0 / ratio,
-            "multipole": params.get("bh_multipole", "monopole"), "steps_per_s": steps / wall,
+            "multipole": params.get("bh_multipole", "monopole"),
+            "quad_near": params.get("bh_quad_near", 4.0) if "bh_multipole" in params else None,
+            "steps_per_s": steps / wall,
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
             "walk_ms": tw.value / max(1, cw.value), "interactions_per_body": cnt[0] / n,
             "force_rel_err": {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
                               "max": float(rel.max()), "samples": samples},
         }
-        if learned:
+        if learned or "bh_multipole" in params:
             run["far_cells_per_group"] = cnt[3] / (n // 64)
             run["near_cells_per_group"] = cnt[4] / (n // 64)
+        if learned:
             run["training"] = train_info
         out["runs"].append(run)
         log(f"BH {label}: {steps / wall:.2f} steps/s, rms err {run['force_rel_err']['rms']:.2e}")
@@ -494,8 +499,11 @@ def main(argv=None) -> int:
         out["cpu_baseline"] = cpu_baseline(yh, mass, n, a.cpu_seconds, threads)
     if not a.no_bh:
         variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False),
-                    ("plain theta=0.25", 4.0, False), ("quadrupole theta=0.5", 2.0, "quadrupole"),
+                    ("plain theta=0.25", 4.0, False),
+                    ("quadrupole theta=0.6, per body", 1.0 / 0.6, "quadrupole", {"bh_quad_near": "inf"}),
+                    ("quadrupole theta=0.5", 2.0, "quadrupole"),
                     ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
+                    ("quadrupole theta=0.7", 1.0 / 0.7, "quadrupole"),
                     ("quadrupole theta=0.8", 1.25, "quadrupole")]
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))

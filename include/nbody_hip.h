@@ -139,10 +139,14 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* tree, const double* y,
 nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f);
 /* nbh_bh_fcompute_group plus the quadrupole term of every internal cell taken as
    a monopole (second central moments of the cell about its mass centre, kept
-   per node by nbh_tree_build / nbh_tree_refit); FP32.  Not in the reference,
-   whose walks are monopole only: an option that lowers the error at a given
-   opening ratio. */
-nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f);
+   per node by nbh_tree_build / nbh_tree_refit); FP32.  A cell whose mass centre
+   lies within near_k half-diagonals of the group's box centre gets the term per
+   body; a farther one gets its quadrupole field and Jacobian at the box centre,
+   once per group, applied to each body to first order (near_k = INFINITY: all
+   per body).  Not in the reference, whose walks are monopole only: an option
+   that lowers the error at a given opening ratio. */
+nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f,
+                                      float near_k);
 /* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may
    be NULL.  cm/bmin/bmax are xyz-interleaved [2n*3]; body_n holds -1 on
    internal nodes.  Used to check the layout against the reference heap
@@ -156,8 +160,9 @@ nbh_status nbh_bh_counters(nbh_ctx* ctx, const nbh_tree* tree, uint64_t* interac
 
 /* Counters of the last traversal: out[0] pair interactions, out[1] node visits,
    out[2] nodes taken as monopoles because the walk stack was full, out[3] far
-   cells corrected by the network and out[4] near cells given their quadrupole
-   term (both of nbh_bh_fcompute_group_mlp). */
+   cells (corrected by the network, or expanded about the group centre by
+   nbh_bh_fcompute_group_quad) and out[4] near cells given their quadrupole term
+   per body. */
 nbh_status nbh_bh_walk_counters(nbh_ctx* ctx, const nbh_tree* tree, uint64_t out[5]);
 
 /* ---- learned correction of the group walk ---------------------------------

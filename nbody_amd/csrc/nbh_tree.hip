@@ -665,6 +665,13 @@ constexpr int kGroup = 64;
 constexpr int kWalkStack = 1024;   // entries of sibling pairs; (levels + 2) * 32 suffice
 constexpr int kWalkBuf = 256;
 
+// Half the diagonal of a box.
+__device__ inline double half_diagonal(double x0, double y0, double z0, double x1, double y1, double z1)
+{
+	const double ex = x1 - x0, ey = y1 - y0, ez = z1 - z0;
+	return 0.5 * sqrt(ex * ex + ey * ey + ez * ez);
+}
+
 // The walk stack holds sibling pairs: an even entry e stands for the nodes e and
 // e + 1, the odd entry kRoot for the root alone.  A step pops up to 32 entries,
 // one node per lane (lane 2i takes the left node of entry i, lane 2i + 1 the
@@ -756,13 +763,52 @@ __device__ inline void put_quad(float4* qb, int slot, const float4& ent, const d
 	qb[3 * slot + 2] = make_float4(static_cast<float>(3.0 * Q[4]), static_cast<float>(3.0 * Q[5]), 0.0f, 0.0f);
 }
 
+// Quadrupole field at the group centre of a cell whose mass centre sits at r
+// from it, and its Jacobian, added to acc[9] (field, then J as xx yy zz xy xz
+// yz).  With Q' = 3 Q - tr(Q) I, s = r.Q'.r and d = |r|:
+//   g = (2.5 s / d^2 r - Q' r) / d^5,
+//   G = dg/dr = (5 (r (Q'r)' + (Q'r) r') + 2.5 s I) / d^7 - 17.5 s r r' / d^9 - Q' / d^5,
+// and since r = cm - x, the field at centre + delta is g - G delta: J = -G.
+__device__ inline void quad_expansion(float rx, float ry, float rz, const double* __restrict__ Q, float* acc)
+{
+	const double tr = Q[0] + Q[1] + Q[2];
+	const float qxx = static_cast<float>(3.0 * Q[0] - tr), qyy = static_cast<float>(3.0 * Q[1] - tr);
+	const float qzz = static_cast<float>(3.0 * Q[2] - tr);
+	const float qxy = static_cast<float>(3.0 * Q[3]), qxz = static_cast<float>(3.0 * Q[4]);
+	const float qyz = static_cast<float>(3.0 * Q[5]);
+	const float Qx = fmaf(qxx, rx, fmaf(qxy, ry, qxz * rz));
+	const float Qy = fmaf(qxy, rx, fmaf(qyy, ry, qyz * rz));
+	const float Qz = fmaf(qxz, rx, fmaf(qyz, ry, qzz * rz));
+	const float sq = fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz));
+	const float ri = __builtin_amdgcn_rsqf(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
+	const float ri2 = ri * ri;
+	const float ri5 = (ri2 * ri2) * ri;
+	const float ri7 = ri5 * ri2;
+	const float c = 2.5f * sq * ri7;
+	acc[0] += fmaf(c, rx, -Qx * ri5);
+	acc[1] += fmaf(c, ry, -Qy * ri5);
+	acc[2] += fmaf(c, rz, -Qz * ri5);
+	const float a = 5.0f * ri7;
+	const float e = 17.5f * sq * (ri7 * ri2);
+	acc[3] -= a * (2.0f * rx * Qx) + c - e * rx * rx - qxx * ri5;
+	acc[4] -= a * (2.0f * ry * Qy) + c - e * ry * ry - qyy * ri5;
+	acc[5] -= a * (2.0f * rz * Qz) + c - e * rz * rz - qzz * ri5;
+	acc[6] -= a * (rx * Qy + ry * Qx) - e * rx * ry - qxy * ri5;
+	acc[7] -= a * (rx * Qz + rz * Qx) - e * rx * rz - qxz * ri5;
+	acc[8] -= a * (ry * Qz + rz * Qy) - e * ry * rz - qyz * ri5;
+}
+
 // kQuad: internal cells taken as monopoles also contribute their quadrupole
-// term (second central moments `quad`) to every body of the group.
+// term (second central moments `quad`).  A cell whose mass centre lies within
+// `near` of the group centre gets it per body (fused monopole + quadrupole
+// sweep); a farther one gets the monopole per body and its quadrupole field
+// expanded to first order about the group centre, once per group.
 template <bool kQuad>
 __global__ void __launch_bounds__(kGroup)
 k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
              const double* __restrict__ bmax, const double* __restrict__ quad, const int32_t* __restrict__ body_n,
-             const double* __restrict__ y, double* __restrict__ f, int n, unsigned long long* __restrict__ counters)
+             const double* __restrict__ y, double* __restrict__ f, int n, float near_k,
+             unsigned long long* __restrict__ counters)
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
@@ -779,7 +825,11 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	const float tz = static_cast<float>(me.z - ccz);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	unsigned long long vis = 0, taken = 0, spill = 0;
+	unsigned long long vis = 0, taken = 0, spill = 0, nfar = 0, nnear = 0;
+	float acc[9];
+	for(int k = 0; k < 9; ++k) { acc[k] = 0.0f; }
+	const float near_d = static_cast<float>(static_cast<double>(near_k) *
+	                                        half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));
 
 	int sp = 1;
 	int nbuf = 0;
@@ -827,13 +877,19 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		const uint64_t bopen2 = __ballot(open);
 		if constexpr(kQuad)
 		{
-			// taken internal cells: monopole + quadrupole sweep; the rest: monopole sweep
-			const uint64_t bcell = __ballot(cell);
-			const uint64_t bmono = bacc & ~bcell;
-			if(cell) { put_quad(qbuf, nq + __popcll(bcell & lt), ent, quad + 6 * node); }
+			// near taken internal cells: monopole + quadrupole sweep; the rest:
+			// monopole sweep, far cells adding their expansion
+			const bool far = cell && !(ent.x * ent.x + ent.y * ent.y + ent.z * ent.z < near_d * near_d);
+			const bool nearc = cell && !far;
+			const uint64_t bnear = __ballot(nearc);
+			const uint64_t bmono = bacc & ~bnear;
+			if(nearc) { put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node); }
 			else if(accept) { buf[nbuf + __popcll(bmono & lt)] = ent; }
-			nq += __popcll(bcell);
+			if(far) { quad_expansion(ent.x, ent.y, ent.z, quad + 6 * node, acc); }
+			nq += __popcll(bnear);
 			nbuf += __popcll(bmono);
+			nfar += far ? 1 : 0;
+			nnear += nearc ? 1 : 0;
 		}
 		else
 		{
@@ -865,29 +921,41 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	{
 		sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
 	}
+	float kx = 0.0f, ky = 0.0f, kz = 0.0f;
 	if constexpr(kQuad)
 	{
 		if(nq > 0) { sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz); }
+		for(int sh = 1; sh < 64; sh <<= 1)
+		{
+			for(int k = 0; k < 9; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
+		}
+		kx = acc[0] + acc[3] * tx + acc[6] * ty + acc[7] * tz;
+		ky = acc[1] + acc[6] * tx + acc[4] * ty + acc[8] * tz;
+		kz = acc[2] + acc[7] * tx + acc[8] * ty + acc[5] * tz;
 	}
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
 	f[n + b] = y[4 * n + b];
 	f[2 * n + b] = y[5 * n + b];
-	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx);
-	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy);
-	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz);
+	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx) + static_cast<double>(kx);
+	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy) + static_cast<double>(ky);
+	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz) + static_cast<double>(kz);
 	// counters: [0] pair interactions (entries x 64 targets), [1] node visits, [2] spills
 	for(int off = 32; off > 0; off >>= 1)
 	{
 		vis += __shfl_down(vis, off, 64);
 		taken += __shfl_down(taken, off, 64);
 		spill += __shfl_down(spill, off, 64);
+		nfar += __shfl_down(nfar, off, 64);
+		nnear += __shfl_down(nnear, off, 64);
 	}
 	if(lane == 0)
 	{
 		atomicAdd(&counters[0], taken * kGroup);
 		atomicAdd(&counters[1], vis);
 		if(spill) { atomicAdd(&counters[2], spill); }
+		if(nfar) { atomicAdd(&counters[3], nfar); }
+		if(nnear) { atomicAdd(&counters[4], nnear); }
 	}
 }
 
@@ -920,12 +988,6 @@ __device__ inline int classify_cell(const double* lo, const double* hi, const do
 	if(dc < static_cast<double>(near_k) * half_diag) { return kCellNear; }
 	const double r = cell_radius(lo, hi, rec.x, rec.y, rec.z);
 	return r > static_cast<double>(rho_min) * dc ? kCellFar : kCellNone;
-}
-
-__device__ inline double half_diagonal(double x0, double y0, double z0, double x1, double y1, double z1)
-{
-	const double ex = x1 - x0, ey = y1 - y0, ez = z1 - z0;
-	return 0.5 * sqrt(ex * ex + ey * ey + ez * ez);
 }
 
 // Group walk without forces: emit (group ordinal, node) for every far cell.
@@ -1519,7 +1581,8 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* t, const double* y, con
 	return NBH_OK;
 }
 
-static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, bool quadrupole)
+static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, bool quadrupole,
+                             float near_k)
 {
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
@@ -1534,12 +1597,12 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 	if(quadrupole)
 	{
 		hipLaunchKernelGGL(k_bh_group32<true>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass,
-		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, t->counters);
+		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, near_k, t->counters);
 	}
 	else
 	{
 		hipLaunchKernelGGL(k_bh_group32<false>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass,
-		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, t->counters);
+		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, 0.0f, t->counters);
 	}
 	NBH_LAUNCHED();
 	return NBH_OK;
@@ -1547,12 +1610,13 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 
 nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f)
 {
-	return group_walk(ctx, t, y, f, false);
+	return group_walk(ctx, t, y, f, false, 0.0f);
 }
 
-nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f)
+nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, float near_k)
 {
-	return group_walk(ctx, t, y, f, true);
+	NBH_CHECK(near_k >= 0.0f, "near_k must be >= 0");
+	return group_walk(ctx, t, y, f, true, near_k);
 }
 
 nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* groups, size_t n_groups,

@@ -642,6 +642,7 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_tree_built(false),
 	m_group_walk(false),
 	m_quadrupole(false),
+	m_quad_near(4.0f),
 	m_net_rho(0.0f),
 	m_net_near(0.0f),
 	m_net(nullptr)
@@ -780,7 +781,8 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 		}
 		if(m_quadrupole)
 		{
-			check(nbh_bh_fcompute_group_quad(m_ctx, m_tree, y->data(), f->data()), "bh fcompute (quadrupole walk)");
+			check(nbh_bh_fcompute_group_quad(m_ctx, m_tree, y->data(), f->data(), m_quad_near),
+			      "bh fcompute (quadrupole walk)");
 			return;
 		}
 		check(nbh_bh_fcompute_group(m_ctx, m_tree, y->data(), f->data()), "bh fcompute (group walk)");
@@ -800,6 +802,10 @@ void nbody_engine_hip_bh::print_info() const
 	nbody_print("\ttree_build_rate: %zu", m_tree_build_rate);
 	nbody_print("\tbh_walk: %s", m_group_walk ? "group" : "body");
 	nbody_print("\tbh_multipole: %s", m_quadrupole ? "quadrupole" : "monopole");
+	if(m_quadrupole)
+	{
+		nbody_print("\tbh_quad_near: %g", static_cast<double>(m_quad_near));
+	}
 	nbody_print("\tlearned_correction: %s", m_net_params.empty() ? "none" : m_net_path.c_str());
 	if(!m_net_params.empty())
 	{

@@ -117,6 +117,7 @@ class nbody_engine_hip_bh : public nbody_engine_hip
 	bool            m_tree_built;
 	bool            m_group_walk;
 	bool            m_quadrupole;
+	float           m_quad_near;
 	std::vector<float> m_net_params;   // learned correction (empty: none)
 	float           m_net_rho;
 	float           m_net_near;
@@ -129,6 +130,9 @@ public:
 	void set_group_walk(bool g) { m_group_walk = g; }
 	//! Add the quadrupole term of taken cells to the group walk.
 	void set_quadrupole(bool q) { m_quadrupole = q; }
+	//! Cells within near_k group half-diagonals get the quadrupole per body,
+	//! farther ones through an expansion about the group centre.
+	void set_quad_near(float near_k) { m_quad_near = near_k; }
 	//! Read a correction network file ("NBHMLP01", int32 parameter count,
 	//! float32 rho_min, float32 near_k, float32 parameters); rho >= 0 and
 	//! near_k >= 0 override the thresholds stored in it.  False if the file is

@@ -120,6 +120,12 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("bh_multipole=quadrupole needs bh_walk=group");
 			return nullptr;
 		}
+		double quad_near = 4.0;
+		if(!param.get_double("bh_quad_near", 4.0, &quad_near) || !(quad_near >= 0.0))
+		{
+			nbody_log("Invalid bh_quad_near");
+			return nullptr;
+		}
 		// learned correction of the group walk (no reference counterpart)
 		std::string net = param.value("learned_correction", "none");
 		double rho = -1.0;
@@ -142,6 +148,7 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 		nbody_engine_hip_bh* engine = new nbody_engine_hip_bh(ratio, tt, static_cast<size_t>(rate), tl);
 		engine->set_group_walk(walk == "group");
 		engine->set_quadrupole(multipole == "quadrupole");
+		engine->set_quad_near(static_cast<float>(quad_near));
 		if(net != "none" && !engine->load_correction(net, rho, near_k))
 		{
 			delete engine;

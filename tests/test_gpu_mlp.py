@@ -359,7 +359,7 @@ def test_quadrupole_walk_equals_all_near_cells(ctx):
     y, m = _state("plummer", n, 25)
     t = Tree(ctx, y, m, ratio)
     df = ctx.alloc(6 * n)
-    native.check(ctx.lib.nbh_bh_fcompute_group_quad(ctx.h, t.h, t.dy.ptr, df.ptr), "quad")
+    native.check(ctx.lib.nbh_bh_fcompute_group_quad(ctx.h, t.h, t.dy.ptr, df.ptr, float("inf")), "quad")
     quad = df.read()
     assert t.counters()[2] == 0                 # no stack spills: identical cell sets
     net = Net(ctx)
@@ -371,6 +371,55 @@ def test_quadrupole_walk_equals_all_near_cells(ctx):
     e_mono, e_quad = _rms_rel(t.group(), a0, n, idx), _rms_rel(quad, a0, n, idx)
     assert e_quad[0] < 0.5 * e_mono[0], (e_quad, e_mono)
     net.close()
+    t.close()
+
+
+def test_quadrupole_walk_far_expansion_matches_oracle(ctx):
+    """Quadrupole walk with far cells expanded about the group centre: the added
+    term equals the oracle's per-body quadrupole of the near cells plus, for the
+    far cells, the quadrupole field at the centre and its finite-difference
+    Jacobian applied to each body's offset."""
+    n, ratio, near_k = 4096, 1.6, 3.0
+    y, m = _state("plummer", n, 26)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    body = _body_order(ctx, t)
+    plain = t.group()
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group_quad(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "quad")
+    got = df.read()
+    c = t.counters()
+    assert c[3] > 0 and c[4] > 0
+    delta = (got - plain)[3 * n:].reshape(3, n).T
+    want = np.zeros((n, 3))
+    for g in range(n // 64):
+        gn = n // 64 + g
+        cen = 0.5 * (a["bmin"][gn] + a["bmax"][gn])
+        # far/near split of every taken internal cell (rho_min 0: all far cells count)
+        far, _, near = mo.group_band(n, g, a["cm"], a["crit"], a["bmin"], a["bmax"], 0.0, near_k)
+        R = np.zeros(3)
+        J = np.zeros((3, 3))
+        for cell in far:
+            l0, l1 = _leaf_slice(n, cell)
+            _, Q = mo.second_moments(a["cm"][l0:l1], a["mass"][l0:l1])
+            R += mo.quadrupole_field(cen, a["cm"][cell], Q)
+            h = 1e-4 * np.linalg.norm(a["cm"][cell] - cen)
+            for j in range(3):
+                e = np.zeros(3); e[j] = h
+                J[:, j] += (mo.quadrupole_field(cen + e, a["cm"][cell], Q) -
+                            mo.quadrupole_field(cen - e, a["cm"][cell], Q)) / (2 * h)
+        Qs = []
+        for cell in near:
+            l0, l1 = _leaf_slice(n, cell)
+            Qs.append(mo.second_moments(a["cm"][l0:l1], a["mass"][l0:l1])[1])
+        for leaf in range(n + 64 * g, n + 64 * g + 64):
+            x = a["cm"][leaf]
+            v = R + J @ (x - cen)
+            for cell, Q in zip(near, Qs):
+                v = v + mo.quadrupole_field(x, a["cm"][cell], Q)
+            want[body[leaf]] = v
+    err = np.linalg.norm(delta - want, axis=1) / np.abs(want).max()
+    assert np.percentile(err, 99) < 1e-3 and err.max() < 5e-3, (np.percentile(err, 99), err.max())
     t.close()
 
 
