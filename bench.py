@@ -154,7 +154,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
         run = {
             "variant": label, "ratio": ratio, "theta": 1.0 / ratio,
             "multipole": params.get("bh_multipole", "monopole"),
-            "quad_near": params.get("bh_quad_near", 4.0) if "bh_multipole" in params else None,
+            "quad_near": params.get("bh_quad_near", 3.0) if "bh_multipole" in params else None,
             "steps_per_s": steps / wall,
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
             "walk_ms": tw.value / max(1, cw.value), "interactions_per_body": cnt[0] / n,
@@ -500,10 +500,9 @@ def main(argv=None) -> int:
     if not a.no_bh:
         variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False),
                     ("plain theta=0.25", 4.0, False),
-                    ("quadrupole theta=0.6, per body", 1.0 / 0.6, "quadrupole", {"bh_quad_near": "inf"}),
-                    ("quadrupole theta=0.5", 2.0, "quadrupole"),
+                    ("quadrupole theta=0.6, all cells per body", 1.0 / 0.6, "quadrupole", {"bh_quad_near": "inf"}),
                     ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
-                    ("quadrupole theta=0.7", 1.0 / 0.7, "quadrupole"),
+                    ("quadrupole theta=0.6, near 2", 1.0 / 0.6, "quadrupole", {"bh_quad_near": 2.0}),
                     ("quadrupole theta=0.8", 1.25, "quadrupole")]
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))

@@ -642,7 +642,7 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_tree_built(false),
 	m_group_walk(false),
 	m_quadrupole(false),
-	m_quad_near(4.0f),
+	m_quad_near(3.0f),
 	m_net_rho(0.0f),
 	m_net_near(0.0f),
 	m_net(nullptr)

@@ -120,8 +120,8 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("bh_multipole=quadrupole needs bh_walk=group");
 			return nullptr;
 		}
-		double quad_near = 4.0;
-		if(!param.get_double("bh_quad_near", 4.0, &quad_near) || !(quad_near >= 0.0))
+		double quad_near = 3.0;
+		if(!param.get_double("bh_quad_near", 3.0, &quad_near) || !(quad_near >= 0.0))
 		{
 			nbody_log("Invalid bh_quad_near");
 			return nullptr;
