@@ -62,7 +62,8 @@ def cpu_baseline(y, m, n, budget_s: float, threads: int) -> dict:
                       f"{t:.1f} s on {threads} host threads"}
 
 
-def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> dict:
+def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kind: str = "plummer",
+             seed: int = 4, rk4_steps: int = 2) -> dict:
     """Barnes-Hut euler steps/s (one fcompute = tree build + group walk, plus one
     fmadd_inplace, as the reference bench's step) on a Plummer sphere of n bodies,
     with the relative force error on `samples` random bodies against FP64 direct
@@ -77,11 +78,14 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
     hip = native.hip_lib()
     data = nbody_data()
     data.set_quiet(True)
-    log(f"BH: generating Plummer N={n}")
-    data.generate("plummer", n, 4)
+    log(f"BH: generating {kind} N={n}")
+    data.generate(kind, n, seed)
     _, _, mass, _ = data.bodies()
-    out = {"workload": f"Barnes-Hut heap kd-tree, group walk, FP32 forces / FP64 state, Plummer N={n} "
-                       f"(a=1, M=1, seed 4), euler dt=1e-4, tree rebuilt every step", "n": n, "runs": []}
+    desc = {"plummer": "Plummer sphere (a=1, M=1)",
+            "two_plummer": "two Plummer spheres of n/2 (a=1, M=1/2 each), centres 10 apart, approaching at 0.5"}
+    out = {"workload": f"Barnes-Hut heap kd-tree, FP32 forces / FP64 state, {desc.get(kind, kind)} N={n}, "
+                       f"seed {seed}, euler dt=1e-4 (and rk4 steps/s), tree rebuilt every force evaluation",
+           "n": n, "runs": []}
     idx = np.sort(np.random.default_rng(0).choice(n, samples, replace=False)).astype(np.uint32)
     tmpdir = tempfile.mkdtemp(prefix="nbh_corr_")
     for label, ratio, learned, *extra in variants:
@@ -151,6 +155,18 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
         a0 = e.read_buffer(da).reshape(-1, 3)
         rel = np.linalg.norm(fa - a0, axis=1) / np.linalg.norm(a0, axis=1)
         e.free_buffers([f, dm, di, da])
+        rk4_rate = None
+        if rk4_steps > 0:
+            s4 = nbody_create_solver({"solver": "rk4", "max_step": 1e-4})
+            s4.set_engine(e)
+            s4.advise(1e-4)
+            native.check(hip.nbh_ctx_synchronize(ctx))
+            t0 = time.perf_counter()
+            for _ in range(rk4_steps):
+                s4.advise(1e-4)
+            native.check(hip.nbh_ctx_synchronize(ctx))
+            rk4_rate = rk4_steps / (time.perf_counter() - t0)
+            s4.destroy()
         run = {
             "variant": label, "ratio": ratio, "theta": 1.0 / ratio,
             "multipole": params.get("bh_multipole", "monopole"),
@@ -158,6 +174,10 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192) -> 
             "steps_per_s": steps / wall,
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
             "walk_ms": tw.value / max(1, cw.value), "interactions_per_body": cnt[0] / n,
+            "node_visits_per_group": cnt[1] / max(1, n // 64),
+            # node records read by the walk: 32 B (cm, crit_r2) + 8 B mass per visit
+            "node_GB_per_s": 40.0 * cnt[1] / (tw.value / max(1, cw.value) * 1e-3) / 1e9,
+            "rk4_steps_per_s": rk4_rate,
             "force_rel_err": {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
                               "max": float(rel.max()), "samples": samples},
         }
@@ -364,6 +384,8 @@ def main(argv=None) -> int:
     ap.add_argument("--bh-steps", type=int, default=5)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction Barnes-Hut run")
     ap.add_argument("--no-vector-ops", action="store_true", help="skip the vector-op bandwidth block")
+    ap.add_argument("--no-two-plummer", action="store_true",
+                    help="skip the colliding two-Plummer Barnes-Hut workload (N=2^20)")
     ap.add_argument("--split-precision", type=int, default=0, metavar="STEPS",
                     help="also run the cold-cube split-precision workload (N=65536) for STEPS steps")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -507,6 +529,10 @@ def main(argv=None) -> int:
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))
         out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)
+    if not a.no_bh and not a.no_two_plummer:
+        out["two_plummer"] = bh_bench(1 << 20, 3, 1, [("monopole theta=0.5, group walk", 2.0, False),
+                                                      ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole")],
+                                      kind="two_plummer", seed=3, rk4_steps=3)
     if not a.no_vector_ops:
         log("vector ops")
         out["vector_ops"] = vector_ops_bench(1 << 22)
