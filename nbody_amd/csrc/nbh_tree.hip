@@ -840,13 +840,33 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	{
 		bool valid;
 		const int node = walk_pop(stack, sp, lane, valid);
+		// issue this step's node loads, then sweep the buffers the previous steps
+		// filled while they are in flight
+		double4 rec = make_double4(0.0, 0.0, 0.0, 0.0);
+		double m = 0.0;
+		if(valid)
+		{
+			rec = xyzr[node];
+			m = nmass[node];
+		}
+		if(nbuf > kWalkBuf - kGroup)
+		{
+			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
+			nbuf = 0;
+		}
+		if constexpr(kQuad)
+		{
+			if(nq > kQuadBuf - kGroup)
+			{
+				sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
+				nq = 0;
+			}
+		}
 		__syncthreads();
 		bool accept = false, open = false, cell = false;
 		float4 ent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		if(valid)
 		{
-			const double4 rec = xyzr[node];
-			const double m = nmass[node];
 			ent = make_float4(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
 			                  static_cast<float>(rec.z - ccz), static_cast<float>(m));
 			if(node >= n)
@@ -901,21 +921,6 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		vis += valid ? 1 : 0;
 		taken += accept ? 1 : 0;
 		__syncthreads();
-		if(nbuf > kWalkBuf - kGroup)
-		{
-			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
-			nbuf = 0;
-			__syncthreads();
-		}
-		if constexpr(kQuad)
-		{
-			if(nq > kQuadBuf - kGroup)
-			{
-				sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
-				nq = 0;
-				__syncthreads();
-			}
-		}
 	}
 	if(nbuf > 0)
 	{
@@ -1214,17 +1219,31 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	{
 		bool valid;
 		const int node = walk_pop(stack, sp, lane, valid);
-		__syncthreads();
-		bool accept = false, open = false;
-		int cls = kCellNone;
-		float4 ent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		// node loads first, then the pending sweeps (same order as k_bh_group32)
 		double4 rec = make_double4(0.0, 0.0, 0.0, 0.0);
 		double m = 0.0;
-		double dc = 0.0;
 		if(valid)
 		{
 			rec = xyzr[node];
 			m = nmass[node];
+		}
+		if(nbuf > kWalkBuf - kGroup)
+		{
+			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
+			nbuf = 0;
+		}
+		if(nq > kQuadBuf - kGroup)
+		{
+			sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
+			nq = 0;
+		}
+		__syncthreads();
+		bool accept = false, open = false;
+		int cls = kCellNone;
+		float4 ent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		double dc = 0.0;
+		if(valid)
+		{
 			ent = make_float4(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
 			                  static_cast<float>(rec.z - ccz), static_cast<float>(m));
 			if(node >= n)
@@ -1285,18 +1304,6 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		vis += valid ? 1 : 0;
 		taken += accept ? 1 : 0;
 		__syncthreads();
-		if(nbuf > kWalkBuf - kGroup)
-		{
-			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
-			nbuf = 0;
-			__syncthreads();
-		}
-		if(nq > kQuadBuf - kGroup)
-		{
-			sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
-			nq = 0;
-			__syncthreads();
-		}
 		if(nband >= 16) { run_tiles(nband, false); }
 	}
 	if(nbuf > 0)
