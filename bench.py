@@ -33,7 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "pair-interactions/s (direct); BH steps/s at N=4M, force err ≤1e-3 vs FP64"
 FLOP_PER_PAIR = 20.0
-VALU_PER_PAIR = 15           # inner loop of k_direct_fp32 per pair (14 VALU + v_rsq_f32)
+VALU_PER_PAIR = 13           # inner loop of k_direct_fp32 per pair (12 VALU + v_rsq_f32 with clamp)
 PEAK_FP32_TFLOPS = 157.3
 
 
@@ -505,10 +505,12 @@ def main(argv=None) -> int:
             "avg_kernel_ms": avg_ms,
             "launches": cnt.value,
             "pairs_per_s_kernel": n * n / (avg_ms * 1e-3),
-            # issued instructions: 14 VALU + 1 v_rsq_f32 per pair against the
-            # 157.3e12 / 2 lane-FMA/s behind the FP32 peak
+            # issued instructions: 12 VALU + 1 v_rsq_f32 per pair against the
+            # 157.3e12 / 2 lane-instructions/s behind the FP32 peak, and as wave
+            # instructions per SIMD per clock (256 CUs x 4 SIMDs, 2.4 GHz)
             "valu_instr_per_pair": VALU_PER_PAIR,
             "valu_issue_frac": VALU_PER_PAIR * n * n / (avg_ms * 1e-3) / (peak * 1e12 / 2),
+            "valu_wave_instr_per_simd_clk": VALU_PER_PAIR * n * n / 64 / (avg_ms * 1e-3) / (1024 * 2.4e9),
         },
         "force_error": {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "max_rel": float(rel.max()),
                         "vs": "FP64 direct kernel (bit-exact with the CPU oracle)"},

@@ -154,6 +154,21 @@ constexpr int kF32Tgt = 4;         // targets per work-item
 constexpr int kF32Tile = 256;      // sources per LDS tile (one per work-item)
 constexpr int kF32TgtPerGroup = kF32Threads * kF32Tgt;
 
+// The tiles work in scaled units: offsets x' = 1e4 x and masses m' = 1e8 m (both
+// formed in FP64 and rounded once).  Then r'^2 = 1e8 r^2, and the reference's
+// clamp r^2 >= 1e-8 (nbody/nbody_data.cpp:35-44) becomes r'^2 >= 1: with
+// q = min(1/r', 1), the rsq instruction's own [0, 1] output clamp,
+//   q = 1e-4 / sqrt(max(r^2, 1e-8)),
+// and m' x' q^3 = m x / max(r^2, 1e-8)^{3/2} with no rescaling of the sums.
+// A coincident pair (r' = 0) gets q = 1 and contributes m' * 0 = 0.
+constexpr double kF32Len = 1e4;
+constexpr double kF32Mass = 1e8;
+
+__device__ inline float rsq_clamped(float r2)
+{
+	return __builtin_amdgcn_fmed3f(__builtin_amdgcn_rsqf(r2), 0.0f, 1.0f);   // folds into v_rsq_f32 clamp
+}
+
 __device__ inline void kahan_f32(float& s, float& c, float v)
 {
 	float corrected = v - c;
@@ -187,16 +202,15 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, con
 		size_t i = tbase + static_cast<size_t>(k) * kF32Threads + threadIdx.x;
 		bool ok = i < n;
 		const size_t b = ok ? perm[i] : 0;
-		xi[k] = ok ? static_cast<float>(y[b] - ox) : 0.0f;
-		yi[k] = ok ? static_cast<float>(y[n + b] - oy) : 0.0f;
-		zi[k] = ok ? static_cast<float>(y[2 * n + b] - oz) : 0.0f;
+		xi[k] = ok ? static_cast<float>((y[b] - ox) * kF32Len) : 0.0f;
+		yi[k] = ok ? static_cast<float>((y[n + b] - oy) * kF32Len) : 0.0f;
+		zi[k] = ok ? static_cast<float>((y[2 * n + b] - oz) * kF32Len) : 0.0f;
 		sx[k] = sy[k] = sz[k] = 0.0f;
 		cx[k] = cy[k] = cz[k] = 0.0f;
 	}
 
 	const size_t jbeg = static_cast<size_t>(blockIdx.y) * jchunk;
 	const size_t jend = jbeg + jchunk < n ? jbeg + jchunk : n;
-	const float rmin2 = static_cast<float>(kMinDistance);
 
 	for(size_t base = jbeg; base < jend; base += kF32Tile)
 	{
@@ -204,10 +218,10 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, con
 		__syncthreads();
 		if(j < jend)
 		{
-			tile[threadIdx.x] = make_float4(static_cast<float>(y[j] - ox),
-			                                static_cast<float>(y[n + j] - oy),
-			                                static_cast<float>(y[2 * n + j] - oz),
-			                                static_cast<float>(mass[j]));
+			tile[threadIdx.x] = make_float4(static_cast<float>((y[j] - ox) * kF32Len),
+			                                static_cast<float>((y[n + j] - oy) * kF32Len),
+			                                static_cast<float>((y[2 * n + j] - oz) * kF32Len),
+			                                static_cast<float>(mass[j] * kF32Mass));
 		}
 		else
 		{
@@ -229,10 +243,8 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, con
 				const float dx = s.x - xi[k];
 				const float dy = s.y - yi[k];
 				const float dz = s.z - zi[k];
-				float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-				r2 = fmaxf(r2, rmin2);
-				const float ri = __builtin_amdgcn_rsqf(r2);
-				const float w = (s.w * ri) * (ri * ri);
+				const float q = rsq_clamped(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+				const float w = (s.w * q) * (q * q);
 				px[k] = fmaf(dx, w, px[k]);
 				py[k] = fmaf(dy, w, py[k]);
 				pz[k] = fmaf(dz, w, pz[k]);
