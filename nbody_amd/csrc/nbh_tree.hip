@@ -694,6 +694,29 @@ __device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid)
 	return node;
 }
 
+// The group walks sweep in scaled units: offsets x' = 1e4 x, masses m' = 1e8 m
+// and second moments Q' = 1e16 Q (mass x length^2), each formed in FP64 and
+// rounded once.  r'^2 >= 1 is then the reference's clamp r^2 >= 1e-8
+// (nbody/nbody_data.cpp:35-44), which the [0, 1] output clamp of v_rsq_f32
+// applies at no cost: q = min(1/r', 1) = 1e-4 / sqrt(max(r^2, 1e-8)), and the
+// monopole term m' x' q^3 and the quadrupole term below come out in unscaled
+// acceleration units.
+constexpr double kSweepLen = 1e4;
+constexpr double kSweepMass = 1e8;
+constexpr double kSweepQuad = 1e16;
+
+__device__ inline float rsq_clamped(float r2)
+{
+	return __builtin_amdgcn_fmed3f(__builtin_amdgcn_rsqf(r2), 0.0f, 1.0f);   // folds into v_rsq_f32 clamp
+}
+
+// Sweep entry (scaled offset from the group centre, scaled mass) of a node.
+__device__ inline float4 sweep_entry(const double4& rec, double m, double ccx, double ccy, double ccz)
+{
+	return make_float4(static_cast<float>((rec.x - ccx) * kSweepLen), static_cast<float>((rec.y - ccy) * kSweepLen),
+	                   static_cast<float>((rec.z - ccz) * kSweepLen), static_cast<float>(m * kSweepMass));
+}
+
 __device__ inline void sweep32(const float4* __restrict__ buf, int count, float tx, float ty, float tz,
                                float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
 {
@@ -705,9 +728,7 @@ __device__ inline void sweep32(const float4* __restrict__ buf, int count, float 
 		const float dx = s.x - tx;
 		const float dy = s.y - ty;
 		const float dz = s.z - tz;
-		float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-		r2 = fmaxf(r2, 1e-8f);
-		const float ri = __builtin_amdgcn_rsqf(r2);
+		const float ri = rsq_clamped(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
 		const float w = (s.w * ri) * (ri * ri);
 		px = fmaf(dx, w, px);
 		py = fmaf(dy, w, py);
@@ -722,8 +743,9 @@ __device__ inline void sweep32(const float4* __restrict__ buf, int count, float 
 // of 3 float4: (mass-centre offset from the group centre, M), (Q'xx Q'yy Q'zz
 // Q'xy), (Q'xz Q'yz - -), with Q' = 3 Q - tr(Q) I the traceless second moment
 // about the mass centre; r = cm - x, d = |r|:
-//   a = M r / d^3 + (2.5 (r.Q'.r) / d^2 r - Q' r) / d^5.
-// The monopole part is summed like sweep32 (same operations, Kahan per sweep).
+//   a = M r / d^3 + (2.5 (r.Q'.r) / d^2 r - Q' r) / d^5,
+// all in the scaled units of sweep32 (Q' times kSweepQuad).  The monopole part
+// is summed like sweep32 (same operations, Kahan per sweep).
 __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float tx, float ty, float tz,
                                 float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
 {
@@ -732,9 +754,7 @@ __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float 
 	{
 		const float4 e0 = qb[3 * j], e1 = qb[3 * j + 1], e2 = qb[3 * j + 2];
 		const float rx = e0.x - tx, ry = e0.y - ty, rz = e0.z - tz;
-		float r2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
-		r2 = fmaxf(r2, 1e-8f);
-		const float ri = __builtin_amdgcn_rsqf(r2);
+		const float ri = rsq_clamped(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
 		const float ri2 = ri * ri;
 		const float w = (e0.w * ri) * ri2;
 		const float ri5 = (ri2 * ri2) * ri;
@@ -753,14 +773,16 @@ __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float 
 
 constexpr int kQuadBuf = 96;     // cells waiting for the monopole + quadrupole sweep
 
-// Entry of node `node` (mass-centre offset and mass in ent) at slot `slot` of qb.
+// Entry of a node (scaled mass-centre offset and mass in ent, second moments Q)
+// at slot `slot` of qb.
 __device__ inline void put_quad(float4* qb, int slot, const float4& ent, const double* __restrict__ Q)
 {
 	const double tr = Q[0] + Q[1] + Q[2];
+	const double k = kSweepQuad;
 	qb[3 * slot] = ent;
-	qb[3 * slot + 1] = make_float4(static_cast<float>(3.0 * Q[0] - tr), static_cast<float>(3.0 * Q[1] - tr),
-	                               static_cast<float>(3.0 * Q[2] - tr), static_cast<float>(3.0 * Q[3]));
-	qb[3 * slot + 2] = make_float4(static_cast<float>(3.0 * Q[4]), static_cast<float>(3.0 * Q[5]), 0.0f, 0.0f);
+	qb[3 * slot + 1] = make_float4(static_cast<float>((3.0 * Q[0] - tr) * k), static_cast<float>((3.0 * Q[1] - tr) * k),
+	                               static_cast<float>((3.0 * Q[2] - tr) * k), static_cast<float>(3.0 * Q[3] * k));
+	qb[3 * slot + 2] = make_float4(static_cast<float>(3.0 * Q[4] * k), static_cast<float>(3.0 * Q[5] * k), 0.0f, 0.0f);
 }
 
 // Quadrupole field at the group centre of a cell whose mass centre sits at r
@@ -820,16 +842,16 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
 	const int tn = n + blockIdx.x * kGroup + lane;     // this lane's leaf
 	const double4 me = xyzr[tn];
-	const float tx = static_cast<float>(me.x - ccx);
-	const float ty = static_cast<float>(me.y - ccy);
-	const float tz = static_cast<float>(me.z - ccz);
+	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
+	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
+	const float tz = static_cast<float>((me.z - ccz) * kSweepLen);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	unsigned long long vis = 0, taken = 0, spill = 0, nfar = 0, nnear = 0;
 	float acc[9];
 	for(int k = 0; k < 9; ++k) { acc[k] = 0.0f; }
-	const float near_d = static_cast<float>(static_cast<double>(near_k) *
-	                                        half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));
+	const float near_d = static_cast<float>(static_cast<double>(near_k) * kSweepLen *
+	                                        half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));   // scaled
 
 	int sp = 1;
 	int nbuf = 0;
@@ -867,8 +889,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		float4 ent = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		if(valid)
 		{
-			ent = make_float4(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
-			                  static_cast<float>(rec.z - ccz), static_cast<float>(m));
+			ent = sweep_entry(rec, m, ccx, ccy, ccz);
 			if(node >= n)
 			{
 				accept = true;
@@ -905,7 +926,11 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			const uint64_t bmono = bacc & ~bnear;
 			if(nearc) { put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node); }
 			else if(accept) { buf[nbuf + __popcll(bmono & lt)] = ent; }
-			if(far) { quad_expansion(ent.x, ent.y, ent.z, quad + 6 * node, acc); }
+			if(far)
+			{
+				quad_expansion(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
+				               static_cast<float>(rec.z - ccz), quad + 6 * node, acc);
+			}
 			nq += __popcll(bnear);
 			nbuf += __popcll(bmono);
 			nfar += far ? 1 : 0;
@@ -934,9 +959,12 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		{
 			for(int k = 0; k < 9; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
 		}
-		kx = acc[0] + acc[3] * tx + acc[6] * ty + acc[7] * tz;
-		ky = acc[1] + acc[6] * tx + acc[4] * ty + acc[8] * tz;
-		kz = acc[2] + acc[7] * tx + acc[8] * ty + acc[5] * tz;
+		const float ux = static_cast<float>(me.x - ccx);   // unscaled offset for the expansion
+		const float uy = static_cast<float>(me.y - ccy);
+		const float uz = static_cast<float>(me.z - ccz);
+		kx = acc[0] + acc[3] * ux + acc[6] * uy + acc[7] * uz;
+		ky = acc[1] + acc[6] * ux + acc[4] * uy + acc[8] * uz;
+		kz = acc[2] + acc[7] * ux + acc[8] * uy + acc[5] * uz;
 	}
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
@@ -1161,9 +1189,9 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	const double hd = half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1);
 	const int tn = n + blockIdx.x * kGroup + lane;
 	const double4 me = xyzr[tn];
-	const float tx = static_cast<float>(me.x - ccx);
-	const float ty = static_cast<float>(me.y - ccy);
-	const float tz = static_cast<float>(me.z - ccz);
+	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
+	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
+	const float tz = static_cast<float>((me.z - ccz) * kSweepLen);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 	float acc[nbh_net::kOut];               // this lane's share of the far-cell correction (R, J)
 	for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] = 0.0f; }
@@ -1244,8 +1272,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		double dc = 0.0;
 		if(valid)
 		{
-			ent = make_float4(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
-			                  static_cast<float>(rec.z - ccz), static_cast<float>(m));
+			ent = sweep_entry(rec, m, ccx, ccy, ccz);
 			if(node >= n)
 			{
 				accept = true;
@@ -1284,8 +1311,9 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 			const int slot = nband + __popcll(bband & lt);
 			float fe[nbh_net::kIn];
 			float be[nbh_net::kBasis];
-			const float scale = nbh_net::cell_features_fast(ent.x, ent.y, ent.z, rec.x, rec.y, rec.z, m,
-			                                                bmin + 3 * node, bmax + 3 * node, quad + 6 * node, fe, be);
+			const float scale = nbh_net::cell_features_fast(
+			    static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy), static_cast<float>(rec.z - ccz),
+			    rec.x, rec.y, rec.z, m, bmin + 3 * node, bmax + 3 * node, quad + 6 * node, fe, be);
 			for(int k = 0; k < nbh_net::kBasis; ++k) { bandb[slot * nbh_net::kBasis + k] = be[k]; }
 			bandw[2 * slot] = scale;
 			bandw[2 * slot + 1] = scale / (static_cast<float>(nbh_net::kJac) * static_cast<float>(dc));
@@ -1321,9 +1349,12 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
 	}
 	const float* S = acc;
-	const float kx = S[0] + S[3] * tx + S[6] * ty + S[7] * tz;
-	const float ky = S[1] + S[6] * tx + S[4] * ty + S[8] * tz;
-	const float kz = S[2] + S[7] * tx + S[8] * ty + S[5] * tz;
+	const float ux = static_cast<float>(me.x - ccx);   // unscaled offset for the correction
+	const float uy = static_cast<float>(me.y - ccy);
+	const float uz = static_cast<float>(me.z - ccz);
+	const float kx = S[0] + S[3] * ux + S[6] * uy + S[7] * uz;
+	const float ky = S[1] + S[6] * ux + S[4] * uy + S[8] * uz;
+	const float kz = S[2] + S[7] * ux + S[8] * uy + S[5] * uz;
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
 	f[n + b] = y[4 * n + b];
