@@ -147,6 +147,17 @@ nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* tree, const doubl
    that lowers the error at a given opening ratio. */
 nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f,
                                       float near_k);
+/* nbh_bh_fcompute_group_quad with the far cells' monopole field expanded too:
+   a cell whose mass centre lies at least near_k (>= 1) half-diagonals from the
+   group's box centre contributes its monopole field and first to third
+   derivatives at the centre plus its quadrupole field and Jacobian, once per
+   group; each body applies the sum (monopole to third order, quadrupole to
+   first) in its offset from the centre.  Nearer cells and leaves are swept per body as in
+   nbh_bh_fcompute_group_quad.  Not in the reference: a local (Taylor) expansion
+   in the manner of fast multipole methods, replacing most of the per-body work
+   of a group walk. */
+nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f,
+                                       float near_k);
 /* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may
    be NULL.  cm/bmin/bmax are xyz-interleaved [2n*3]; body_n holds -1 on
    internal nodes.  Used to check the layout against the reference heap

@@ -663,7 +663,19 @@ k_bh_stackless32(const double4* __restrict__ xyzr, const double* __restrict__ nm
 // takes the node as a monopole instead (counted in counters[2]).
 constexpr int kGroup = 64;
 constexpr int kWalkStack = 1024;   // entries of sibling pairs; (levels + 2) * 32 suffice
-constexpr int kWalkBuf = 256;
+constexpr int kWalkBuf = 128;
+
+// Group handled by this work-group.  Work-groups are dispatched round-robin
+// over the 8 XCDs, each with its own L2; consecutive groups are spatial
+// neighbours that read mostly the same tree nodes, so the ids are permuted to
+// give each XCD one contiguous run of groups (when their count divides by 8).
+constexpr int kXcds = 8;
+
+__device__ inline int xcd_group(int block, int nblocks)
+{
+	if(nblocks % kXcds != 0) { return block; }
+	return (block % kXcds) * (nblocks / kXcds) + block / kXcds;
+}
 
 // Half the diagonal of a box.
 __device__ inline double half_diagonal(double x0, double y0, double z0, double x1, double y1, double z1)
@@ -715,6 +727,23 @@ __device__ inline float4 sweep_entry(const double4& rec, double m, double ccx, d
 {
 	return make_float4(static_cast<float>((rec.x - ccx) * kSweepLen), static_cast<float>((rec.y - ccy) * kSweepLen),
 	                   static_cast<float>((rec.z - ccz) * kSweepLen), static_cast<float>(m * kSweepMass));
+}
+
+// Pop the next chunk of the walk stack (walk_pop) and request its node records;
+// returns the number of stack entries taken.
+__device__ inline int walk_fetch(const int* stack, int& sp, int lane, const double4* __restrict__ xyzr,
+                                 const double* __restrict__ nmass, int& node, bool& valid, double4& rec, double& m)
+{
+	const int cnt = sp < kGroup / 2 ? sp : kGroup / 2;
+	node = walk_pop(stack, sp, lane, valid);
+	rec = make_double4(0.0, 0.0, 0.0, 0.0);
+	m = 0.0;
+	if(valid)
+	{
+		rec = xyzr[node];
+		m = nmass[node];
+	}
+	return cnt;
 }
 
 __device__ inline void sweep32(const float4* __restrict__ buf, int count, float tx, float ty, float tz,
@@ -791,13 +820,9 @@ __device__ inline void put_quad(float4* qb, int slot, const float4& ent, const d
 //   g = (2.5 s / d^2 r - Q' r) / d^5,
 //   G = dg/dr = (5 (r (Q'r)' + (Q'r) r') + 2.5 s I) / d^7 - 17.5 s r r' / d^9 - Q' / d^5,
 // and since r = cm - x, the field at centre + delta is g - G delta: J = -G.
-__device__ inline void quad_expansion(float rx, float ry, float rz, const double* __restrict__ Q, float* acc)
+__device__ inline void quad_expansion(float rx, float ry, float rz, float qxx, float qyy, float qzz, float qxy,
+                                      float qxz, float qyz, float* acc)
 {
-	const double tr = Q[0] + Q[1] + Q[2];
-	const float qxx = static_cast<float>(3.0 * Q[0] - tr), qyy = static_cast<float>(3.0 * Q[1] - tr);
-	const float qzz = static_cast<float>(3.0 * Q[2] - tr);
-	const float qxy = static_cast<float>(3.0 * Q[3]), qxz = static_cast<float>(3.0 * Q[4]);
-	const float qyz = static_cast<float>(3.0 * Q[5]);
 	const float Qx = fmaf(qxx, rx, fmaf(qxy, ry, qxz * rz));
 	const float Qy = fmaf(qxy, rx, fmaf(qyy, ry, qyz * rz));
 	const float Qz = fmaf(qxz, rx, fmaf(qyz, ry, qzz * rz));
@@ -820,12 +845,144 @@ __device__ inline void quad_expansion(float rx, float ry, float rz, const double
 	acc[8] -= a * (ry * Qz + rz * Qy) - e * ry * rz - qyz * ri5;
 }
 
+// Monopole field of a cell of mass M whose mass centre sits at r from the group
+// centre, expanded about the centre to third order, added to acc[34]: field
+// acc[0..2], Jacobian acc[3..8] (xx yy zz xy xz yz, as quad_expansion), the sum
+// T = M r r r / d^7 in acc[9..18] (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz) and
+// W = M r r r r / d^9 in acc[19..33] (xxxx xxxy xxxz xxyy xxyz xxzz xyyy xyyz
+// xyzz xzzz yyyy yyyz yyzz yzzz zzzz).  With f(r) = M r / d^3, the field at
+// centre + delta is f(r - delta):
+//   a(delta) = f - D1 delta + 1/2 D2[delta^2] - 1/6 D3[delta^3] + O(|delta|^4),
+//   D1 = M (I / d^3 - 3 r r' / d^5)            (J = -D1),
+//   1/2 D2[delta^2] = -3 (V.delta) delta - 1.5 |delta|^2 V + 7.5 T[delta^2],
+//   -1/6 D3[delta^3] = 1.5 s |delta|^2 delta - 7.5 (delta.A.delta) delta
+//                      - 7.5 |delta|^2 A delta + 17.5 W[delta^3],
+// where V = sum M r / d^5 = T_ikk, A = sum M r r' / d^7 = W_ijkk and
+// s = sum M / d^5 = A_kk, so only T and W are accumulated (local_terms
+// evaluates the last two lines per body).
+__device__ inline void local_expansion(float rx, float ry, float rz, float M, float* acc)
+{
+	const float u = __builtin_amdgcn_rsqf(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
+	const float u2 = u * u;
+	const float w3 = (M * u) * u2;
+	const float w5 = w3 * u2;
+	const float w7 = w5 * u2;
+	acc[0] = fmaf(w3, rx, acc[0]);
+	acc[1] = fmaf(w3, ry, acc[1]);
+	acc[2] = fmaf(w3, rz, acc[2]);
+	const float t5 = 3.0f * w5;
+	const float tx = t5 * rx, ty = t5 * ry;
+	acc[3] += fmaf(tx, rx, -w3);
+	acc[4] += fmaf(ty, ry, -w3);
+	acc[5] += fmaf(t5 * rz, rz, -w3);
+	acc[6] = fmaf(tx, ry, acc[6]);
+	acc[7] = fmaf(tx, rz, acc[7]);
+	acc[8] = fmaf(ty, rz, acc[8]);
+	const float px = w7 * rx, py = w7 * ry, pz = w7 * rz;
+	const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+	acc[9] = fmaf(px, xx, acc[9]);
+	acc[10] = fmaf(px * rx, ry, acc[10]);
+	acc[11] = fmaf(px * rx, rz, acc[11]);
+	acc[12] = fmaf(px, yy, acc[12]);
+	acc[13] = fmaf(px * ry, rz, acc[13]);
+	acc[14] = fmaf(px, zz, acc[14]);
+	acc[15] = fmaf(py, yy, acc[15]);
+	acc[16] = fmaf(py * ry, rz, acc[16]);
+	acc[17] = fmaf(py, zz, acc[17]);
+	acc[18] = fmaf(pz, zz, acc[18]);
+	const float w9 = w7 * u2;
+	const float a = w9 * xx, b = w9 * yy, c = w9 * zz;
+	const float xy = rx * ry, xz = rx * rz, yz = ry * rz;
+	acc[19] = fmaf(a, xx, acc[19]);
+	acc[20] = fmaf(a, xy, acc[20]);
+	acc[21] = fmaf(a, xz, acc[21]);
+	acc[22] = fmaf(a, yy, acc[22]);
+	acc[23] = fmaf(a, yz, acc[23]);
+	acc[24] = fmaf(a, zz, acc[24]);
+	acc[25] = fmaf(b, xy, acc[25]);
+	acc[26] = fmaf(b, xz, acc[26]);
+	acc[27] = fmaf(c, xy, acc[27]);
+	acc[28] = fmaf(c, xz, acc[28]);
+	acc[29] = fmaf(b, yy, acc[29]);
+	acc[30] = fmaf(b, yz, acc[30]);
+	acc[31] = fmaf(b, zz, acc[31]);
+	acc[32] = fmaf(c, yz, acc[32]);
+	acc[33] = fmaf(c, zz, acc[33]);
+}
+
+// Second- and third-order terms of the local expansion (T = acc[9..18] and
+// W = acc[19..33] of local_expansion, summed over the cells) at offset
+// (dx, dy, dz) from the group centre.
+__device__ inline void local_terms(const float* T, float dx, float dy, float dz, float& ox, float& oy, float& oz)
+{
+	const float vx = T[0] + T[3] + T[5], vy = T[1] + T[6] + T[8], vz = T[2] + T[7] + T[9];
+	const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+	const float xy2 = 2.0f * dx * dy, xz2 = 2.0f * dx * dz, yz2 = 2.0f * dy * dz;
+	const float tdx = T[0] * xx + T[3] * yy + T[5] * zz + T[1] * xy2 + T[2] * xz2 + T[4] * yz2;
+	const float tdy = T[1] * xx + T[6] * yy + T[8] * zz + T[3] * xy2 + T[4] * xz2 + T[7] * yz2;
+	const float tdz = T[2] * xx + T[7] * yy + T[9] * zz + T[4] * xy2 + T[5] * xz2 + T[8] * yz2;
+	const float vd = vx * dx + vy * dy + vz * dz;
+	const float dd = xx + yy + zz;
+	ox = -3.0f * vd * dx - 1.5f * dd * vx + 7.5f * tdx;
+	oy = -3.0f * vd * dy - 1.5f * dd * vy + 7.5f * tdy;
+	oz = -3.0f * vd * dz - 1.5f * dd * vz + 7.5f * tdz;
+	const float* W = T + 10;
+	// monomials of delta^3 times their multiplicities
+	const float m0 = xx * dx, m1 = 3.0f * xx * dy, m2 = 3.0f * xx * dz, m3 = 3.0f * dx * yy, m4 = 3.0f * xy2 * dz;
+	const float m5 = 3.0f * dx * zz, m6 = yy * dy, m7 = 3.0f * yy * dz, m8 = 3.0f * dy * zz, m9 = zz * dz;
+	const float wx = W[0] * m0 + W[1] * m1 + W[2] * m2 + W[3] * m3 + W[4] * m4 + W[5] * m5 + W[6] * m6 + W[7] * m7 +
+	                 W[8] * m8 + W[9] * m9;
+	const float wy = W[1] * m0 + W[3] * m1 + W[4] * m2 + W[6] * m3 + W[7] * m4 + W[8] * m5 + W[10] * m6 +
+	                 W[11] * m7 + W[12] * m8 + W[13] * m9;
+	const float wz = W[2] * m0 + W[4] * m1 + W[5] * m2 + W[7] * m3 + W[8] * m4 + W[9] * m5 + W[11] * m6 +
+	                 W[12] * m7 + W[13] * m8 + W[14] * m9;
+	const float axx = W[0] + W[3] + W[5], ayy = W[3] + W[10] + W[12], azz = W[5] + W[12] + W[14];
+	const float axy = W[1] + W[6] + W[8], axz = W[2] + W[7] + W[9], ayz = W[4] + W[11] + W[13];
+	const float adx = axx * dx + axy * dy + axz * dz;
+	const float ady = axy * dx + ayy * dy + ayz * dz;
+	const float adz = axz * dx + ayz * dy + azz * dz;
+	const float dad = dx * adx + dy * ady + dz * adz;
+	const float s5 = 1.5f * (axx + ayy + azz) * dd - 7.5f * dad;
+	ox += s5 * dx - 7.5f * dd * adx + 17.5f * wx;
+	oy += s5 * dy - 7.5f * dd * ady + 17.5f * wy;
+	oz += s5 * dz - 7.5f * dd * adz + 17.5f * wz;
+}
+
+// Far cells wait in an LDS ring of kFarRing entries -- the unscaled offset of the
+// mass centre from the group centre and the mass as a float4, and the node --
+// and are expanded 64 at a time, one per lane, so the expansion runs on full
+// waves (the second moments are read from the tree there).  A walk step adds at
+// most 64 entries and a batch is taken as soon as 64 wait, so at most 127 are
+// ever pending.
+constexpr int kFarRing = 128;
+
+// Expand `count` (<= 64) ring entries from `head`, one per lane.
+template <bool kLocal>
+__device__ inline void far_batch(const float4* fq, const int* fn, const double* __restrict__ quad, int head,
+                                 int count, int lane, float* acc)
+{
+	if(lane < count)
+	{
+		const int s = (head + lane) & (kFarRing - 1);
+		const float4 e = fq[s];
+		const double* Q = quad + 6 * static_cast<size_t>(fn[s]);
+		const double q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5];
+		const double tr = q0 + q1 + q2;
+		quad_expansion(e.x, e.y, e.z, static_cast<float>(3.0 * q0 - tr), static_cast<float>(3.0 * q1 - tr),
+		               static_cast<float>(3.0 * q2 - tr), static_cast<float>(3.0 * q3), static_cast<float>(3.0 * q4),
+		               static_cast<float>(3.0 * q5), acc);
+		if constexpr(kLocal) { local_expansion(e.x, e.y, e.z, e.w, acc); }
+	}
+}
+
 // kQuad: internal cells taken as monopoles also contribute their quadrupole
 // term (second central moments `quad`).  A cell whose mass centre lies within
 // `near` of the group centre gets it per body (fused monopole + quadrupole
 // sweep); a farther one gets the monopole per body and its quadrupole field
 // expanded to first order about the group centre, once per group.
-template <bool kQuad>
+// kLocal (with kQuad): a far cell's monopole field is expanded too, to third
+// order (local_expansion), instead of being swept per body.
+template <bool kQuad, bool kLocal = false>
 __global__ void __launch_bounds__(kGroup)
 k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
              const double* __restrict__ bmax, const double* __restrict__ quad, const int32_t* __restrict__ body_n,
@@ -835,12 +992,15 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
 	__shared__ float4 qbuf[kQuad ? 3 * kQuadBuf : 1];
+	__shared__ float4 farq[kQuad ? kFarRing : 1];
+	__shared__ int farn[kQuad ? kFarRing : 1];
 	const int lane = threadIdx.x;
-	const int g = n / kGroup + blockIdx.x;              // heap node of this group
+	const int gi = xcd_group(blockIdx.x, gridDim.x);    // group ordinal
+	const int g = n / kGroup + gi;                      // heap node of this group
 	const double gx0 = bmin[3 * g], gy0 = bmin[3 * g + 1], gz0 = bmin[3 * g + 2];
 	const double gx1 = bmax[3 * g], gy1 = bmax[3 * g + 1], gz1 = bmax[3 * g + 2];
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
-	const int tn = n + blockIdx.x * kGroup + lane;     // this lane's leaf
+	const int tn = n + gi * kGroup + lane;             // this lane's leaf
 	const double4 me = xyzr[tn];
 	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
 	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
@@ -848,29 +1008,27 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	unsigned long long vis = 0, taken = 0, spill = 0, nfar = 0, nnear = 0;
-	float acc[9];
-	for(int k = 0; k < 9; ++k) { acc[k] = 0.0f; }
+	constexpr int kAcc = kLocal ? 34 : 9;
+	float acc[kAcc];
+	for(int k = 0; k < kAcc; ++k) { acc[k] = 0.0f; }
 	const float near_d = static_cast<float>(static_cast<double>(near_k) * kSweepLen *
 	                                        half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));   // scaled
 
 	int sp = 1;
 	int nbuf = 0;
 	int nq = 0;
+	int fhead = 0, ftail = 0;     // far-cell ring: entries [fhead, ftail) pending
 	if(lane == 0) { stack[0] = kRoot; }
 	__syncthreads();
 	while(sp > 0)
 	{
 		bool valid;
-		const int node = walk_pop(stack, sp, lane, valid);
+		int node;
+		double4 rec;
+		double m;
 		// issue this step's node loads, then sweep the buffers the previous steps
 		// filled while they are in flight
-		double4 rec = make_double4(0.0, 0.0, 0.0, 0.0);
-		double m = 0.0;
-		if(valid)
-		{
-			rec = xyzr[node];
-			m = nmass[node];
-		}
+		walk_fetch(stack, sp, lane, xyzr, nmass, node, valid, rec, m);
 		if(nbuf > kWalkBuf - kGroup)
 		{
 			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
@@ -882,6 +1040,11 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			{
 				sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
 				nq = 0;
+			}
+			if(ftail - fhead >= kGroup)
+			{
+				far_batch<kLocal>(farq, farn, quad, fhead, kGroup, lane, acc);
+				fhead += kGroup;
 			}
 		}
 		__syncthreads();
@@ -923,14 +1086,18 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			const bool far = cell && !(ent.x * ent.x + ent.y * ent.y + ent.z * ent.z < near_d * near_d);
 			const bool nearc = cell && !far;
 			const uint64_t bnear = __ballot(nearc);
-			const uint64_t bmono = bacc & ~bnear;
+			const uint64_t bfar = __ballot(far);
+			const uint64_t bmono = kLocal ? bacc & ~bnear & ~bfar : bacc & ~bnear;
 			if(nearc) { put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node); }
-			else if(accept) { buf[nbuf + __popcll(bmono & lt)] = ent; }
+			else if(accept && !(kLocal && far)) { buf[nbuf + __popcll(bmono & lt)] = ent; }
 			if(far)
 			{
-				quad_expansion(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
-				               static_cast<float>(rec.z - ccz), quad + 6 * node, acc);
+				const int slot = (ftail + __popcll(bfar & lt)) & (kFarRing - 1);
+				farq[slot] = make_float4(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
+				                         static_cast<float>(rec.z - ccz), static_cast<float>(m));
+				farn[slot] = node;
 			}
+			ftail += __popcll(bfar);
 			nq += __popcll(bnear);
 			nbuf += __popcll(bmono);
 			nfar += far ? 1 : 0;
@@ -955,9 +1122,13 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	if constexpr(kQuad)
 	{
 		if(nq > 0) { sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz); }
+		for(; fhead < ftail; fhead += kGroup)
+		{
+			far_batch<kLocal>(farq, farn, quad, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, acc);
+		}
 		for(int sh = 1; sh < 64; sh <<= 1)
 		{
-			for(int k = 0; k < 9; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
+			for(int k = 0; k < kAcc; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
 		}
 		const float ux = static_cast<float>(me.x - ccx);   // unscaled offset for the expansion
 		const float uy = static_cast<float>(me.y - ccy);
@@ -965,6 +1136,14 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		kx = acc[0] + acc[3] * ux + acc[6] * uy + acc[7] * uz;
 		ky = acc[1] + acc[6] * ux + acc[4] * uy + acc[8] * uz;
 		kz = acc[2] + acc[7] * ux + acc[8] * uy + acc[5] * uz;
+		if constexpr(kLocal)
+		{
+			float sx, sy, sz;
+			local_terms(acc + 9, ux, uy, uz, sx, sy, sz);
+			kx += sx;
+			ky += sy;
+			kz += sz;
+		}
 	}
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
@@ -1182,12 +1361,13 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	__shared__ __align__(16) uint16_t mlds[2 * 16 * nbh_net::kHid];
 	const int lane = threadIdx.x;
 	const int col = lane & 15;
-	const int g = n / kGroup + blockIdx.x;
+	const int gi = xcd_group(blockIdx.x, gridDim.x);
+	const int g = n / kGroup + gi;
 	const double gx0 = bmin[3 * g], gy0 = bmin[3 * g + 1], gz0 = bmin[3 * g + 2];
 	const double gx1 = bmax[3 * g], gy1 = bmax[3 * g + 1], gz1 = bmax[3 * g + 2];
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
 	const double hd = half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1);
-	const int tn = n + blockIdx.x * kGroup + lane;
+	const int tn = n + gi * kGroup + lane;
 	const double4 me = xyzr[tn];
 	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
 	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
@@ -1246,15 +1426,11 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	while(sp > 0)
 	{
 		bool valid;
-		const int node = walk_pop(stack, sp, lane, valid);
+		int node;
+		double4 rec;
+		double m;
 		// node loads first, then the pending sweeps (same order as k_bh_group32)
-		double4 rec = make_double4(0.0, 0.0, 0.0, 0.0);
-		double m = 0.0;
-		if(valid)
-		{
-			rec = xyzr[node];
-			m = nmass[node];
-		}
+		walk_fetch(stack, sp, lane, xyzr, nmass, node, valid, rec, m);
 		if(nbuf > kWalkBuf - kGroup)
 		{
 			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
@@ -1619,7 +1795,9 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* t, const double* y, con
 	return NBH_OK;
 }
 
-static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, bool quadrupole,
+enum GroupMode { kGroupMonopole, kGroupQuad, kGroupLocal };
+
+static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, GroupMode mode,
                              float near_k)
 {
 	NBH_TRY(nbh::enter(ctx));
@@ -1632,10 +1810,15 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
 	NBH_LAUNCHED();
 	nbh::scoped_timer timer(ctx, "bh_traverse");
-	if(quadrupole)
+	if(mode == kGroupQuad)
 	{
 		hipLaunchKernelGGL(k_bh_group32<true>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass,
 		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, near_k, t->counters);
+	}
+	else if(mode == kGroupLocal)
+	{
+		hipLaunchKernelGGL((k_bh_group32<true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr,
+		                   t->mass, t->bmin, t->bmax, t->quad, t->body_n, y, f, n, near_k, t->counters);
 	}
 	else
 	{
@@ -1648,13 +1831,19 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 
 nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f)
 {
-	return group_walk(ctx, t, y, f, false, 0.0f);
+	return group_walk(ctx, t, y, f, kGroupMonopole, 0.0f);
 }
 
 nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, float near_k)
 {
 	NBH_CHECK(near_k >= 0.0f, "near_k must be >= 0");
-	return group_walk(ctx, t, y, f, true, near_k);
+	return group_walk(ctx, t, y, f, kGroupQuad, near_k);
+}
+
+nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, float near_k)
+{
+	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1 (the expansion needs the cell outside the group)");
+	return group_walk(ctx, t, y, f, kGroupLocal, near_k);
 }
 
 nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* groups, size_t n_groups,

@@ -643,6 +643,7 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_group_walk(false),
 	m_quadrupole(false),
 	m_quad_near(3.0f),
+	m_local_far(false),
 	m_net_rho(0.0f),
 	m_net_near(0.0f),
 	m_net(nullptr)
@@ -779,6 +780,12 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 			      "bh fcompute (group walk, learned correction)");
 			return;
 		}
+		if(m_quadrupole && m_local_far)
+		{
+			check(nbh_bh_fcompute_group_local(m_ctx, m_tree, y->data(), f->data(), m_quad_near),
+			      "bh fcompute (quadrupole walk, local far field)");
+			return;
+		}
 		if(m_quadrupole)
 		{
 			check(nbh_bh_fcompute_group_quad(m_ctx, m_tree, y->data(), f->data(), m_quad_near),
@@ -805,6 +812,7 @@ void nbody_engine_hip_bh::print_info() const
 	if(m_quadrupole)
 	{
 		nbody_print("\tbh_quad_near: %g", static_cast<double>(m_quad_near));
+		nbody_print("\tbh_far_field: %s", m_local_far ? "local" : "per_body");
 	}
 	nbody_print("\tlearned_correction: %s", m_net_params.empty() ? "none" : m_net_path.c_str());
 	if(!m_net_params.empty())

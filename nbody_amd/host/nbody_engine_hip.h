@@ -118,6 +118,7 @@ class nbody_engine_hip_bh : public nbody_engine_hip
 	bool            m_group_walk;
 	bool            m_quadrupole;
 	float           m_quad_near;
+	bool            m_local_far;
 	std::vector<float> m_net_params;   // learned correction (empty: none)
 	float           m_net_rho;
 	float           m_net_near;
@@ -133,6 +134,9 @@ public:
 	//! Cells within near_k group half-diagonals get the quadrupole per body,
 	//! farther ones through an expansion about the group centre.
 	void set_quad_near(float near_k) { m_quad_near = near_k; }
+	//! Expand the far cells' monopole field about the group centre too (second
+	//! order) instead of sweeping it per body.
+	void set_local_far_field(bool l) { m_local_far = l; }
 	//! Read a correction network file ("NBHMLP01", int32 parameter count,
 	//! float32 rho_min, float32 near_k, float32 parameters); rho >= 0 and
 	//! near_k >= 0 override the thresholds stored in it.  False if the file is

@@ -126,6 +126,17 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("Invalid bh_quad_near");
 			return nullptr;
 		}
+		std::string far_field = param.value("bh_far_field", "per_body");
+		if(far_field != "per_body" && far_field != "local")
+		{
+			nbody_log("Invalid bh_far_field. Allowed values are 'per_body' or 'local'");
+			return nullptr;
+		}
+		if(far_field == "local" && (multipole != "quadrupole" || !(quad_near >= 1.0)))
+		{
+			nbody_log("bh_far_field=local needs bh_multipole=quadrupole and bh_quad_near >= 1");
+			return nullptr;
+		}
 		// learned correction of the group walk (no reference counterpart)
 		std::string net = param.value("learned_correction", "none");
 		double rho = -1.0;
@@ -149,6 +160,7 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 		engine->set_group_walk(walk == "group");
 		engine->set_quadrupole(multipole == "quadrupole");
 		engine->set_quad_near(static_cast<float>(quad_near));
+		engine->set_local_far_field(far_field == "local");
 		if(net != "none" && !engine->load_correction(net, rho, near_k))
 		{
 			delete engine;

@@ -16,7 +16,11 @@ Contents
   * cell_residual: exact residual field and Jacobian of a cell by brute force;
   * group_band: the near and far cells of one 64-body group, by a plain stack
     walk of the heap tree with the group walk's acceptance rule;
-  * quadrupole_field: the textbook quadrupole term of a cell's field.
+  * quadrupole_field: the textbook quadrupole term of a cell's field;
+  * local_expansion_tensors / local_expansion_eval: second- and third-order
+    Taylor expansions of cells' monopole fields about a group centre (the far
+    field of nbh_bh_fcompute_group_local), and quadrupole_fields (vectorised
+    quadrupole_field summed over cells).
 
 Pinning (tests/test_oracle_mlp.py)
   * loss_grad (float64): central finite differences of the loss, every
@@ -29,7 +33,12 @@ Pinning (tests/test_oracle_mlp.py)
     the textbook quadrupole field and its finite-difference Jacobian;
   * group_band: a cell set property (every body of the group is covered once by
     accepted cells and opened cells' descendants), checked on small trees;
-  * quadrupole_field: limit of the exact residual of a small distant cell.
+  * quadrupole_field: limit of the exact residual of a small distant cell;
+  * local_expansion_*: field at the centre equals the exact monopole field,
+    first derivatives equal finite differences, and the error of the expansion
+    falls as |delta|^3 at second order and |delta|^4 at third (halving the offset
+    divides it by about 8 and 16, which a wrong coefficient of any order would
+    not).
 """
 from __future__ import annotations
 
@@ -284,3 +293,70 @@ def group_band(n, group, cm, crit, bmin, bmax, rho_min, near_k=0.0):
         else:
             stack += [2 * node + 1, 2 * node]
     return far, taken, near
+
+
+def quadrupole_fields(P, cm, Q):
+    """Sum over cells of quadrupole_field(P[b], cm[c], Q[c]) for points P[b,3]."""
+    P = np.atleast_2d(np.asarray(P, np.float64))
+    cm = np.atleast_2d(np.asarray(cm, np.float64))
+    Q = np.atleast_2d(np.asarray(Q, np.float64))
+    if cm.shape[0] == 0:
+        return np.zeros_like(P)
+    Qm = np.zeros((Q.shape[0], 3, 3))
+    for (i, j), k in zip(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)), range(6)):
+        Qm[:, i, j] = Q[:, k]
+        Qm[:, j, i] = Q[:, k]
+    s = P[:, None, :] - cm[None, :, :]                      # [b,c,3], from the cell to the point
+    d = np.linalg.norm(s, axis=2)
+    nv = s / d[..., None]
+    qn = np.einsum("cij,bcj->bci", Qm, nv)
+    nqn = np.einsum("bci,bci->bc", nv, qn)
+    tr = np.trace(Qm, axis1=1, axis2=2)[None, :]
+    grad = (-(3.0 * (2.0 * qn - 2.0 * nqn[..., None] * nv) / d[..., None]) / (2 * d[..., None] ** 3)
+            + 3.0 * (3 * nqn - tr)[..., None] * nv / (2 * d[..., None] ** 4))
+    return -grad.sum(1)
+
+
+def local_expansion_tensors(R, M, order: int = 2):
+    """Taylor coefficients, about an expansion centre, of the monopole fields of
+    cells of mass M[c] whose mass centres sit at R[c,3] from the centre.
+
+    With f(R) = M R / d^3 the field at centre + delta is f(R - delta), so
+      a(delta) = f - D1 delta + 1/2 D2[delta, delta] - 1/6 D3[delta, delta, delta] + ...,
+      D1_ij = M (delta_ij / d^3 - 3 R_i R_j / d^5),
+      D2_ijk = M (-3 (delta_ij R_k + delta_ik R_j + delta_jk R_i) / d^5 + 15 R_i R_j R_k / d^7),
+      D3_ijkl = M (-3 (delta_ij delta_kl + delta_ik delta_jl + delta_il delta_jk) / d^5
+                   + 15 (delta_ij R_k R_l + delta_ik R_j R_l + delta_il R_j R_k
+                         + delta_jk R_i R_l + delta_jl R_i R_k + delta_kl R_i R_j) / d^7
+                   - 105 R_i R_j R_k R_l / d^9).
+    Returns the sums (f[3], J = -D1 [3,3], D2 [3,3,3]) and, for order 3, D3 [3,3,3,3]."""
+    R = np.atleast_2d(np.asarray(R, np.float64))
+    M = np.atleast_1d(np.asarray(M, np.float64))
+    d = np.linalg.norm(R, axis=1)
+    I = np.eye(3)
+    w3, w5, w7, w9 = M / d ** 3, M / d ** 5, M / d ** 7, M / d ** 9
+    f = np.einsum("c,ci->i", w3, R)
+    D1 = w3.sum() * I - 3.0 * np.einsum("c,ci,cj->ij", w5, R, R)
+    D2 = (-3.0 * (np.einsum("ij,k->ijk", I, w5 @ R) + np.einsum("ik,j->ijk", I, w5 @ R)
+                  + np.einsum("jk,i->ijk", I, w5 @ R))
+          + 15.0 * np.einsum("c,ci,cj,ck->ijk", w7, R, R, R))
+    if order < 3:
+        return f, -D1, D2
+    A = np.einsum("c,ci,cj->ij", w7, R, R)
+    D3 = (-3.0 * w5.sum() * (np.einsum("ij,kl->ijkl", I, I) + np.einsum("ik,jl->ijkl", I, I)
+                             + np.einsum("il,jk->ijkl", I, I))
+          + 15.0 * (np.einsum("ij,kl->ijkl", I, A) + np.einsum("ik,jl->ijkl", I, A) + np.einsum("il,jk->ijkl", I, A)
+                    + np.einsum("jk,il->ijkl", I, A) + np.einsum("jl,ik->ijkl", I, A)
+                    + np.einsum("kl,ij->ijkl", I, A))
+          - 105.0 * np.einsum("c,ci,cj,ck,cl->ijkl", w9, R, R, R, R))
+    return f, -D1, D2, D3
+
+
+def local_expansion_eval(f, J, D2, delta, D3=None):
+    """a(delta) = f + J delta + 1/2 D2[delta, delta] (- 1/6 D3[delta, delta, delta])
+    for offsets delta[b,3]."""
+    delta = np.atleast_2d(np.asarray(delta, np.float64))
+    a = f[None, :] + delta @ J.T + 0.5 * np.einsum("ijk,bj,bk->bi", D2, delta, delta)
+    if D3 is not None:
+        a -= np.einsum("ijkl,bj,bk,bl->bi", D3, delta, delta, delta) / 6.0
+    return a

@@ -306,17 +306,22 @@ def test_two_plummer_1m_layout_bit_exact(ctx):
 
 
 def test_engine_quadrupole_walk_under_rk4():
-    """The hip_bh engine with the quadrupole group walk drives rk4 (tree rebuilt
-    every force evaluation) and its forces are closer to the FP64 direct sum
-    than the monopole walk's at the same opening ratio."""
+    """The hip_bh engine with the quadrupole group walk (far cells per body, or
+    through the local expansion) drives rk4 (tree rebuilt every force
+    evaluation) and its forces are closer to the FP64 direct sum than the
+    monopole walk's at the same opening ratio."""
     n = 1 << 14
     errs = {}
-    for mp in ("monopole", "quadrupole"):
+    for mp in ("monopole", "quadrupole", "local"):
         d = nbody_data()
         d.set_quiet(True)
         d.generate("plummer", n, 16)
-        e = nbody_create_engine({"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
-                                 "bh_multipole": mp, "distance_to_node_radius_ratio": 1.6})
+        params = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
+                  "bh_multipole": "monopole" if mp == "monopole" else "quadrupole",
+                  "distance_to_node_radius_ratio": 1.6}
+        if mp == "local":
+            params["bh_far_field"] = "local"
+        e = nbody_create_engine(params)
         assert e.init(d)
         from nbody_amd import nbody_create_solver
         s = nbody_create_solver({"solver": "rk4", "max_step": 1e-3})
@@ -336,4 +341,5 @@ def test_engine_quadrupole_walk_under_rk4():
         s.destroy()
         e.destroy()
     assert errs["quadrupole"] < 0.3 * errs["monopole"], errs
+    assert errs["local"] < 0.3 * errs["monopole"], errs
 

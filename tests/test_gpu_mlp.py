@@ -423,6 +423,85 @@ def test_quadrupole_walk_far_expansion_matches_oracle(ctx):
     t.close()
 
 
+def _oracle_local_walk(n, a, body, near_k):
+    """Accelerations (original body order) of the quadrupole walk with a local far
+    field: leaves and near cells per body (monopole, plus quadrupole for cells),
+    far cells through the third-order expansion of their monopole and the
+    first-order expansion of their quadrupole (finite-difference Jacobian) about
+    the group centre."""
+    want = np.zeros((n, 3))
+    for g in range(n // 64):
+        gn = n // 64 + g
+        cen = 0.5 * (a["bmin"][gn] + a["bmax"][gn])
+        hd = 0.5 * np.linalg.norm(a["bmax"][gn] - a["bmin"][gn])
+        far, taken, near = mo.group_band(n, g, a["cm"], a["crit"], a["bmin"], a["bmax"], 0.0, near_k)
+        X = a["cm"][n + 64 * g:n + 64 * g + 64]
+        src = [t for t in taken if t >= n] + list(near)
+        r = a["cm"][src][None, :, :] - X[:, None, :]
+        r2 = np.maximum(np.einsum("bci,bci->bc", r, r), 1e-8)
+        v = np.einsum("bc,bci->bi", a["mass"][src][None, :] / (r2 * np.sqrt(r2)), r)
+        moments = lambda cells: np.array([mo.second_moments(a["cm"][slice(*_leaf_slice(n, c))],
+                                                            a["mass"][slice(*_leaf_slice(n, c))])[1]
+                                          for c in cells]).reshape(-1, 6)
+        v += mo.quadrupole_fields(X, a["cm"][near], moments(near))
+        if far:
+            f, J, D2, D3 = mo.local_expansion_tensors(a["cm"][far] - cen, a["mass"][far], 3)
+            Qf = moments(far)
+            h = 1e-4 * hd
+            gq = mo.quadrupole_fields(cen[None, :], a["cm"][far], Qf)[0]
+            Jq = np.stack([(mo.quadrupole_fields((cen + h * e)[None, :], a["cm"][far], Qf)[0]
+                            - mo.quadrupole_fields((cen - h * e)[None, :], a["cm"][far], Qf)[0]) / (2 * h)
+                           for e in np.eye(3)], axis=1)
+            v += mo.local_expansion_eval(f + gq, J + Jq, D2, X - cen, D3)
+        want[body[n + 64 * g:n + 64 * g + 64]] = v
+    return want
+
+
+def test_local_far_field_walk_matches_oracle(ctx):
+    """nbh_bh_fcompute_group_local (far cells' monopole to third order and
+    quadrupole to first order about the group centre, near cells and leaves per
+    body) against the oracle's restatement of the same sums in FP64."""
+    n, ratio, near_k = 4096, 1.6, 3.0
+    y, m = _state("plummer", n, 27)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group_local(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local")
+    got = df.read()
+    c = t.counters()
+    assert c[2] == 0 and c[3] > 0 and c[4] > 0
+    assert np.array_equal(got[:3 * n], y[3 * n:])
+    want = _oracle_local_walk(n, a, _body_order(ctx, t), near_k)
+    acc = got[3 * n:].reshape(3, n).T
+    err = np.linalg.norm(acc - want, axis=1) / np.linalg.norm(want, axis=1)
+    print(f"\n[local walk vs oracle] rel err p50 {np.median(err):.2e} p99 {np.percentile(err, 99):.2e} "
+          f"max {err.max():.2e}")
+    assert np.percentile(err, 99) < 1e-4 and err.max() < 3e-3, (np.percentile(err, 99), err.max())
+    bad = ctx.alloc(6 * n)
+    assert ctx.lib.nbh_bh_fcompute_group_local(ctx.h, t.h, t.dy.ptr, bad.ptr, 0.5) != 0
+    t.close()
+
+
+def test_local_far_field_walk_accuracy(ctx):
+    """At theta 0.6 the local far field keeps the quadrupole walk's accuracy
+    against the FP64 direct sum (every cell per body as the reference point)."""
+    n, ratio, near_k = 1 << 16, 1.0 / 0.6, 3.0
+    y, m = _state("plummer", n, 28)
+    t = Tree(ctx, y, m, ratio)
+    idx = np.random.default_rng(5).choice(n, 2048, replace=False)
+    a0 = _sample_accel(ctx, t.dy, t.dm, n, idx)
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group_quad(ctx.h, t.h, t.dy.ptr, df.ptr, float("inf")), "quad")
+    e_all = _rms_rel(df.read(), a0, n, idx)
+    native.check(ctx.lib.nbh_bh_fcompute_group_local(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local")
+    e_local = _rms_rel(df.read(), a0, n, idx)
+    e_mono = _rms_rel(t.group(), a0, n, idx)
+    print(f"\n[local walk accuracy] rms/p99: every cell per body {e_all[0]:.3e}/{e_all[1]:.3e}  "
+          f"local far field {e_local[0]:.3e}/{e_local[1]:.3e}  monopole {e_mono[0]:.3e}/{e_mono[1]:.3e}")
+    assert e_local[0] < 1.15 * e_all[0] and e_local[0] < 0.5 * e_mono[0]
+    t.close()
+
+
 def test_corrected_walk_matches_oracle(ctx):
     """Correction added by the kernel == the oracle's near-cell quadrupoles plus
     its network evaluated on the oracle's far cells, expanded to first order

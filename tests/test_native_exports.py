@@ -50,3 +50,8 @@ def test_factory_validates_before_device():
     assert nbody_create_engine({"engine": "hip_bh", "traverse_type": "zigzag"}) is None
     assert nbody_create_engine({"engine": "hip_bh", "tree_build_rate": "-1"}) is None
     assert nbody_create_engine({"engine": "hip_bh", "distance_to_node_radius_ratio": "0"}) is None
+    grp = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group"}
+    assert nbody_create_engine({**grp, "bh_far_field": "tree"}) is None
+    assert nbody_create_engine({**grp, "bh_far_field": "local"}) is None                     # needs quadrupole
+    assert nbody_create_engine({**grp, "bh_multipole": "quadrupole", "bh_far_field": "local",
+                                "bh_quad_near": "0.5"}) is None                              # cell inside the group

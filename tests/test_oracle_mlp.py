@@ -247,3 +247,50 @@ def test_quadrupole_sum_improves_monopole():
             quad += mo.quadrupole_field(x, a["cm"][c], Q)
         worse += np.linalg.norm(quad - exact) > np.linalg.norm(mono - exact)
     assert worse <= 3
+
+
+def _point_fields(x, R, M):
+    """Exact monopole fields of point masses M[c] at R[c] at the point x."""
+    return sum(mo.field(x, r[None, :], np.array([mm]))[0] for r, mm in zip(R, M))
+
+
+def test_local_expansion_field_jacobian_and_error_order():
+    """Expansions of point-mass fields about a centre: the field at the centre
+    is the exact field, the Jacobian is the finite-difference one, and the error
+    falls as |delta|^3 at second order and |delta|^4 at third -- halving the
+    offsets divides the largest error by about 8 and 16, where dropping the
+    second-order term (planted here in a copy of the coefficients) leaves a
+    ratio near 4, and a sign error in the third-order term one near 8."""
+    rng = np.random.default_rng(11)
+    R = rng.normal(size=(6, 3)) * 2.0 + np.array([5.0, -4.0, 3.0])
+    M = rng.uniform(0.1, 1.0, 6)
+    f, J, D2, D3 = mo.local_expansion_tensors(R, M, 3)
+    assert np.allclose(f, _point_fields(np.zeros(3), R, M), rtol=1e-13, atol=0)
+    h = 1e-5
+    Jfd = np.stack([(_point_fields(h * e, R, M) - _point_fields(-h * e, R, M)) / (2 * h) for e in np.eye(3)], axis=1)
+    assert np.allclose(J, Jfd, rtol=1e-7, atol=1e-9 * np.abs(J).max())
+    U = rng.normal(size=(12, 3))
+    U /= np.linalg.norm(U, axis=1)[:, None]
+
+    def worst(D2c, s, D3c=None):
+        return max(np.linalg.norm(mo.local_expansion_eval(f, J, D2c, s * u[None, :], D3c)[0]
+                                  - _point_fields(s * u, R, M)) for u in U)
+
+    ratio = worst(D2, 0.2) / worst(D2, 0.1)
+    assert 6.5 < ratio < 9.5, ratio
+    planted = worst(np.zeros_like(D2), 0.2) / worst(np.zeros_like(D2), 0.1)
+    assert 3.0 < planted < 5.0, planted
+    ratio3 = worst(D2, 0.2, D3) / worst(D2, 0.1, D3)
+    assert 13.0 < ratio3 < 19.0, ratio3
+    assert worst(D2, 0.1, D3) < 0.2 * worst(D2, 0.1)
+    planted3 = worst(D2, 0.2, -D3) / worst(D2, 0.1, -D3)
+    assert 6.5 < planted3 < 9.5, planted3
+
+
+def test_quadrupole_fields_matches_single_cell_form():
+    rng = np.random.default_rng(12)
+    P = rng.normal(size=(5, 3))
+    cm = rng.normal(size=(4, 3)) + 6.0
+    Q = rng.normal(size=(4, 6)) * 0.1
+    want = np.array([sum(mo.quadrupole_field(p, c, q) for c, q in zip(cm, Q)) for p in P])
+    assert np.allclose(mo.quadrupole_fields(P, cm, Q), want, rtol=1e-12, atol=1e-15)
