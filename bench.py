@@ -67,9 +67,10 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
     """Barnes-Hut euler steps/s (one fcompute = tree build + group walk, plus one
     fmadd_inplace, as the reference bench's step) on a Plummer sphere of n bodies,
     with the relative force error on `samples` random bodies against FP64 direct
-    sums.  variants: (label, ratio, kind) -- kind False: monopole walk;
-    "quadrupole": monopole + quadrupole walk; True: learned correction, trained
-    first on another Plummer realisation of the same size (seed 5)."""
+    sums.  variants: (label, ratio, kind[, engine keys]) -- kind False: monopole
+    walk; "quadrupole": monopole + quadrupole walk (keys bh_quad_near,
+    bh_far_field); True: learned correction, trained first on another Plummer
+    realisation of the same size (seed 5)."""
     import tempfile
 
     import numpy as np
@@ -171,6 +172,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             "variant": label, "ratio": ratio, "theta": 1.0 / ratio,
             "multipole": params.get("bh_multipole", "monopole"),
             "quad_near": params.get("bh_quad_near", 3.0) if "bh_multipole" in params else None,
+            "far_field": params.get("bh_far_field", "per_body") if "bh_multipole" in params else None,
             "steps_per_s": steps / wall,
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
             "walk_ms": tw.value / max(1, cw.value), "interactions_per_body": cnt[0] / n,
@@ -522,18 +524,24 @@ def main(argv=None) -> int:
         log(f"cpu baseline ({threads} threads, ~{a.cpu_seconds:g} s)")
         out["cpu_baseline"] = cpu_baseline(yh, mass, n, a.cpu_seconds, threads)
     if not a.no_bh:
+        local = {"bh_far_field": "local"}
         variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False),
                     ("plain theta=0.25", 4.0, False),
                     ("quadrupole theta=0.6, all cells per body", 1.0 / 0.6, "quadrupole", {"bh_quad_near": "inf"}),
                     ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
                     ("quadrupole theta=0.6, near 2", 1.0 / 0.6, "quadrupole", {"bh_quad_near": 2.0}),
-                    ("quadrupole theta=0.8", 1.25, "quadrupole")]
+                    ("quadrupole theta=0.8", 1.25, "quadrupole"),
+                    ("quadrupole theta=0.6, local far field", 1.0 / 0.6, "quadrupole", local),
+                    ("quadrupole theta=0.65, local far field", 1.0 / 0.65, "quadrupole", local),
+                    ("quadrupole theta=0.8, local far field", 1.25, "quadrupole", local)]
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))
         out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)
     if not a.no_bh and not a.no_two_plummer:
         out["two_plummer"] = bh_bench(1 << 20, 3, 1, [("monopole theta=0.5, group walk", 2.0, False),
-                                                      ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole")],
+                                                      ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
+                                                      ("quadrupole theta=0.6, local far field", 1.0 / 0.6,
+                                                       "quadrupole", {"bh_far_field": "local"})],
                                       kind="two_plummer", seed=3, rk4_steps=3)
     if not a.no_vector_ops:
         log("vector ops")

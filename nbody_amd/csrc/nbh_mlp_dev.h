@@ -247,11 +247,10 @@ __device__ __host__ inline float cell_features(double px, double py, double pz, 
 // FP32 features for the walk: the differences are taken in FP64 and rounded
 // once, the rest in FP32 (the inputs are rounded to BF16 anyway).
 __device__ inline float cell_features_fast(float ux, float uy, float uz, double cx, double cy, double cz, double M,
-                                           const double* bmin, const double* bmax, const double* Q, float* f,
+                                           const double* bmin, const double* bmax, const float* Q, float* f,
                                            float* b)
 {
-	const float q[6] = {static_cast<float>(Q[0]), static_cast<float>(Q[1]), static_cast<float>(Q[2]),
-	                    static_cast<float>(Q[3]), static_cast<float>(Q[4]), static_cast<float>(Q[5])};
+	const float q[6] = {Q[0], Q[1], Q[2], Q[3], Q[4], Q[5]};
 	return cell_features_t<float>(ux, uy, uz, static_cast<float>(M), static_cast<float>(bmax[0] - bmin[0]),
 	                              static_cast<float>(bmax[1] - bmin[1]), static_cast<float>(bmax[2] - bmin[2]),
 	                              static_cast<float>(0.5 * (bmax[0] + bmin[0]) - cx),

@@ -42,6 +42,15 @@ constexpr int kScanParts = 1024;                     // chunks of one scan
 
 }  // namespace
 
+// Node record of the group walks: what a walk step reads for one node, in 64
+// bytes, so that a sibling pair (2i, 2i + 1) is one 128-byte cache line.
+struct __align__(64) nbh_walk_rec {
+	double4 c;      // cm.x, cm.y, cm.z, crit_r2 (as xyzr)
+	double  m;      // mass
+	float   q[6];   // second central moments xx yy zz xy xz yz (0 on leaves), rounded to float
+};
+static_assert(sizeof(nbh_walk_rec) == 64, "walk record must be 64 bytes");
+
 struct nbh_tree {
 	nbh_ctx*  ctx = nullptr;
 	size_t    n = 0;
@@ -60,6 +69,7 @@ struct nbh_tree {
 	double*   bmin = nullptr;      // [2n*3]
 	double*   bmax = nullptr;      // [2n*3]
 	double*   quad = nullptr;      // [2n*6] second central moments (0 on leaves)
+	nbh_walk_rec* wrec = nullptr;  // [2n] packed copy for the group walks
 	int32_t*  body_n = nullptr;    // -1 on internal nodes
 	unsigned long long* counters = nullptr;   // [5] interactions, visits, stack spills, far cells, near cells
 	bool      built = false;
@@ -731,8 +741,8 @@ __device__ inline float4 sweep_entry(const double4& rec, double m, double ccx, d
 
 // Pop the next chunk of the walk stack (walk_pop) and request its node records;
 // returns the number of stack entries taken.
-__device__ inline int walk_fetch(const int* stack, int& sp, int lane, const double4* __restrict__ xyzr,
-                                 const double* __restrict__ nmass, int& node, bool& valid, double4& rec, double& m)
+__device__ inline int walk_fetch(const int* stack, int& sp, int lane, const nbh_walk_rec* __restrict__ wr, int& node,
+                                 bool& valid, double4& rec, double& m)
 {
 	const int cnt = sp < kGroup / 2 ? sp : kGroup / 2;
 	node = walk_pop(stack, sp, lane, valid);
@@ -740,8 +750,8 @@ __device__ inline int walk_fetch(const int* stack, int& sp, int lane, const doub
 	m = 0.0;
 	if(valid)
 	{
-		rec = xyzr[node];
-		m = nmass[node];
+		rec = wr[node].c;
+		m = wr[node].m;
 	}
 	return cnt;
 }
@@ -804,14 +814,14 @@ constexpr int kQuadBuf = 96;     // cells waiting for the monopole + quadrupole 
 
 // Entry of a node (scaled mass-centre offset and mass in ent, second moments Q)
 // at slot `slot` of qb.
-__device__ inline void put_quad(float4* qb, int slot, const float4& ent, const double* __restrict__ Q)
+__device__ inline void put_quad(float4* qb, int slot, const float4& ent, const float* __restrict__ Q)
 {
-	const double tr = Q[0] + Q[1] + Q[2];
-	const double k = kSweepQuad;
+	const float q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5];
+	const float tr = q0 + q1 + q2;
+	const float k = static_cast<float>(kSweepQuad);
 	qb[3 * slot] = ent;
-	qb[3 * slot + 1] = make_float4(static_cast<float>((3.0 * Q[0] - tr) * k), static_cast<float>((3.0 * Q[1] - tr) * k),
-	                               static_cast<float>((3.0 * Q[2] - tr) * k), static_cast<float>(3.0 * Q[3] * k));
-	qb[3 * slot + 2] = make_float4(static_cast<float>(3.0 * Q[4] * k), static_cast<float>(3.0 * Q[5] * k), 0.0f, 0.0f);
+	qb[3 * slot + 1] = make_float4((3.0f * q0 - tr) * k, (3.0f * q1 - tr) * k, (3.0f * q2 - tr) * k, 3.0f * q3 * k);
+	qb[3 * slot + 2] = make_float4(3.0f * q4 * k, 3.0f * q5 * k, 0.0f, 0.0f);
 }
 
 // Quadrupole field at the group centre of a cell whose mass centre sits at r
@@ -958,19 +968,18 @@ constexpr int kFarRing = 128;
 
 // Expand `count` (<= 64) ring entries from `head`, one per lane.
 template <bool kLocal>
-__device__ inline void far_batch(const float4* fq, const int* fn, const double* __restrict__ quad, int head,
+__device__ inline void far_batch(const float4* fq, const int* fn, const nbh_walk_rec* __restrict__ wr, int head,
                                  int count, int lane, float* acc)
 {
 	if(lane < count)
 	{
 		const int s = (head + lane) & (kFarRing - 1);
 		const float4 e = fq[s];
-		const double* Q = quad + 6 * static_cast<size_t>(fn[s]);
-		const double q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5];
-		const double tr = q0 + q1 + q2;
-		quad_expansion(e.x, e.y, e.z, static_cast<float>(3.0 * q0 - tr), static_cast<float>(3.0 * q1 - tr),
-		               static_cast<float>(3.0 * q2 - tr), static_cast<float>(3.0 * q3), static_cast<float>(3.0 * q4),
-		               static_cast<float>(3.0 * q5), acc);
+		const float* Q = wr[fn[s]].q;
+		const float q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5];
+		const float tr = q0 + q1 + q2;
+		quad_expansion(e.x, e.y, e.z, 3.0f * q0 - tr, 3.0f * q1 - tr, 3.0f * q2 - tr, 3.0f * q3, 3.0f * q4, 3.0f * q5,
+		               acc);
 		if constexpr(kLocal) { local_expansion(e.x, e.y, e.z, e.w, acc); }
 	}
 }
@@ -984,10 +993,9 @@ __device__ inline void far_batch(const float4* fq, const int* fn, const double* 
 // order (local_expansion), instead of being swept per body.
 template <bool kQuad, bool kLocal = false>
 __global__ void __launch_bounds__(kGroup)
-k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
-             const double* __restrict__ bmax, const double* __restrict__ quad, const int32_t* __restrict__ body_n,
-             const double* __restrict__ y, double* __restrict__ f, int n, float near_k,
-             unsigned long long* __restrict__ counters)
+k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
+             const int32_t* __restrict__ body_n, const double* __restrict__ y, double* __restrict__ f, int n,
+             float near_k, unsigned long long* __restrict__ counters)
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
@@ -1001,7 +1009,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 	const double gx1 = bmax[3 * g], gy1 = bmax[3 * g + 1], gz1 = bmax[3 * g + 2];
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
 	const int tn = n + gi * kGroup + lane;             // this lane's leaf
-	const double4 me = xyzr[tn];
+	const double4 me = wr[tn].c;
 	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
 	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
 	const float tz = static_cast<float>((me.z - ccz) * kSweepLen);
@@ -1028,7 +1036,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		double m;
 		// issue this step's node loads, then sweep the buffers the previous steps
 		// filled while they are in flight
-		walk_fetch(stack, sp, lane, xyzr, nmass, node, valid, rec, m);
+		walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
 		if(nbuf > kWalkBuf - kGroup)
 		{
 			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
@@ -1043,7 +1051,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			}
 			if(ftail - fhead >= kGroup)
 			{
-				far_batch<kLocal>(farq, farn, quad, fhead, kGroup, lane, acc);
+				far_batch<kLocal>(farq, farn, wr, fhead, kGroup, lane, acc);
 				fhead += kGroup;
 			}
 		}
@@ -1088,7 +1096,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 			const uint64_t bnear = __ballot(nearc);
 			const uint64_t bfar = __ballot(far);
 			const uint64_t bmono = kLocal ? bacc & ~bnear & ~bfar : bacc & ~bnear;
-			if(nearc) { put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node); }
+			if(nearc) { put_quad(qbuf, nq + __popcll(bnear & lt), ent, wr[node].q); }
 			else if(accept && !(kLocal && far)) { buf[nbuf + __popcll(bmono & lt)] = ent; }
 			if(far)
 			{
@@ -1124,7 +1132,7 @@ k_bh_group32(const double4* __restrict__ xyzr, const double* __restrict__ nmass,
 		if(nq > 0) { sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz); }
 		for(; fhead < ftail; fhead += kGroup)
 		{
-			far_batch<kLocal>(farq, farn, quad, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, acc);
+			far_batch<kLocal>(farq, farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, acc);
 		}
 		for(int sh = 1; sh < 64; sh <<= 1)
 		{
@@ -1346,8 +1354,8 @@ constexpr int kBand = 80;        // far-cell rows waiting for the network (< 16 
 // far cells into C (3) and J (6), the correction of a body at offset delta from
 // the centre being C + J delta.
 __global__ void __launch_bounds__(kGroup)
-k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
-                 const double* __restrict__ bmax, const double* __restrict__ quad, const int32_t* __restrict__ body_n,
+k_bh_group32_mlp(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
+                 const double* __restrict__ bmax, const int32_t* __restrict__ body_n,
                  const double* __restrict__ y, double* __restrict__ f, int n, const uint2* __restrict__ frags,
                  const float* __restrict__ mlp_p, float rho_min, float near_k,
                  unsigned long long* __restrict__ counters)
@@ -1368,7 +1376,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
 	const double hd = half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1);
 	const int tn = n + gi * kGroup + lane;
-	const double4 me = xyzr[tn];
+	const double4 me = wr[tn].c;
 	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
 	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
 	const float tz = static_cast<float>((me.z - ccz) * kSweepLen);
@@ -1430,7 +1438,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		double4 rec;
 		double m;
 		// node loads first, then the pending sweeps (same order as k_bh_group32)
-		walk_fetch(stack, sp, lane, xyzr, nmass, node, valid, rec, m);
+		walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
 		if(nbuf > kWalkBuf - kGroup)
 		{
 			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
@@ -1489,7 +1497,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 			float be[nbh_net::kBasis];
 			const float scale = nbh_net::cell_features_fast(
 			    static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy), static_cast<float>(rec.z - ccz),
-			    rec.x, rec.y, rec.z, m, bmin + 3 * node, bmax + 3 * node, quad + 6 * node, fe, be);
+			    rec.x, rec.y, rec.z, m, bmin + 3 * node, bmax + 3 * node, wr[node].q, fe, be);
 			for(int k = 0; k < nbh_net::kBasis; ++k) { bandb[slot * nbh_net::kBasis + k] = be[k]; }
 			bandw[2 * slot] = scale;
 			bandw[2 * slot + 1] = scale / (static_cast<float>(nbh_net::kJac) * static_cast<float>(dc));
@@ -1497,7 +1505,7 @@ k_bh_group32_mlp(const double4* __restrict__ xyzr, const double* __restrict__ nm
 		}
 		if(isnear)
 		{
-			put_quad(qbuf, nq + __popcll(bnear & lt), ent, quad + 6 * node);
+			put_quad(qbuf, nq + __popcll(bnear & lt), ent, wr[node].q);
 		}
 		nbuf += __popcll(bmono);
 		sp += __popcll(bopen);
@@ -1586,6 +1594,22 @@ nbh_status radix_sort(nbh_ctx* ctx, nbh_tree* t, const double* coord, uint32_t* 
 	return NBH_OK;
 }
 
+// Packed walk records of nodes 1 .. count-1.
+__global__ void __launch_bounds__(256)
+k_pack_records(const double4* __restrict__ xyzr, const double* __restrict__ mass, const double* __restrict__ quad,
+               nbh_walk_rec* __restrict__ wr, size_t count)
+{
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = 1 + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride)
+	{
+		nbh_walk_rec r;
+		r.c = xyzr[i];
+		r.m = mass[i];
+		for(int k = 0; k < 6; ++k) { r.q[k] = static_cast<float>(quad[6 * i + k]); }
+		wr[i] = r;
+	}
+}
+
 nbh_status moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mass, double ratio, bool build)
 {
 	const size_t n = t->n;
@@ -1606,6 +1630,9 @@ nbh_status moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mas
 		                   t->quad, ratio_sqr);
 		NBH_LAUNCHED();
 	}
+	hipLaunchKernelGGL(k_pack_records, dim3(nbh::grid_for(ctx, 2 * n, 256)), dim3(256), 0, ctx->stream, t->xyzr,
+	                   t->mass, t->quad, t->wrec, 2 * n);
+	NBH_LAUNCHED();
 	return NBH_OK;
 }
 
@@ -1673,11 +1700,13 @@ nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 	alloc(reinterpret_cast<void**>(&t->bmin), 2 * n * 3 * sizeof(double));
 	alloc(reinterpret_cast<void**>(&t->bmax), 2 * n * 3 * sizeof(double));
 	alloc(reinterpret_cast<void**>(&t->quad), 2 * n * 6 * sizeof(double));
+	alloc(reinterpret_cast<void**>(&t->wrec), 2 * n * sizeof(nbh_walk_rec));
 	alloc(reinterpret_cast<void**>(&t->body_n), 2 * n * sizeof(int32_t));
 	alloc(reinterpret_cast<void**>(&t->counters), 5 * sizeof(unsigned long long));
 	if(e == hipSuccess) { e = hipMemsetAsync(t->xyzr, 0, 2 * n * sizeof(double4), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->mass, 0, 2 * n * sizeof(double), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->quad, 0, 2 * n * 6 * sizeof(double), ctx->stream); }
+	if(e == hipSuccess) { e = hipMemsetAsync(t->wrec, 0, 2 * n * sizeof(nbh_walk_rec), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->body_n, 0xff, 2 * n * sizeof(int32_t), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->counters, 0, 5 * sizeof(unsigned long long), ctx->stream); }
 	if(e == hipSuccess) { e = hipStreamSynchronize(ctx->stream); }
@@ -1702,7 +1731,7 @@ nbh_status nbh_tree_destroy(nbh_tree* t)
 		(void)hipStreamSynchronize(t->ctx->stream);
 	}
 	void* ptrs[] = { t->list[0], t->list[1], t->list[2], t->tmp, t->side, t->keys, t->vals, t->counts, t->scan_part,
-	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->body_n, t->counters };
+	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->wrec, t->body_n, t->counters };
 	for(void* p : ptrs)
 	{
 		if(p != nullptr) { (void)hipFree(p); }
@@ -1812,18 +1841,18 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 	nbh::scoped_timer timer(ctx, "bh_traverse");
 	if(mode == kGroupQuad)
 	{
-		hipLaunchKernelGGL(k_bh_group32<true>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass,
-		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, near_k, t->counters);
+		hipLaunchKernelGGL(k_bh_group32<true>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin,
+		                   t->bmax, t->body_n, y, f, n, near_k, t->counters);
 	}
 	else if(mode == kGroupLocal)
 	{
-		hipLaunchKernelGGL((k_bh_group32<true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr,
-		                   t->mass, t->bmin, t->bmax, t->quad, t->body_n, y, f, n, near_k, t->counters);
+		hipLaunchKernelGGL((k_bh_group32<true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec,
+		                   t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters);
 	}
 	else
 	{
-		hipLaunchKernelGGL(k_bh_group32<false>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass,
-		                   t->bmin, t->bmax, t->quad, t->body_n, y, f, n, 0.0f, t->counters);
+		hipLaunchKernelGGL(k_bh_group32<false>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin,
+		                   t->bmax, t->body_n, y, f, n, 0.0f, t->counters);
 	}
 	NBH_LAUNCHED();
 	return NBH_OK;
@@ -1917,9 +1946,8 @@ nbh_status nbh_bh_fcompute_group_mlp(nbh_ctx* ctx, const nbh_tree* t, const nbh_
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
 	NBH_LAUNCHED();
 	nbh::scoped_timer timer(ctx, "bh_traverse");
-	hipLaunchKernelGGL(k_bh_group32_mlp, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->xyzr, t->mass, t->bmin,
-	                   t->bmax, t->quad, t->body_n, y, f, n, nbh::mlp_frags(net), nbh::mlp_params(net), rho_min,
-	                   near_k, t->counters);
+	hipLaunchKernelGGL(k_bh_group32_mlp, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin, t->bmax,
+	                   t->body_n, y, f, n, nbh::mlp_frags(net), nbh::mlp_params(net), rho_min, near_k, t->counters);
 	NBH_LAUNCHED();
 	return NBH_OK;
 }
