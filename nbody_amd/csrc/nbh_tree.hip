@@ -72,6 +72,12 @@ struct nbh_tree {
 	nbh_walk_rec* wrec = nullptr;  // [2n] packed copy for the group walks
 	int32_t*  body_n = nullptr;    // -1 on internal nodes
 	unsigned long long* counters = nullptr;   // [5] interactions, visits, stack spills, far cells, near cells
+	// two-level local walk (allocated on first use): per super-group node lists,
+	// their lengths and the super-group expansions
+	int*      two_lists = nullptr;
+	int*      two_count = nullptr;
+	float*    two_coef = nullptr;
+	double    ratio = 0.0;         // opening ratio of the last build / refit
 	bool      built = false;
 };
 
@@ -984,18 +990,58 @@ __device__ inline void far_batch(const float4* fq, const int* fn, const nbh_walk
 	}
 }
 
+// Two-level local walk.  A super-group is the heap node of 2^kSuperShift
+// consecutive groups (512 bodies).  One wave walks the tree once for the
+// super-group's box: a node it accepts is accepted by every group inside that
+// box too (the distance from a smaller box inside is no shorter), so accepted
+// cells at least near_k half-diagonals of the super box from its centre go into
+// one local expansion about that centre (far_batch, third order).  Every other
+// accepted node, and every rejected node whose bodies all lie within that
+// distance, goes into a list instead of being opened; other rejected nodes are
+// opened.  Each group then walks from the listed nodes instead of the
+// root, with its own acceptance test (so near its box the cells are those of
+// its own walk), sweeping and expanding about its own centre as the one-level
+// walk does; each body adds both expansions.  A list longer than `cap`, or a
+// stack spill in the super walk, makes the groups of that super-group walk from
+// the root instead.
+constexpr int kSuperShift = 3;
+
+struct TwoLevel {
+	int*   lists = nullptr;   // [super][cap] listed nodes
+	int*   count = nullptr;   // [super] list length (> cap: overflowed)
+	float* coef = nullptr;    // [super][34] expansion about the super box centre (local_expansion layout)
+	int    cap = 0;
+};
+
+// Next 64 listed nodes from position pos, and their records.
+__device__ inline void list_fetch(const int* __restrict__ list, int pos, int count, int lane,
+                                  const nbh_walk_rec* __restrict__ wr, int& node, bool& valid, double4& rec, double& m)
+{
+	const int i = pos + lane;
+	valid = i < count;
+	node = valid ? list[i] : kRoot;
+	rec = make_double4(0.0, 0.0, 0.0, 0.0);
+	m = 0.0;
+	if(valid)
+	{
+		rec = wr[node].c;
+		m = wr[node].m;
+	}
+}
+
 // kQuad: internal cells taken as monopoles also contribute their quadrupole
 // term (second central moments `quad`).  A cell whose mass centre lies within
 // `near` of the group centre gets it per body (fused monopole + quadrupole
 // sweep); a farther one gets the monopole per body and its quadrupole field
 // expanded to first order about the group centre, once per group.
 // kLocal (with kQuad): a far cell's monopole field is expanded too, to third
-// order (local_expansion), instead of being swept per body.
-template <bool kQuad, bool kLocal = false>
+// order (local_expansion), instead of being swept per body.  kTwo (with kLocal):
+// the group takes its super-group's list instead of walking (TwoLevel).
+template <bool kQuad, bool kLocal = false, bool kTwo = false>
 __global__ void __launch_bounds__(kGroup)
 k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
              const int32_t* __restrict__ body_n, const double* __restrict__ y, double* __restrict__ f, int n,
-             float near_k, unsigned long long* __restrict__ counters)
+             float near_k, unsigned long long* __restrict__ counters, TwoLevel two = TwoLevel())
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
@@ -1022,21 +1068,39 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	const float near_d = static_cast<float>(static_cast<double>(near_k) * kSweepLen *
 	                                        half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));   // scaled
 
-	int sp = 1;
+	int si = 0, lcnt = 0, lpos = 0;
+	bool use_list = false;
+	if constexpr(kTwo)
+	{
+		si = gi >> kSuperShift;
+		use_list = two.count[si] <= two.cap;
+		lcnt = use_list ? two.count[si] : 0;
+	}
+	const int* list = kTwo ? two.lists + static_cast<size_t>(si) * two.cap : nullptr;
+	int sp = use_list ? 0 : 1;
 	int nbuf = 0;
 	int nq = 0;
 	int fhead = 0, ftail = 0;     // far-cell ring: entries [fhead, ftail) pending
 	if(lane == 0) { stack[0] = kRoot; }
 	__syncthreads();
-	while(sp > 0)
+	while(sp > 0 || lpos < lcnt)
 	{
 		bool valid;
 		int node;
 		double4 rec;
 		double m;
 		// issue this step's node loads, then sweep the buffers the previous steps
-		// filled while they are in flight
-		walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
+		// filled while they are in flight; listed nodes are taken when the stack
+		// below them is done
+		if(sp == 0)
+		{
+			list_fetch(list, lpos, lcnt, lane, wr, node, valid, rec, m);
+			lpos += kGroup;
+		}
+		else
+		{
+			walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
+		}
 		if(nbuf > kWalkBuf - kGroup)
 		{
 			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
@@ -1152,6 +1216,27 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			ky += sy;
 			kz += sz;
 		}
+		if constexpr(kTwo)
+		{
+			if(use_list)
+			{
+				// the super-group's expansion about its own box centre
+				const int sg = (n >> (6 + kSuperShift)) + si;
+				const double scx = 0.5 * (bmin[3 * sg] + bmax[3 * sg]);
+				const double scy = 0.5 * (bmin[3 * sg + 1] + bmax[3 * sg + 1]);
+				const double scz = 0.5 * (bmin[3 * sg + 2] + bmax[3 * sg + 2]);
+				const float* c = two.coef + 34 * static_cast<size_t>(si);
+				float e[34];
+				for(int k = 0; k < 34; ++k) { e[k] = c[k]; }
+				const float vx = static_cast<float>(me.x - scx), vy = static_cast<float>(me.y - scy);
+				const float vz = static_cast<float>(me.z - scz);
+				float sx, sy, sz;
+				local_terms(e + 9, vx, vy, vz, sx, sy, sz);
+				kx += e[0] + e[3] * vx + e[6] * vy + e[7] * vz + sx;
+				ky += e[1] + e[6] * vx + e[4] * vy + e[8] * vz + sy;
+				kz += e[2] + e[7] * vx + e[8] * vy + e[5] * vz + sz;
+			}
+		}
 	}
 	const int b = body_n[tn];
 	f[b] = y[3 * n + b];
@@ -1176,6 +1261,128 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 		if(spill) { atomicAdd(&counters[2], spill); }
 		if(nfar) { atomicAdd(&counters[3], nfar); }
 		if(nnear) { atomicAdd(&counters[4], nnear); }
+	}
+}
+
+// Super-group walk of the two-level local walk (TwoLevel): the group walk's
+// loop with the super box, far cells into the super expansion, every other
+// accepted node into the list.
+__global__ void __launch_bounds__(kGroup)
+k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
+           int n, float near_k, double inv_ratio, TwoLevel two, unsigned long long* __restrict__ counters)
+{
+	__shared__ int stack[kWalkStack];
+	__shared__ float4 farq[kFarRing];
+	__shared__ int farn[kFarRing];
+	const int lane = threadIdx.x;
+	const int si = xcd_group(blockIdx.x, gridDim.x);
+	const int sg = (n >> (6 + kSuperShift)) + si;        // heap node of the super-group
+	const double gx0 = bmin[3 * sg], gy0 = bmin[3 * sg + 1], gz0 = bmin[3 * sg + 2];
+	const double gx1 = bmax[3 * sg], gy1 = bmax[3 * sg + 1], gz1 = bmax[3 * sg + 2];
+	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
+	const float near_d = static_cast<float>(static_cast<double>(near_k) * half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));
+	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	int* list = two.lists + static_cast<size_t>(si) * two.cap;
+	float acc[34];
+	for(int k = 0; k < 34; ++k) { acc[k] = 0.0f; }
+	unsigned long long vis = 0, nfar = 0;
+	int lcount = 0;
+	bool overflow = false;
+	int sp = 1;
+	int fhead = 0, ftail = 0;
+	if(lane == 0) { stack[0] = kRoot; }
+	__syncthreads();
+	while(sp > 0)
+	{
+		bool valid;
+		int node;
+		double4 rec;
+		double m;
+		walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
+		if(ftail - fhead >= kGroup)
+		{
+			far_batch<true>(farq, farn, wr, fhead, kGroup, lane, acc);
+			fhead += kGroup;
+		}
+		__syncthreads();
+		bool accept = false, open = false, far = false;
+		float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+		if(valid)
+		{
+			ux = static_cast<float>(rec.x - ccx);
+			uy = static_cast<float>(rec.y - ccy);
+			uz = static_cast<float>(rec.z - ccz);
+			if(node >= n)
+			{
+				accept = true;
+			}
+			else
+			{
+				const double ex = fmax(fmax(gx0 - rec.x, rec.x - gx1), 0.0);
+				const double ey = fmax(fmax(gy0 - rec.y, rec.y - gy1), 0.0);
+				const double ez = fmax(fmax(gz0 - rec.z, rec.z - gz1), 0.0);
+				accept = ex * ex + ey * ey + ez * ez > rec.w;
+				const float du = sqrtf(ux * ux + uy * uy + uz * uz);
+				far = accept && !(du < near_d);
+				// a rejected node whose bodies all lie within near_d of the centre
+				// (ball of the cell radius sqrt(crit_r2) / ratio about its mass
+				// centre) is left to the groups; others are opened here
+				open = !accept && !(du + static_cast<float>(sqrt(rec.w) * inv_ratio) < near_d);
+			}
+		}
+		const bool tolist = valid && !far && !open;
+		if(sp + __popcll(__ballot(open)) > kWalkStack)
+		{
+			overflow = true;           // the groups of this super-group will walk for themselves
+			open = false;
+		}
+		const uint64_t bopen = __ballot(open);
+		const uint64_t bfar = __ballot(far);
+		const uint64_t blist = __ballot(tolist);
+		if(far)
+		{
+			const int slot = (ftail + __popcll(bfar & lt)) & (kFarRing - 1);
+			farq[slot] = make_float4(ux, uy, uz, static_cast<float>(m));
+			farn[slot] = node;
+		}
+		if(tolist)
+		{
+			const int slot = lcount + __popcll(blist & lt);
+			if(slot < two.cap) { list[slot] = node; }
+		}
+		if(open) { stack[sp + __popcll(bopen & lt)] = 2 * node; }
+		ftail += __popcll(bfar);
+		lcount += __popcll(blist);
+		sp += __popcll(bopen);
+		vis += valid ? 1 : 0;
+		nfar += far ? 1 : 0;
+		__syncthreads();
+	}
+	for(; fhead < ftail; fhead += kGroup)
+	{
+		far_batch<true>(farq, farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, acc);
+	}
+	for(int sh = 1; sh < 64; sh <<= 1)
+	{
+		for(int k = 0; k < 34; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
+	}
+	overflow = __ballot(overflow) != 0;
+	if(lane == 0)
+	{
+		float* c = two.coef + 34 * static_cast<size_t>(si);
+		for(int k = 0; k < 34; ++k) { c[k] = acc[k]; }
+		two.count[si] = overflow || lcount > two.cap ? two.cap + 1 : lcount;
+	}
+	for(int off = 32; off > 0; off >>= 1)
+	{
+		vis += __shfl_down(vis, off, 64);
+		nfar += __shfl_down(nfar, off, 64);
+	}
+	if(lane == 0)
+	{
+		atomicAdd(&counters[1], vis);
+		if(nfar) { atomicAdd(&counters[3], nfar); }
+		if(overflow || lcount > two.cap) { atomicAdd(&counters[2], 1ull); }
 	}
 }
 
@@ -1612,6 +1819,7 @@ k_pack_records(const double4* __restrict__ xyzr, const double* __restrict__ mass
 
 nbh_status moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mass, double ratio, bool build)
 {
+	t->ratio = ratio;
 	const size_t n = t->n;
 	hipLaunchKernelGGL(k_leaves, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->list[0], y, mass,
 	                   n, t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, build ? 1 : 0);
@@ -1731,7 +1939,8 @@ nbh_status nbh_tree_destroy(nbh_tree* t)
 		(void)hipStreamSynchronize(t->ctx->stream);
 	}
 	void* ptrs[] = { t->list[0], t->list[1], t->list[2], t->tmp, t->side, t->keys, t->vals, t->counts, t->scan_part,
-	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->wrec, t->body_n, t->counters };
+	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->wrec, t->body_n, t->counters,
+	                 t->two_lists, t->two_count, t->two_coef };
 	for(void* p : ptrs)
 	{
 		if(p != nullptr) { (void)hipFree(p); }
@@ -1873,6 +2082,46 @@ nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* t, const do
 {
 	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1 (the expansion needs the cell outside the group)");
 	return group_walk(ctx, t, y, f, kGroupLocal, near_k);
+}
+
+// List capacity per super-group of the two-level walk (4 bytes per entry).
+constexpr int kTwoCap = 32768;
+
+nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* t, const double* y, double* f, float near_k)
+{
+	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1 (the expansion needs the cell outside the group)");
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	const size_t supers = t->n >> (6 + kSuperShift);
+	if(supers < 1)
+	{
+		return group_walk(ctx, t, y, f, kGroupLocal, near_k);
+	}
+	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
+	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
+	if(t->two_lists == nullptr)
+	{
+		NBH_HIP(hipMalloc(reinterpret_cast<void**>(&t->two_lists), supers * kTwoCap * sizeof(int)));
+		NBH_HIP(hipMalloc(reinterpret_cast<void**>(&t->two_count), supers * sizeof(int)));
+		NBH_HIP(hipMalloc(reinterpret_cast<void**>(&t->two_coef), supers * 34 * sizeof(float)));
+	}
+	TwoLevel two;
+	two.lists = t->two_lists;
+	two.count = t->two_count;
+	two.coef = t->two_coef;
+	two.cap = kTwoCap;
+	const int n = static_cast<int>(t->n);
+	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
+	NBH_LAUNCHED();
+	nbh::scoped_timer timer(ctx, "bh_traverse");
+	hipLaunchKernelGGL(k_bh_super, dim3(static_cast<unsigned>(supers)), dim3(kGroup), 0, ctx->stream, t->wrec,
+	                   t->bmin, t->bmax, n, near_k, 1.0 / t->ratio, two, t->counters);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec,
+	                   t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two);
+	NBH_LAUNCHED();
+	return NBH_OK;
 }
 
 nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* groups, size_t n_groups,

@@ -782,7 +782,7 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 		}
 		if(m_quadrupole && m_local_far)
 		{
-			check(nbh_bh_fcompute_group_local(m_ctx, m_tree, y->data(), f->data(), m_quad_near),
+			check(nbh_bh_fcompute_group_local2(m_ctx, m_tree, y->data(), f->data(), m_quad_near),
 			      "bh fcompute (quadrupole walk, local far field)");
 			return;
 		}

@@ -15,7 +15,9 @@ Contents
     (evaluation point, cell) and the basis the 9 outputs are coefficients of;
   * cell_residual: exact residual field and Jacobian of a cell by brute force;
   * group_band: the near and far cells of one 64-body group, by a plain stack
-    walk of the heap tree with the group walk's acceptance rule;
+    walk of the heap tree with the group walk's acceptance rule; super_band /
+    group_band_from: the two walks of the two-level local walk (super-group of
+    8 groups, then each group from the super-group's list);
   * quadrupole_field: the textbook quadrupole term of a cell's field;
   * local_expansion_tensors / local_expansion_eval: second- and third-order
     Taylor expansions of cells' monopole fields about a group centre (the far
@@ -360,3 +362,62 @@ def local_expansion_eval(f, J, D2, delta, D3=None):
     if D3 is not None:
         a -= np.einsum("ijkl,bj,bk,bl->bi", D3, delta, delta, delta) / 6.0
     return a
+
+
+def _box(bmin, bmax, node):
+    lo, hi = bmin[node], bmax[node]
+    return lo, hi, 0.5 * (lo + hi), 0.5 * np.linalg.norm(hi - lo)
+
+
+def super_band(n, s, cm, crit, bmin, bmax, near_k, ratio, shift=3):
+    """(far, listed) heap nodes of super-group s (heap node n / 64 / 2^shift + s)
+    of the two-level local walk: walking from the root with the super-group's
+    box, an accepted internal node whose mass centre lies at least near_k
+    half-diagonals from the box centre is far (expanded about that centre), any
+    other accepted node is listed; a rejected node is listed when the ball about
+    its mass centre that holds all its bodies (radius sqrt(crit) / ratio, the
+    cell radius of the opening rule) lies within that distance of the centre,
+    and opened otherwise."""
+    sg = (n >> (6 + shift)) + s
+    lo, hi, c, half = _box(bmin, bmax, sg)
+    nk = float(np.float32(near_k))
+    far, listed = [], []
+    stack = [1]
+    while stack:
+        node = stack.pop()
+        if node >= n:
+            listed.append(node)
+            continue
+        u = cm[node] - c
+        du = np.sqrt(u @ u)
+        e = np.maximum(np.maximum(lo - cm[node], cm[node] - hi), 0.0)
+        if e @ e > crit[node]:
+            (listed if du < nk * half else far).append(node)
+        elif du + np.sqrt(crit[node]) / ratio < nk * half:
+            listed.append(node)
+        else:
+            stack += [2 * node + 1, 2 * node]
+    return far, listed
+
+
+def group_band_from(n, group, start, cm, crit, bmin, bmax, near_k):
+    """group_band (rho_min 0) for a walk that starts from the nodes `start`
+    instead of the root: (far, taken, near)."""
+    g = n // 64 + group
+    lo, hi, c, half = _box(bmin, bmax, g)
+    nk = float(np.float32(near_k))
+    far, taken, near = [], [], []
+    stack = list(start)[::-1]
+    while stack:
+        node = stack.pop()
+        if node >= n:
+            taken.append(node)
+            continue
+        e = np.maximum(np.maximum(lo - cm[node], cm[node] - hi), 0.0)
+        if e @ e > crit[node]:
+            taken.append(node)
+            u = cm[node] - c
+            (near if np.sqrt(u @ u) < nk * half else far).append(node)
+        else:
+            stack += [2 * node + 1, 2 * node]
+    return far, taken, near

@@ -423,6 +423,44 @@ def test_quadrupole_walk_far_expansion_matches_oracle(ctx):
     t.close()
 
 
+def _moments(a, n, cells):
+    return np.array([mo.second_moments(a["cm"][slice(*_leaf_slice(n, c))], a["mass"][slice(*_leaf_slice(n, c))])[1]
+                     for c in cells]).reshape(-1, 6)
+
+
+def _per_body(a, n, src, near, X):
+    """Monopoles of the nodes src (r^2 clamped at 1e-8) and quadrupoles of the
+    cells near at the points X."""
+    r = a["cm"][src][None, :, :] - X[:, None, :]
+    r2 = np.maximum(np.einsum("bci,bci->bc", r, r), 1e-8)
+    v = np.einsum("bc,bci->bi", a["mass"][src][None, :] / (r2 * np.sqrt(r2)), r)
+    return v + mo.quadrupole_fields(X, a["cm"][near], _moments(a, n, near))
+
+
+def _far_expansion(a, n, far, cen, hd, X):
+    """Far cells about cen: monopole to third order, quadrupole field plus its
+    finite-difference Jacobian."""
+    if not far:
+        return np.zeros_like(X)
+    f, J, D2, D3 = mo.local_expansion_tensors(a["cm"][far] - cen, a["mass"][far], 3)
+    Qf = _moments(a, n, far)
+    h = 1e-4 * hd
+    gq = mo.quadrupole_fields(cen[None, :], a["cm"][far], Qf)[0]
+    Jq = np.stack([(mo.quadrupole_fields((cen + h * e)[None, :], a["cm"][far], Qf)[0]
+                    - mo.quadrupole_fields((cen - h * e)[None, :], a["cm"][far], Qf)[0]) / (2 * h)
+                   for e in np.eye(3)], axis=1)
+    return mo.local_expansion_eval(f + gq, J + Jq, D2, X - cen, D3)
+
+
+def _group_terms(n, a, g, far, taken, near):
+    gn = n // 64 + g
+    cen = 0.5 * (a["bmin"][gn] + a["bmax"][gn])
+    hd = 0.5 * np.linalg.norm(a["bmax"][gn] - a["bmin"][gn])
+    X = a["cm"][n + 64 * g:n + 64 * g + 64]
+    src = [t for t in taken if t >= n] + list(near)
+    return X, _per_body(a, n, src, near, X) + _far_expansion(a, n, far, cen, hd, X)
+
+
 def _oracle_local_walk(n, a, body, near_k):
     """Accelerations (original body order) of the quadrupole walk with a local far
     field: leaves and near cells per body (monopole, plus quadrupole for cells),
@@ -431,28 +469,8 @@ def _oracle_local_walk(n, a, body, near_k):
     the group centre."""
     want = np.zeros((n, 3))
     for g in range(n // 64):
-        gn = n // 64 + g
-        cen = 0.5 * (a["bmin"][gn] + a["bmax"][gn])
-        hd = 0.5 * np.linalg.norm(a["bmax"][gn] - a["bmin"][gn])
         far, taken, near = mo.group_band(n, g, a["cm"], a["crit"], a["bmin"], a["bmax"], 0.0, near_k)
-        X = a["cm"][n + 64 * g:n + 64 * g + 64]
-        src = [t for t in taken if t >= n] + list(near)
-        r = a["cm"][src][None, :, :] - X[:, None, :]
-        r2 = np.maximum(np.einsum("bci,bci->bc", r, r), 1e-8)
-        v = np.einsum("bc,bci->bi", a["mass"][src][None, :] / (r2 * np.sqrt(r2)), r)
-        moments = lambda cells: np.array([mo.second_moments(a["cm"][slice(*_leaf_slice(n, c))],
-                                                            a["mass"][slice(*_leaf_slice(n, c))])[1]
-                                          for c in cells]).reshape(-1, 6)
-        v += mo.quadrupole_fields(X, a["cm"][near], moments(near))
-        if far:
-            f, J, D2, D3 = mo.local_expansion_tensors(a["cm"][far] - cen, a["mass"][far], 3)
-            Qf = moments(far)
-            h = 1e-4 * hd
-            gq = mo.quadrupole_fields(cen[None, :], a["cm"][far], Qf)[0]
-            Jq = np.stack([(mo.quadrupole_fields((cen + h * e)[None, :], a["cm"][far], Qf)[0]
-                            - mo.quadrupole_fields((cen - h * e)[None, :], a["cm"][far], Qf)[0]) / (2 * h)
-                           for e in np.eye(3)], axis=1)
-            v += mo.local_expansion_eval(f + gq, J + Jq, D2, X - cen, D3)
+        _, v = _group_terms(n, a, g, far, taken, near)
         want[body[n + 64 * g:n + 64 * g + 64]] = v
     return want
 
@@ -482,6 +500,42 @@ def test_local_far_field_walk_matches_oracle(ctx):
     t.close()
 
 
+def test_two_level_local_walk_matches_oracle(ctx):
+    """nbh_bh_fcompute_group_local2 on the bodies of two super-groups: the
+    oracle's super-group walk (far cells expanded about the super-group's box
+    centre, the rest listed) and each group's walk from the list (near cells and
+    leaves per body, farther cells expanded about the group centre)."""
+    n, ratio, near_k = 1 << 17, 1.0 / 0.6, 3.0
+    y, m = _state("plummer", n, 29)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    body = _body_order(ctx, t)
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local2")
+    got = df.read()[3 * n:].reshape(3, n).T
+    assert t.counters()[2] == 0                     # no list overflow, no spill
+    errs, shared = [], 0
+    for s in (5, 170):
+        sg = (n >> 9) + s
+        scen = 0.5 * (a["bmin"][sg] + a["bmax"][sg])
+        shd = 0.5 * np.linalg.norm(a["bmax"][sg] - a["bmin"][sg])
+        far_s, listed = mo.super_band(n, s, a["cm"], a["crit"], a["bmin"], a["bmax"], near_k, ratio)
+        shared += len(far_s)
+        for gsub in range(8):
+            g = 8 * s + gsub
+            far, taken, near = mo.group_band_from(n, g, listed, a["cm"], a["crit"], a["bmin"], a["bmax"], near_k)
+            X, v = _group_terms(n, a, g, far, taken, near)
+            v = v + _far_expansion(a, n, far_s, scen, shd, X)
+            b = body[n + 64 * g:n + 64 * g + 64]
+            errs.append(np.linalg.norm(got[b] - v, axis=1) / np.linalg.norm(v, axis=1))
+    err = np.concatenate(errs)
+    print(f"\n[two-level walk vs oracle] shared far cells {shared}, rel err p50 {np.median(err):.2e} "
+          f"p99 {np.percentile(err, 99):.2e} max {err.max():.2e}")
+    assert shared > 0
+    assert np.percentile(err, 99) < 1e-4 and err.max() < 3e-3, (np.percentile(err, 99), err.max())
+    t.close()
+
+
 def test_local_far_field_walk_accuracy(ctx):
     """At theta 0.6 the local far field keeps the quadrupole walk's accuracy
     against the FP64 direct sum (every cell per body as the reference point)."""
@@ -495,10 +549,14 @@ def test_local_far_field_walk_accuracy(ctx):
     e_all = _rms_rel(df.read(), a0, n, idx)
     native.check(ctx.lib.nbh_bh_fcompute_group_local(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local")
     e_local = _rms_rel(df.read(), a0, n, idx)
+    native.check(ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local2")
+    e_local2 = _rms_rel(df.read(), a0, n, idx)
     e_mono = _rms_rel(t.group(), a0, n, idx)
     print(f"\n[local walk accuracy] rms/p99: every cell per body {e_all[0]:.3e}/{e_all[1]:.3e}  "
-          f"local far field {e_local[0]:.3e}/{e_local[1]:.3e}  monopole {e_mono[0]:.3e}/{e_mono[1]:.3e}")
+          f"local far field {e_local[0]:.3e}/{e_local[1]:.3e}  two-level {e_local2[0]:.3e}/{e_local2[1]:.3e}  "
+          f"monopole {e_mono[0]:.3e}/{e_mono[1]:.3e}")
     assert e_local[0] < 1.15 * e_all[0] and e_local[0] < 0.5 * e_mono[0]
+    assert e_local2[0] < 1.15 * e_all[0]
     t.close()
 
 

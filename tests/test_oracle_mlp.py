@@ -294,3 +294,28 @@ def test_quadrupole_fields_matches_single_cell_form():
     Q = rng.normal(size=(4, 6)) * 0.1
     want = np.array([sum(mo.quadrupole_field(p, c, q) for c, q in zip(cm, Q)) for p in P])
     assert np.allclose(mo.quadrupole_fields(P, cm, Q), want, rtol=1e-12, atol=1e-15)
+
+
+def test_two_level_cell_sets_cover_every_body_once():
+    """The two-level local walk's cell sets (super-group walk, then each of its
+    8 groups from the list) cover every body of the tree exactly once per group,
+    and the super-group walk does share far cells at this size."""
+    n, ratio, near_k = 1 << 17, 1.0 / 0.6, 3.0
+    y, m = _plummer_like(n, 40)
+    h = ol.Heap(n)
+    h.build(y, m, ratio)
+    o = h.arrays()
+    shared = 0
+    for s in (3, 200):
+        far, listed = mo.super_band(n, s, o["cm"], o["crit_r2"], o["bmin"], o["bmax"], near_k, ratio)
+        shared += len(far)
+        for gsub in (0, 5):
+            _, taken, _ = mo.group_band_from(n, 8 * s + gsub, listed, o["cm"], o["crit_r2"], o["bmin"], o["bmax"],
+                                             near_k)
+            cover = np.zeros(n, int)
+            for node in far + taken:
+                depth = int(node).bit_length() - 1
+                span = int(np.log2(n)) - depth
+                cover[(node << span) - n:((node + 1) << span) - n] += 1
+            assert cover.min() == 1 and cover.max() == 1
+    assert shared > 0
