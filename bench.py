@@ -531,9 +531,12 @@ def main(argv=None) -> int:
                     ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
                     ("quadrupole theta=0.6, near 2", 1.0 / 0.6, "quadrupole", {"bh_quad_near": 2.0}),
                     ("quadrupole theta=0.8", 1.25, "quadrupole"),
+                    ("quadrupole theta=0.55, local far field", 1.0 / 0.55, "quadrupole", local),
                     ("quadrupole theta=0.6, local far field", 1.0 / 0.6, "quadrupole", local),
-                    ("quadrupole theta=0.65, local far field", 1.0 / 0.65, "quadrupole", local),
-                    ("quadrupole theta=0.8, local far field", 1.25, "quadrupole", local)]
+                    ("quadrupole theta=0.65, local far field, near 4", 1.0 / 0.65, "quadrupole",
+                     {**local, "bh_quad_near": 4.0}),
+                    ("quadrupole theta=0.7, local far field, near 4", 1.0 / 0.7, "quadrupole",
+                     {**local, "bh_quad_near": 4.0})]
         if not a.no_learned:
             variants.append(("learned theta=0.8", 1.25, True))
         out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)
