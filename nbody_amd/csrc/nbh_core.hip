@@ -181,8 +181,29 @@ k_fill(double* __restrict__ a, double v, size_t n)
 	}
 }
 
+// Streams larger than the 256 MB last-level cache (MALL) are read and written
+// with nontemporal hints: measured 5.0-5.3 -> 5.8-5.9 TB/s for 12 terms over
+// 6 x 2^22 doubles; smaller ones keep the cache, which the next force
+// evaluation reads from.
+constexpr size_t kStreamBytes = static_cast<size_t>(256) << 20;
+
+template <bool kNT>
+__device__ inline double vld(const double* p)
+{
+	if constexpr(kNT) { return __builtin_nontemporal_load(p); }
+	else { return *p; }
+}
+
+template <bool kNT>
+__device__ inline void vst(double* p, double v)
+{
+	if constexpr(kNT) { __builtin_nontemporal_store(v, p); }
+	else { *p = v; }
+}
+
 // acc = init (or 0); acc = fma(src[t], coef[t], acc) for t in order; a = acc.
-// Two doubles per work-item per pass when `pairs`; `init` and any src may alias a.
+// Two doubles per work-item per pass; `init` and any src may alias a.
+template <bool kNT>
 __global__ void __launch_bounds__(kVecThreads)
 k_fmaddn(double* a, const double* init, term_list terms, int k, size_t n)
 {
@@ -194,16 +215,27 @@ k_fmaddn(double* a, const double* init, term_list terms, int k, size_t n)
 		double2 acc = make_double2(0.0, 0.0);
 		if(init != nullptr)
 		{
-			acc = *reinterpret_cast<const double2*>(init + i);
+			if constexpr(kNT) { acc = make_double2(vld<true>(init + i), vld<true>(init + i + 1)); }
+			else { acc = *reinterpret_cast<const double2*>(init + i); }
 		}
 		for(int t = 0; t < k; ++t)
 		{
-			double2 s = *reinterpret_cast<const double2*>(terms.src[t] + i);
+			double2 s;
+			if constexpr(kNT) { s = make_double2(vld<true>(terms.src[t] + i), vld<true>(terms.src[t] + i + 1)); }
+			else { s = *reinterpret_cast<const double2*>(terms.src[t] + i); }
 			double c = terms.coef[t];
 			acc.x = fma(s.x, c, acc.x);
 			acc.y = fma(s.y, c, acc.y);
 		}
-		*reinterpret_cast<double2*>(a + i) = acc;
+		if constexpr(kNT)
+		{
+			vst<true>(a + i, acc.x);
+			vst<true>(a + i + 1, acc.y);
+		}
+		else
+		{
+			*reinterpret_cast<double2*>(a + i) = acc;
+		}
 	}
 	// odd tail element
 	if((n & 1) && blockIdx.x == 0 && threadIdx.x == 0)
@@ -243,21 +275,22 @@ __device__ inline double kahan(double a, double term, double& corr)
 	return new_sum;
 }
 
+template <bool kNT>
 __global__ void __launch_bounds__(kVecThreads)
 k_fmaddn_corr(double* a, double* corr, term_list terms, int k, size_t n)
 {
 	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
 	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
 	{
-		double s = a[i];
-		double c = corr[i];
+		double s = vld<kNT>(a + i);
+		double c = vld<kNT>(corr + i);
 		for(int t = 0; t < k; ++t)
 		{
-			double term = terms.src[t][i] * terms.coef[t];
+			double term = vld<kNT>(terms.src[t] + i) * terms.coef[t];
 			s = kahan(s, term, c);
 		}
-		a[i] = s;
-		corr[i] = c;
+		vst<kNT>(a + i, s);
+		vst<kNT>(corr + i, c);
 	}
 }
 
@@ -713,9 +746,15 @@ nbh_status nbh_fmaddn(nbh_ctx* ctx, double* a, const double* init,
 			tl.coef[t] = coef[done + t];
 			al = al && aligned16(tl.src[t]);
 		}
-		if(al)
+		if(al && 8 * n * (kk + 2) > kStreamBytes)
 		{
-			hipLaunchKernelGGL(k_fmaddn, dim3(nbh::grid_for(ctx, n / 2 + 1, kVecThreads)),
+			hipLaunchKernelGGL(k_fmaddn<true>, dim3(nbh::grid_for(ctx, n / 2 + 1, kVecThreads)),
+			                   dim3(kVecThreads), 0, ctx->stream, a, cur_init, tl,
+			                   static_cast<int>(kk), n);
+		}
+		else if(al)
+		{
+			hipLaunchKernelGGL(k_fmaddn<false>, dim3(nbh::grid_for(ctx, n / 2 + 1, kVecThreads)),
 			                   dim3(kVecThreads), 0, ctx->stream, a, cur_init, tl,
 			                   static_cast<int>(kk), n);
 		}
@@ -753,9 +792,16 @@ nbh_status nbh_fmaddn_corr(nbh_ctx* ctx, double* a, double* corr,
 			tl.src[t] = src[done + t];
 			tl.coef[t] = coef[done + t];
 		}
-		hipLaunchKernelGGL(k_fmaddn_corr, dim3(nbh::grid_for(ctx, n, kVecThreads)),
-		                   dim3(kVecThreads), 0, ctx->stream, a, corr, tl,
-		                   static_cast<int>(kk), n);
+		if(8 * n * (kk + 4) > kStreamBytes)
+		{
+			hipLaunchKernelGGL(k_fmaddn_corr<true>, dim3(nbh::grid_for(ctx, n, kVecThreads)),
+			                   dim3(kVecThreads), 0, ctx->stream, a, corr, tl, static_cast<int>(kk), n);
+		}
+		else
+		{
+			hipLaunchKernelGGL(k_fmaddn_corr<false>, dim3(nbh::grid_for(ctx, n, kVecThreads)),
+			                   dim3(kVecThreads), 0, ctx->stream, a, corr, tl, static_cast<int>(kk), n);
+		}
 		NBH_LAUNCHED();
 		done += kk;
 	}

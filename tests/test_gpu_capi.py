@@ -81,6 +81,36 @@ def test_fmaddn_corr_long(ctx):
     assert np.array_equal(dc.read(), ec)
 
 
+def test_streaming_sizes_bit_exact(ctx):
+    """Calls that stream more than the 256 MB last-level cache take the
+    nontemporal kernels: same results as the oracle bit for bit (fmaddn with an
+    odd length and k = 2, aliased init; fmaddn_corr with k = 3)."""
+    n = 12_000_001                      # 8 n (k + 2) bytes > 256 MB for k = 2
+    rng = np.random.default_rng(9)
+    init = rng.standard_normal(n)
+    terms = [rng.standard_normal(n) for _ in range(2)]
+    coef = np.array([0.25, -3.0])
+    da = ctx.put(init)
+    dts = [ctx.put(t) for t in terms]
+    cf, pcf = dbl_array(coef)
+    native.check(ctx.lib.nbh_fmaddn(ctx.h, da.ptr, da.ptr, ptr_array([t.ptr for t in dts]), pcf, 2, n))
+    assert np.array_equal(da.read(), _oracle_fmaddn(init, terms, coef))
+    m, k = 9_000_000, 3                 # 8 m (k + 4) bytes > 256 MB
+    a0 = rng.standard_normal(m)
+    c0 = rng.standard_normal(m) * 1e-17
+    terms = [rng.standard_normal(m) for _ in range(k)]
+    coef = rng.standard_normal(k) * 1e-3
+    da, dc = ctx.put(a0), ctx.put(c0)
+    dts = [ctx.put(t) for t in terms]
+    cf, pcf = dbl_array(coef)
+    native.check(ctx.lib.nbh_fmaddn_corr(ctx.h, da.ptr, dc.ptr, ptr_array([t.ptr for t in dts]), pcf, k, m))
+    ea, ec = a0.copy(), c0.copy()
+    arr = (oracle_lib.PD * k)(*[oracle_lib.p(t) for t in terms])
+    oracle_lib.lib().oracle_fmaddn_corr(oracle_lib.p(ea), oracle_lib.p(ec), arr, oracle_lib.p(cf), k, m)
+    assert np.array_equal(da.read(), ea)
+    assert np.array_equal(dc.read(), ec)
+
+
 @pytest.mark.parametrize("n", [1, 5, 4096, 1000003])
 def test_maxabs(ctx, n):
     rng = np.random.default_rng(n)
