@@ -785,12 +785,12 @@ __device__ inline void sweep32(const float4* __restrict__ buf, int count, float 
 }
 
 // Monopole + quadrupole field of the cells in qb at the lane's body.  Entries
-// of 3 float4: (mass-centre offset from the group centre, M), (Q'xx Q'yy Q'zz
-// Q'xy), (Q'xz Q'yz - -), with Q' = 3 Q - tr(Q) I the traceless second moment
-// about the mass centre; r = cm - x, d = |r|:
-//   a = M r / d^3 + (2.5 (r.Q'.r) / d^2 r - Q' r) / d^5,
-// all in the scaled units of sweep32 (Q' times kSweepQuad).  The monopole part
-// is summed like sweep32 (same operations, Kahan per sweep).
+// of 3 float4: (mass-centre offset from the group centre, M), 2.5 x (Q'xx Q'yy
+// Q'zz Q'xy), 2.5 x (Q'xz Q'yz - -), with Q' = 3 Q - tr(Q) I the traceless second
+// moment about the mass centre; r = cm - x, d = |r|, u = 1/d:
+//   a = M r / d^3 + (2.5 (r.Q'.r) / d^2 r - Q' r) / d^5
+//     = (M + s u^4) u^3 r - 0.4 u^5 (2.5 Q' r),   s = r.(2.5 Q').r,
+// all in the scaled units of sweep32 (Q' times kSweepQuad).  Kahan per sweep.
 __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float tx, float ty, float tz,
                                 float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
 {
@@ -799,17 +799,19 @@ __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float 
 	{
 		const float4 e0 = qb[3 * j], e1 = qb[3 * j + 1], e2 = qb[3 * j + 2];
 		const float rx = e0.x - tx, ry = e0.y - ty, rz = e0.z - tz;
-		const float ri = rsq_clamped(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
-		const float ri2 = ri * ri;
-		const float w = (e0.w * ri) * ri2;
-		const float ri5 = (ri2 * ri2) * ri;
+		const float u = rsq_clamped(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
+		const float u2 = u * u;
+		const float u3 = u2 * u;
+		const float u4 = u2 * u2;
 		const float Qx = fmaf(e1.x, rx, fmaf(e1.w, ry, e2.x * rz));
 		const float Qy = fmaf(e1.w, rx, fmaf(e1.y, ry, e2.y * rz));
 		const float Qz = fmaf(e2.x, rx, fmaf(e2.y, ry, e1.z * rz));
-		const float c = (2.5f * fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz))) * ri2;
-		px = fmaf(fmaf(c, rx, -Qx), ri5, fmaf(rx, w, px));
-		py = fmaf(fmaf(c, ry, -Qy), ri5, fmaf(ry, w, py));
-		pz = fmaf(fmaf(c, rz, -Qz), ri5, fmaf(rz, w, pz));
+		const float s = fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz));
+		const float k = fmaf(s, u4, e0.w) * u3;
+		const float g = -0.4f * (u4 * u);
+		px = fmaf(rx, k, fmaf(Qx, g, px));
+		py = fmaf(ry, k, fmaf(Qy, g, py));
+		pz = fmaf(rz, k, fmaf(Qz, g, pz));
 	}
 	float v = px - cx; float t = ax + v; cx = (t - ax) - v; ax = t;
 	v = py - cy; t = ay + v; cy = (t - ay) - v; ay = t;
@@ -819,12 +821,12 @@ __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float 
 constexpr int kQuadBuf = 96;     // cells waiting for the monopole + quadrupole sweep
 
 // Entry of a node (scaled mass-centre offset and mass in ent, second moments Q)
-// at slot `slot` of qb.
+// at slot `slot` of qb, with the 2.5 x Q' that sweep_mq expects.
 __device__ inline void put_quad(float4* qb, int slot, const float4& ent, const float* __restrict__ Q)
 {
 	const float q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5];
 	const float tr = q0 + q1 + q2;
-	const float k = static_cast<float>(kSweepQuad);
+	const float k = static_cast<float>(2.5 * kSweepQuad);
 	qb[3 * slot] = ent;
 	qb[3 * slot + 1] = make_float4((3.0f * q0 - tr) * k, (3.0f * q1 - tr) * k, (3.0f * q2 - tr) * k, 3.0f * q3 * k);
 	qb[3 * slot + 2] = make_float4(3.0f * q4 * k, 3.0f * q5 * k, 0.0f, 0.0f);
