@@ -206,8 +206,9 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
 def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dict:
     """Cold cube [-1,1]^3, v = 0, masses lognormal(0, 0.5) normalised to 1 (seed 5):
     `steps` steps of max_step 1e-3 (rkfeagin10 with one level of 8 adaptive
-    substeps, bs with max_level 8) with FP32 force tiles (+ the compensated FP64
-    Butcher combination for rkfeagin10) against all-FP64; energy drift and the
+    substeps, bs with max_level 8) with FP32 force tiles, alone and with the FP64
+    correction pass of the nearest sources (+ the compensated FP64 Butcher
+    combination for rkfeagin10), against all-FP64; energy drift and the
     absolute total momentum / angular momentum of both runs (P0 = L0 = 0, so the
     reference's percentage errors are undefined for this state)."""
     import numpy as np
@@ -216,7 +217,7 @@ def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dic
     out = {"workload": f"cold cube N={n}, lognormal(0.5) masses, {steps} steps of 1e-3", "runs": []}
     for solver in solvers:
         res = {"solver": solver}
-        for prec in ("fp64", "fp32"):
+        for prec in ("fp64", "fp32", "fp32_corr"):
             d = nbody_data()
             d.set_quiet(True)
             d.generate("cube", n, 5)
@@ -231,7 +232,7 @@ def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dic
                 # one level of 8 substeps: with the default 8 levels the adaptive rule
                 # can subdivide without bound on close pairs of this cold start
                 params["max_recursion"] = 1
-            if solver == "rkfeagin10" and prec == "fp32":
+            if solver == "rkfeagin10" and prec != "fp64":
                 params["correction"] = True
             s = nbody_create_solver(params)
             s.set_engine(e)

@@ -32,8 +32,12 @@ typedef enum nbh_status {
 /* Arithmetic of the force kernels. */
 typedef enum nbh_precision {
 	NBH_FP64 = 0,   /* IEEE binary64, same operation order as the reference CPU engines */
-	NBH_FP32 = 1    /* FP32 interaction math on relative coordinates, compensated sums,
+	NBH_FP32 = 1,   /* FP32 interaction math on relative coordinates, compensated sums,
 	                   FP64 state in and out */
+	NBH_FP32_CORR = 2  /* direct sum only: NBH_FP32 plus an FP64 correction pass -- for each
+	                   target, the sources within 32 places of it in the Morton order of the
+	                   FP32 tiles (its nearest neighbours along the curve) are recomputed in
+	                   FP64 and their FP32 terms, recomputed the tiles' way, taken out */
 } nbh_precision;
 
 /* Tree layouts of the Barnes-Hut engine (nbody/nbody_engine_simple_bh.h:13-20). */

@@ -273,6 +273,52 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, con
 	}
 }
 
+// FP64 correction pass of NBH_FP32_CORR: for the target at Morton position p,
+// the sources at positions p - w .. p + w (its neighbours along the curve, where
+// the FP32 rounding of the offsets weighs most) are recomputed in FP64 (force
+// law of nbody/nbody_data.cpp:35-44) and in the FP32 tiles' own arithmetic (same
+// work-group origin, scaled units and clamped rsq), and f gets the difference.
+constexpr int kCorrWindow = 32;
+
+__global__ void __launch_bounds__(256)
+k_direct_window(const double* __restrict__ y, const double* __restrict__ mass, const uint32_t* __restrict__ perm,
+                double* __restrict__ f, size_t n, int w)
+{
+	const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if(p >= n) { return; }
+	const uint32_t i = perm[p];
+	const uint32_t o = perm[p / kF32TgtPerGroup * kF32TgtPerGroup];     // the tile kernel's origin
+	const double ox = y[o], oy = y[n + o], oz = y[2 * n + o];
+	const double xi = y[i], yi = y[n + i], zi = y[2 * n + i];
+	const float txf = static_cast<float>((xi - ox) * kF32Len);
+	const float tyf = static_cast<float>((yi - oy) * kF32Len);
+	const float tzf = static_cast<float>((zi - oz) * kF32Len);
+	double cx = 0.0, cy = 0.0, cz = 0.0;
+	const size_t q0 = p > static_cast<size_t>(w) ? p - w : 0;
+	const size_t q1 = p + w < n - 1 ? p + w : n - 1;
+	for(size_t q = q0; q <= q1; ++q)
+	{
+		if(q == p) { continue; }
+		const uint32_t j = perm[q];
+		const double xj = y[j], yj = y[n + j], zj = y[2 * n + j], mj = mass[j];
+		const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+		double r2 = dx * dx + dy * dy + dz * dz;
+		if(r2 < kMinDistance) { r2 = kMinDistance; }
+		const double c = mj / (r2 * sqrt(r2));
+		const float fx = static_cast<float>((xj - ox) * kF32Len) - txf;
+		const float fy = static_cast<float>((yj - oy) * kF32Len) - tyf;
+		const float fz = static_cast<float>((zj - oz) * kF32Len) - tzf;
+		const float qf = rsq_clamped(fmaf(fz, fz, fmaf(fy, fy, fx * fx)));
+		const float wf = (static_cast<float>(mj * kF32Mass) * qf) * (qf * qf);
+		cx += dx * c - static_cast<double>(fx * wf);
+		cy += dy * c - static_cast<double>(fy * wf);
+		cz += dz * c - static_cast<double>(fz * wf);
+	}
+	f[3 * n + i] += cx;
+	f[4 * n + i] += cy;
+	f[5 * n + i] += cz;
+}
+
 // f[0:3n] = v, f[3n:6n] = sum over slices (fixed order).
 __global__ void __launch_bounds__(256)
 k_direct_fp32_finish(const double* __restrict__ y, const uint32_t* __restrict__ perm,
@@ -435,7 +481,8 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 		NBH_LAUNCHED();
 		return NBH_OK;
 	}
-	NBH_CHECK(precision == NBH_FP32, "unknown precision %d", static_cast<int>(precision));
+	NBH_CHECK(precision == NBH_FP32 || precision == NBH_FP32_CORR, "unknown precision %d",
+	          static_cast<int>(precision));
 	// Split the sources so that the grid holds ~8 work-groups per CU.
 	size_t gx = (n + kF32TgtPerGroup - 1) / kF32TgtPerGroup;
 	size_t want = static_cast<size_t>(ctx->cu_count > 0 ? ctx->cu_count : 256) * 8;
@@ -484,5 +531,11 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	hipLaunchKernelGGL(k_direct_fp32_finish, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0,
 	                   ctx->stream, y, perm, part, f, n, static_cast<int>(nsplit));
 	NBH_LAUNCHED();
+	if(precision == NBH_FP32_CORR)
+	{
+		hipLaunchKernelGGL(k_direct_window, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream,
+		                   y, mass, perm, f, n, kCorrWindow);
+		NBH_LAUNCHED();
+	}
 	return NBH_OK;
 }

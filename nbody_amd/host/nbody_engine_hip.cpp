@@ -594,7 +594,8 @@ void nbody_engine_hip::print_info() const
 	}
 	nbody_print("\tdevice: %d (%s, %d CUs)", m_device, name, cus);
 	nbody_print("\tblock_size: %d", m_block_size);
-	nbody_print("\tforce_precision: %s", m_precision == NBH_FP64 ? "fp64" : "fp32");
+	nbody_print("\tforce_precision: %s",
+	            m_precision == NBH_FP64 ? "fp64" : m_precision == NBH_FP32 ? "fp32" : "fp32_corr");
 }
 
 bool nbody_engine_hip::compute_statistics(bool with_energy, double out[12])

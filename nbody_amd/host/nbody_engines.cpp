@@ -34,9 +34,14 @@ bool common_hip_params(const nbody_params& param, nbody_engine_hip* engine)
 		return false;
 	}
 	std::string prec = param.value("force_precision", "fp64");
-	if(prec != "fp64" && prec != "fp32")
+	if(prec != "fp64" && prec != "fp32" && prec != "fp32_corr")
 	{
-		nbody_log("Invalid force_precision. Allowed values are 'fp64' or 'fp32'");
+		nbody_log("Invalid force_precision. Allowed values are 'fp64', 'fp32' or 'fp32_corr'");
+		return false;
+	}
+	if(prec == "fp32_corr" && param.value("engine", "") != "hip")
+	{
+		nbody_log("force_precision=fp32_corr is a direct-sum mode (engine hip)");
 		return false;
 	}
 	if(engine->select_device(device) != 0)
@@ -44,7 +49,7 @@ bool common_hip_params(const nbody_params& param, nbody_engine_hip* engine)
 		return false;
 	}
 	engine->set_block_size(static_cast<int>(block_size));
-	engine->set_precision(prec == "fp64" ? NBH_FP64 : NBH_FP32);
+	engine->set_precision(prec == "fp64" ? NBH_FP64 : prec == "fp32" ? NBH_FP32 : NBH_FP32_CORR);
 	return true;
 }
 

@@ -8,7 +8,7 @@
 //   block_size                      work-group size of the FP64 direct kernel
 //                                   (default 64; any other positive value is
 //                                   accepted and mapped to a valid size)
-//   force_precision                 "fp64" (default) | "fp32"
+//   force_precision                 "fp64" (default) | "fp32" | "fp32_corr" (hip only)
 //   distance_to_node_radius_ratio   lambda = 1/theta (hip_bh), default 10
 //   tree_build_rate                 0 = rebuild every fcompute (default); k > 0
 //                                   = rebuild when step % k == 0, refit otherwise

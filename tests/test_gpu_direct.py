@@ -15,7 +15,7 @@ from nbody_amd import native, nbody_data
 
 pytestmark = pytest.mark.gpu
 
-NBH_FP64, NBH_FP32 = 0, 1
+NBH_FP64, NBH_FP32, NBH_FP32_CORR = 0, 1, 2
 
 # Bounds on the relative error |a32 - a64| / |a64| of the FP32-tile kernel.
 # Sources enter as FP32 offsets from the work-group's first target and the
@@ -114,3 +114,40 @@ def test_fp32_matches_fp64_kernel_large(ctx):
     # momentum balance of the FP64 kernel: sum m a ~ 0 relative to sum m |a|
     a = f64[3 * n:].reshape(3, n)
     assert np.linalg.norm((a * m).sum(axis=1)) / np.sum(m * np.linalg.norm(a, axis=0)) < 1e-12
+
+
+def test_fp32_corr_close_pair_far_from_origin(ctx):
+    """A close pair 10 units from the tiles' origin: the FP32 rounding of the
+    offsets (about 7e-7 here) is a 1e-3 error on their 1e-3 separation, which the
+    FP64 correction pass removes; the other bodies keep FP32 accuracy or better."""
+    rng = np.random.default_rng(3)
+    n = 64
+    pos = rng.normal(size=(n, 3)) * 0.3
+    pos[-2] = [10.0, 0.0, 0.0]
+    pos[-1] = [10.00123456789, 0.000333333, -0.000271828]   # no multiple of the float spacing
+    m = rng.uniform(0.5, 1.5, n) / n
+    y = np.concatenate([pos[:, 0], pos[:, 1], pos[:, 2], rng.normal(size=3 * n)])
+    f0 = oracle_lib.fcompute_simple(y, m)
+    e32 = _rel_err(_run(ctx, y, m, NBH_FP32), f0, n)
+    fc = _run(ctx, y, m, NBH_FP32_CORR)
+    assert np.array_equal(fc[:3 * n], y[3 * n:])
+    ec = _rel_err(fc, f0, n)
+    print(f"\n[fp32 correction] close pair rel err: FP32 {e32[-2:].max():.2e}, corrected {ec[-2:].max():.2e}; "
+          f"others max {e32[:-2].max():.2e} -> {ec[:-2].max():.2e}")
+    assert e32[-2:].min() > 1e-5                  # the FP32 tiles do lose the pair
+    assert ec[-2:].max() < 1e-7
+    assert ec.max() < 1e-6
+
+
+@pytest.mark.parametrize("kind,n", [("plummer", 32768), ("cube", 16384)])
+def test_fp32_corr_accuracy(ctx, kind, n):
+    """With the correction pass the force error against the FP64 kernel drops
+    below that of the FP32 tiles alone (largest and RMS relative error)."""
+    y, m = _state(kind, n, 7)
+    f64 = _run(ctx, y, m, NBH_FP64)
+    e32 = _rel_err(_run(ctx, y, m, NBH_FP32), f64, n)
+    ec = _rel_err(_run(ctx, y, m, NBH_FP32_CORR), f64, n)
+    r32, rc = float(np.sqrt(np.mean(e32 ** 2))), float(np.sqrt(np.mean(ec ** 2)))
+    print(f"\n[fp32 correction] {kind} n={n}: rms {r32:.2e} -> {rc:.2e}, max {e32.max():.2e} -> {ec.max():.2e}")
+    _check_rel(ec)
+    assert rc < r32 and ec.max() < e32.max()

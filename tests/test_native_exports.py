@@ -50,6 +50,7 @@ def test_factory_validates_before_device():
     assert nbody_create_engine({"engine": "hip_bh", "traverse_type": "zigzag"}) is None
     assert nbody_create_engine({"engine": "hip_bh", "tree_build_rate": "-1"}) is None
     assert nbody_create_engine({"engine": "hip_bh", "distance_to_node_radius_ratio": "0"}) is None
+    assert nbody_create_engine({"engine": "hip_bh", "force_precision": "fp32_corr"}) is None   # direct only
     grp = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group"}
     assert nbody_create_engine({**grp, "bh_far_field": "tree"}) is None
     assert nbody_create_engine({**grp, "bh_far_field": "local"}) is None                     # needs quadrupole
