@@ -536,6 +536,28 @@ def test_two_level_local_walk_matches_oracle(ctx):
     t.close()
 
 
+def test_two_level_walk_overflow_fallback(ctx):
+    """At 2^22 Plummer bodies a few super-groups list more nodes than the list
+    holds; their groups walk from the root instead.  The whole walk keeps the
+    one-level local walk's accuracy against the FP64 direct sum."""
+    n, ratio, near_k = 1 << 22, 1.0 / 0.6, 3.0
+    y, m = _state("plummer", n, 4)
+    t = Tree(ctx, y, m, ratio)
+    idx = np.random.default_rng(6).choice(n, 1024, replace=False)
+    a0 = _sample_accel(ctx, t.dy, t.dm, n, idx)
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group_local(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local")
+    e1 = _rms_rel(df.read(), a0, n, idx)
+    native.check(ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local2")
+    overflowed = t.counters()[2]
+    e2 = _rms_rel(df.read(), a0, n, idx)
+    print(f"\n[two-level walk, 2^22] {overflowed} super-groups fell back; rms one-level {e1[0]:.3e}, "
+          f"two-level {e2[0]:.3e}")
+    assert overflowed > 0
+    assert e2[0] < 1.1 * e1[0] and e2[0] < 1.2e-3
+    t.close()
+
+
 def test_local_far_field_walk_accuracy(ctx):
     """At theta 0.6 the local far field keeps the quadrupole walk's accuracy
     against the FP64 direct sum (every cell per body as the reference point)."""
