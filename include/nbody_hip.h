@@ -163,7 +163,7 @@ nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* tree, const 
 nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f,
                                        float near_k);
 /* nbh_bh_fcompute_group_local in two levels: the tree is walked once per
-   super-group of 512 bodies (8 groups) with the super-group's box; accepted
+   super-group of 1024 bodies (16 groups) with the super-group's box; accepted
    cells at least near_k half-diagonals of that box from its centre form one
    expansion about it (monopole to third order, quadrupole to first), and the
    other accepted nodes a list that each group takes instead of walking (leaves
@@ -171,7 +171,7 @@ nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const
    node accepted for the super-group's box is accepted for every group box
    inside it, so the groups keep the acceptance rule.  Lists longer than 32768
    entries fall back to the one-level walk for that super-group (counted in
-   walk counter out[2]).  The first call allocates n * 64 bytes of lists.
+   walk counter out[2]).  The first call allocates n * 128 bytes of lists.
    Not in the reference. */
 nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* tree, const double* y, double* f, float near_k);
 /* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may

@@ -993,7 +993,7 @@ __device__ inline void far_batch(const float4* fq, const int* fn, const nbh_walk
 }
 
 // Two-level local walk.  A super-group is the heap node of 2^kSuperShift
-// consecutive groups (512 bodies).  One wave walks the tree once for the
+// consecutive groups (1024 bodies).  One wave walks the tree once for the
 // super-group's box: a node it accepts is accepted by every group inside that
 // box too (the distance from a smaller box inside is no shorter), so accepted
 // cells at least near_k half-diagonals of the super box from its centre go into
@@ -1006,7 +1006,7 @@ __device__ inline void far_batch(const float4* fq, const int* fn, const nbh_walk
 // walk does; each body adds both expansions.  A list longer than `cap`, or a
 // stack spill in the super walk, makes the groups of that super-group walk from
 // the root instead.
-constexpr int kSuperShift = 3;
+constexpr int kSuperShift = 4;
 
 struct TwoLevel {
 	int*   lists = nullptr;   // [super][cap] listed nodes

@@ -17,7 +17,7 @@ Contents
   * group_band: the near and far cells of one 64-body group, by a plain stack
     walk of the heap tree with the group walk's acceptance rule; super_band /
     group_band_from: the two walks of the two-level local walk (super-group of
-    8 groups, then each group from the super-group's list);
+    2^shift groups, 16 in the kernels, then each group from the super-group's list);
   * quadrupole_field: the textbook quadrupole term of a cell's field;
   * local_expansion_tensors / local_expansion_eval: second- and third-order
     Taylor expansions of cells' monopole fields about a group centre (the far
@@ -369,7 +369,7 @@ def _box(bmin, bmax, node):
     return lo, hi, 0.5 * (lo + hi), 0.5 * np.linalg.norm(hi - lo)
 
 
-def super_band(n, s, cm, crit, bmin, bmax, near_k, ratio, shift=3):
+def super_band(n, s, cm, crit, bmin, bmax, near_k, ratio, shift=4):
     """(far, listed) heap nodes of super-group s (heap node n / 64 / 2^shift + s)
     of the two-level local walk: walking from the root with the super-group's
     box, an accepted internal node whose mass centre lies at least near_k

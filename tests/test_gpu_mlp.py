@@ -515,14 +515,14 @@ def test_two_level_local_walk_matches_oracle(ctx):
     got = df.read()[3 * n:].reshape(3, n).T
     assert t.counters()[2] == 0                     # no list overflow, no spill
     errs, shared = [], 0
-    for s in (5, 170):
-        sg = (n >> 9) + s
+    for s in (5, 100):
+        sg = (n >> 10) + s
         scen = 0.5 * (a["bmin"][sg] + a["bmax"][sg])
         shd = 0.5 * np.linalg.norm(a["bmax"][sg] - a["bmin"][sg])
-        far_s, listed = mo.super_band(n, s, a["cm"], a["crit"], a["bmin"], a["bmax"], near_k, ratio)
+        far_s, listed = mo.super_band(n, s, a["cm"], a["crit"], a["bmin"], a["bmax"], near_k, ratio, shift=4)
         shared += len(far_s)
-        for gsub in range(8):
-            g = 8 * s + gsub
+        for gsub in range(0, 16, 2):
+            g = 16 * s + gsub
             far, taken, near = mo.group_band_from(n, g, listed, a["cm"], a["crit"], a["bmin"], a["bmax"], near_k)
             X, v = _group_terms(n, a, g, far, taken, near)
             v = v + _far_expansion(a, n, far_s, scen, shd, X)

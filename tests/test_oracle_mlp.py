@@ -298,7 +298,7 @@ def test_quadrupole_fields_matches_single_cell_form():
 
 def test_two_level_cell_sets_cover_every_body_once():
     """The two-level local walk's cell sets (super-group walk, then each of its
-    8 groups from the list) cover every body of the tree exactly once per group,
+    16 groups from the list) cover every body of the tree exactly once per group,
     and the super-group walk does share far cells at this size."""
     n, ratio, near_k = 1 << 17, 1.0 / 0.6, 3.0
     y, m = _plummer_like(n, 40)
@@ -306,11 +306,11 @@ def test_two_level_cell_sets_cover_every_body_once():
     h.build(y, m, ratio)
     o = h.arrays()
     shared = 0
-    for s in (3, 200):
-        far, listed = mo.super_band(n, s, o["cm"], o["crit_r2"], o["bmin"], o["bmax"], near_k, ratio)
+    for s in (3, 100):
+        far, listed = mo.super_band(n, s, o["cm"], o["crit_r2"], o["bmin"], o["bmax"], near_k, ratio, shift=4)
         shared += len(far)
         for gsub in (0, 5):
-            _, taken, _ = mo.group_band_from(n, 8 * s + gsub, listed, o["cm"], o["crit_r2"], o["bmin"], o["bmax"],
+            _, taken, _ = mo.group_band_from(n, 16 * s + gsub, listed, o["cm"], o["crit_r2"], o["bmin"], o["bmax"],
                                              near_k)
             cover = np.zeros(n, int)
             for node in far + taken:
