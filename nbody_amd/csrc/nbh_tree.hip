@@ -39,6 +39,11 @@ constexpr int kPartItems = 4;
 constexpr int kPartTile = kPartThreads * kPartItems;   // 1024 entries per work-group
 constexpr int kScanThreads = 256;
 constexpr int kScanParts = 1024;                     // chunks of one scan
+constexpr int kScanOneThreads = 1024;
+constexpr int kScanOneItems = 16;                    // scans of up to 16384 counts in one work-group
+constexpr int kLocalSeg = 4096;                      // segments split in LDS, one work-group each
+constexpr int kLocalThreads = 256;
+constexpr int kLocalItems = kLocalSeg / kLocalThreads;
 
 }  // namespace
 
@@ -203,6 +208,42 @@ k_scan_apply(uint32_t* __restrict__ c, size_t m, size_t chunk, const uint32_t* _
 	}
 }
 
+// Exclusive scan in place of m <= kScanOneThreads * kScanOneItems counts by one
+// work-group: each work-item scans a run of consecutive counts.
+__global__ void __launch_bounds__(kScanOneThreads)
+k_scan_one(uint32_t* __restrict__ c, int m)
+{
+	__shared__ uint32_t ws[kScanOneThreads / 64];
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	const int per = (m + kScanOneThreads - 1) / kScanOneThreads;
+	const int i0 = static_cast<int>(threadIdx.x) * per;
+	uint32_t v[kScanOneItems];
+	uint32_t s = 0;
+#pragma unroll
+	for(int k = 0; k < kScanOneItems; ++k)
+	{
+		v[k] = k < per && i0 + k < m ? c[i0 + k] : 0;
+		s += v[k];
+	}
+	uint32_t x = s;
+	for(int off = 1; off < 64; off <<= 1)
+	{
+		const uint32_t u = __shfl_up(x, off, 64);
+		if(lane >= off) { x += u; }
+	}
+	if(lane == 63) { ws[wave] = x; }
+	__syncthreads();
+	uint32_t run = x - s;
+	for(int w = 0; w < wave; ++w) { run += ws[w]; }
+#pragma unroll
+	for(int k = 0; k < kScanOneItems; ++k)
+	{
+		if(k < per && i0 + k < m) { c[i0 + k] = run; }
+		run += v[k];
+	}
+}
+
 // Stable scatter: rows of 256 keys are ranked in index order, wave by wave,
 // with a ballot match on the 8 digit bits.
 __global__ void __launch_bounds__(kSortThreads)
@@ -348,6 +389,104 @@ k_part_scatter(const uint32_t* __restrict__ list, const uint8_t* __restrict__ si
 		size_t pos = fl[k] ? seg_start + half + before : p - before;
 		out[pos] = body[k];
 	}
+}
+
+// Levels first_level .. levels-1 of the split (the steps of k_mark_side and
+// k_part_scatter) for segments of seg0 <= kLocalSeg bodies, one work-group per
+// segment, in LDS.  Bodies get local numbers (their position in list0's
+// segment, through the global table g2l) so that the side flags live in LDS.
+// Only the final order is written back, to list0: after the last level every
+// segment holds one body and the three lists agree.
+__global__ void __launch_bounds__(kLocalThreads)
+k_split_local(uint32_t* __restrict__ list0, const uint32_t* __restrict__ list1, const uint32_t* __restrict__ list2,
+              uint32_t* __restrict__ g2l, int seg0, int first_level, int levels)
+{
+	__shared__ uint16_t ls[3][kLocalSeg];
+	__shared__ uint16_t tmp[kLocalSeg];
+	__shared__ uint16_t excl[kLocalSeg];
+	__shared__ uint32_t l2g[kLocalSeg];
+	__shared__ uint8_t side[kLocalSeg];
+	__shared__ uint32_t ws[kLocalThreads / 64];
+	const size_t base = static_cast<size_t>(blockIdx.x) * seg0;
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	for(int p = threadIdx.x; p < seg0; p += kLocalThreads)
+	{
+		const uint32_t b = list0[base + p];
+		l2g[p] = b;
+		ls[0][p] = static_cast<uint16_t>(p);
+		g2l[b] = static_cast<uint32_t>(p);
+	}
+	__syncthreads();
+	for(int p = threadIdx.x; p < seg0; p += kLocalThreads)
+	{
+		ls[1][p] = static_cast<uint16_t>(g2l[list1[base + p]]);
+		ls[2][p] = static_cast<uint16_t>(g2l[list2[base + p]]);
+	}
+	__syncthreads();
+	// work-item t owns positions [t * per, t * per + per) of a list
+	const int per = (seg0 + kLocalThreads - 1) / kLocalThreads;
+	const int p0 = static_cast<int>(threadIdx.x) * per;
+	for(int level = first_level; level < levels; ++level)
+	{
+		const int seg = seg0 >> (level - first_level);
+		const int half = seg >> 1;
+		const int a = level % 3;
+		for(int p = threadIdx.x; p < seg0; p += kLocalThreads)
+		{
+			side[ls[a][p]] = (p & (seg - 1)) >= half ? 1 : 0;
+		}
+		__syncthreads();
+		for(int b = 0; b < 3; ++b)
+		{
+			if(b == a) { continue; }
+			uint32_t c = 0;
+			for(int k = 0; k < per; ++k)
+			{
+				const int p = p0 + k;
+				if(p < seg0)
+				{
+					excl[p] = static_cast<uint16_t>(c);
+					c += side[ls[b][p]];
+				}
+			}
+			uint32_t x = c;
+			for(int off = 1; off < 64; off <<= 1)
+			{
+				const uint32_t u = __shfl_up(x, off, 64);
+				if(lane >= off) { x += u; }
+			}
+			if(lane == 63) { ws[wave] = x; }
+			__syncthreads();
+			uint32_t add = x - c;
+			for(int w = 0; w < wave; ++w) { add += ws[w]; }
+			for(int k = 0; k < per; ++k)
+			{
+				const int p = p0 + k;
+				if(p < seg0) { excl[p] = static_cast<uint16_t>(excl[p] + add); }
+			}
+			__syncthreads();
+			for(int k = 0; k < per; ++k)
+			{
+				const int p = p0 + k;
+				if(p < seg0)
+				{
+					const uint16_t id = ls[b][p];
+					const int seg_start = p & ~(seg - 1);
+					const int before = excl[p] - excl[seg_start];
+					tmp[side[id] ? seg_start + half + before : p - before] = id;
+				}
+			}
+			__syncthreads();
+			for(int k = 0; k < per; ++k)
+			{
+				const int p = p0 + k;
+				if(p < seg0) { ls[b][p] = tmp[p]; }
+			}
+			__syncthreads();
+		}
+	}
+	for(int p = threadIdx.x; p < seg0; p += kLocalThreads) { list0[base + p] = l2g[ls[0][p]]; }
 }
 
 // ---------------------------------------------------------------------------
@@ -1778,6 +1917,12 @@ k_bh_group32_mlp(const nbh_walk_rec* __restrict__ wr, const double* __restrict__
 // ---------------------------------------------------------------------------
 nbh_status scan_exclusive(nbh_ctx* ctx, uint32_t* c, size_t m, uint32_t* scan_part)
 {
+	if(m <= static_cast<size_t>(kScanOneThreads) * kScanOneItems)
+	{
+		hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, ctx->stream, c, static_cast<int>(m));
+		NBH_LAUNCHED();
+		return NBH_OK;
+	}
 	size_t chunk = (m + kScanParts - 1) / kScanParts;
 	chunk = (chunk + 4 * kScanThreads - 1) / (4 * kScanThreads) * (4 * kScanThreads);
 	const unsigned g = static_cast<unsigned>((m + chunk - 1) / chunk);
@@ -1963,7 +2108,11 @@ nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* t, const double* y, const doub
 		NBH_TRY(radix_sort(ctx, t, y + static_cast<size_t>(a) * n, t->list[a]));
 	}
 	const size_t ntiles = (n + kPartTile - 1) / kPartTile;
-	for(int level = 0; level < t->levels; ++level)
+	// the last levels (segments of at most kLocalSeg bodies) run in LDS
+	const int local_levels = std::min(t->levels, 12);
+	static_assert(kLocalSeg == 1 << 12, "kLocalSeg is 2^12");
+	const int global_levels = t->levels - local_levels;
+	for(int level = 0; level < global_levels; ++level)
 	{
 		const size_t seg = n >> level;
 		const int a = level % 3;
@@ -1985,6 +2134,14 @@ nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* t, const double* y, const doub
 			NBH_LAUNCHED();
 			std::swap(t->list[b], t->tmp);
 		}
+	}
+	if(local_levels > 0)
+	{
+		const size_t seg0 = n >> global_levels;
+		hipLaunchKernelGGL(k_split_local, dim3(static_cast<unsigned>(n / seg0)), dim3(kLocalThreads), 0, ctx->stream,
+		                   t->list[0], t->list[1], t->list[2], t->tmp, static_cast<int>(seg0), global_levels,
+		                   t->levels);
+		NBH_LAUNCHED();
 	}
 	NBH_TRY(moments(ctx, t, y, mass, ratio, true));
 	t->built = true;
