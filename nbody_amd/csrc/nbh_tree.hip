@@ -42,8 +42,7 @@ constexpr int kScanParts = 1024;                     // chunks of one scan
 constexpr int kScanOneThreads = 1024;
 constexpr int kScanOneItems = 16;                    // scans of up to 16384 counts in one work-group
 constexpr int kLocalSeg = 4096;                      // segments split in LDS, one work-group each
-constexpr int kLocalThreads = 256;
-constexpr int kLocalItems = kLocalSeg / kLocalThreads;
+constexpr int kLocalThreads = 1024;
 
 }  // namespace
 
