@@ -32,8 +32,8 @@ namespace {
 constexpr int kRoot = 1;               // NBODY_HEAP_ROOT_INDEX
 constexpr int kMaxStack = 64;          // MAX_STACK_SIZE of the reference traversal
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 4;
-constexpr int kSortTile = kSortThreads * kSortItems;   // 1024 keys per work-group
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;   // 4096 keys per work-group
 constexpr int kPartThreads = 256;
 constexpr int kPartItems = 4;
 constexpr int kPartTile = kPartThreads * kPartItems;   // 1024 entries per work-group
@@ -243,54 +243,88 @@ k_scan_one(uint32_t* __restrict__ c, int m)
 	}
 }
 
-// Stable scatter: rows of 256 keys are ranked in index order, wave by wave,
-// with a ballot match on the 8 digit bits.
+// Stable scatter of one 4096-key tile: rows of 256 keys are ranked in index
+// order, wave by wave, with a ballot match on the 8 digit bits, and placed in
+// LDS in digit order; the tile then leaves as one contiguous run per digit.
+// `offsets` is the exclusive scan (digit-major) of the per-tile digit counts,
+// so a tile's count of digit d is the next entry minus its own (n after the
+// last).
 __global__ void __launch_bounds__(kSortThreads)
 k_radix_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                 uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n, int shift,
                 const uint32_t* __restrict__ offsets, int nblocks)
 {
 	constexpr int kWaves = kSortThreads / 64;
-	__shared__ uint32_t running[256];
+	__shared__ uint64_t skey[kSortTile];
+	__shared__ uint32_t sval[kSortTile];
+	__shared__ uint32_t goff[256];     // first output position of digit d for this tile
+	__shared__ uint32_t loff[256];     // first LDS position of digit d
+	__shared__ uint32_t running[256];  // keys of digit d placed so far
 	__shared__ uint32_t wcount[kWaves][256];
+	__shared__ uint32_t wsum[kWaves];
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
-	running[threadIdx.x] = offsets[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x];
 	const size_t base = static_cast<size_t>(blockIdx.x) * kSortTile;
+	const int tcount = n - base < static_cast<size_t>(kSortTile) ? static_cast<int>(n - base) : kSortTile;
+	{
+		const size_t idx = static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x;
+		const uint32_t g0 = offsets[idx];
+		const size_t m = static_cast<size_t>(nblocks) * 256;
+		const uint32_t cnt = (idx + 1 < m ? offsets[idx + 1] : static_cast<uint32_t>(n)) - g0;
+		goff[threadIdx.x] = g0;
+		running[threadIdx.x] = 0;
+		uint32_t x = cnt;
+		for(int off = 1; off < 64; off <<= 1)
+		{
+			const uint32_t u = __shfl_up(x, off, 64);
+			if(lane >= off) { x += u; }
+		}
+		if(lane == 63) { wsum[wave] = x; }
+		__syncthreads();
+		uint32_t ex = x - cnt;
+		for(int w = 0; w < wave; ++w) { ex += wsum[w]; }
+		loff[threadIdx.x] = ex;
+	}
 	const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 	for(int k = 0; k < kSortItems; ++k)
 	{
 		for(int w = 0; w < kWaves; ++w) { wcount[w][threadIdx.x] = 0; }
 		__syncthreads();
-		const size_t i = base + static_cast<size_t>(k) * kSortThreads + threadIdx.x;
-		const bool ok = i < n;
-		uint64_t key = ok ? kin[i] : 0;
-		uint32_t val = ok ? vin[i] : 0;
-		uint32_t d = static_cast<uint32_t>((key >> shift) & 0xFF);
-		uint64_t active = __ballot(ok);
-		uint64_t peers = active;
+		const int r = k * kSortThreads + threadIdx.x;
+		const bool ok = r < tcount;
+		const uint64_t key = ok ? kin[base + r] : 0;
+		const uint32_t val = ok ? vin[base + r] : 0;
+		const uint32_t d = static_cast<uint32_t>((key >> shift) & 0xFF);
+		uint64_t peers = __ballot(ok);
 		for(int bit = 0; bit < 8; ++bit)
 		{
-			uint64_t bal = __ballot((d >> bit) & 1u);
+			const uint64_t bal = __ballot((d >> bit) & 1u);
 			peers &= ((d >> bit) & 1u) ? bal : ~bal;
 		}
-		uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt_mask));
-		uint32_t cnt = static_cast<uint32_t>(__popcll(peers));
-		bool leader = ok && (peers & lt_mask) == 0;
-		if(leader) { wcount[wave][d] = cnt; }
+		const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt_mask));
+		const uint32_t cnt = static_cast<uint32_t>(__popcll(peers));
+		if(ok && (peers & lt_mask) == 0) { wcount[wave][d] = cnt; }
 		__syncthreads();
 		if(ok)
 		{
-			uint32_t pos = running[d] + rank;
+			uint32_t pos = loff[d] + running[d] + rank;
 			for(int w = 0; w < wave; ++w) { pos += wcount[w][d]; }
-			kout[pos] = key;
-			vout[pos] = val;
+			skey[pos] = key;
+			sval[pos] = val;
 		}
 		__syncthreads();
 		uint32_t add = 0;
 		for(int w = 0; w < kWaves; ++w) { add += wcount[w][threadIdx.x]; }
 		running[threadIdx.x] += add;
-		__syncthreads();
+	}
+	__syncthreads();
+	for(int i = threadIdx.x; i < tcount; i += kSortThreads)
+	{
+		const uint64_t key = skey[i];
+		const uint32_t d = static_cast<uint32_t>((key >> shift) & 0xFF);
+		const size_t pos = static_cast<size_t>(goff[d]) + static_cast<uint32_t>(i - static_cast<int>(loff[d]));
+		kout[pos] = key;
+		vout[pos] = sval[i];
 	}
 }
 
