@@ -14,6 +14,12 @@
 //   rsq           v_rsq_f32
 //   pair_regs     the direct kernel's pair arithmetic (13 instructions per
 //                 pair) with the sources held in registers
+//   pair_regs_v   the same with targets that differ in every coordinate per lane
+//   pair_mfma     the pair arithmetic up to q^3 (8 VALU + v_rsq_f32), the sum
+//                 over sources on v_mfma_f32_4x4x1_16b_f32 (one per 64 pairs)
+//   pair_mfma_v   the same with the per-lane targets of pair_regs_v
+// and checks the lane layout of v_mfma_f32_4x4x1_16b_f32 (A lane 4b+i is row i
+// of block b, B lane 4b+j column j, D lane 4b+j register i).
 //
 // Usage: valu_issue  (prints one line per kernel: best of 3 launches)
 #include <hip/hip_runtime.h>
@@ -147,6 +153,112 @@ __global__ void __launch_bounds__(kThreads) k_pair_regs(float* out, float a, flo
 	out[blockIdx.x * kThreads + threadIdx.x] = s;
 }
 
+template <bool kVary>
+__global__ void __launch_bounds__(kThreads) k_pair_regs_t(float* out, float a, float b)
+{
+	float xi[4], yi[4], zi[4], px[4], py[4], pz[4];
+	for(int k = 0; k < 4; ++k)
+	{
+		xi[k] = threadIdx.x * 1e-3f + k;
+		yi[k] = kVary ? threadIdx.x * 0.5f * k : 0.5f * k;
+		zi[k] = kVary ? -0.25f * k * threadIdx.x : -0.25f * k;
+		px[k] = py[k] = pz[k] = 0.0f;
+	}
+	float4 s4[4];
+	for(int j = 0; j < 4; ++j) { s4[j] = make_float4(a * (j + 1), b * j - 1.0f, 0.5f * j, 0.1f * (j + 1)); }
+	for(int i = 0; i < kPairIters; ++i)
+	{
+#pragma unroll
+		for(int j = 0; j < 4; ++j)
+		{
+			const float4 s = s4[j];
+#pragma unroll
+			for(int k = 0; k < 4; ++k)
+			{
+				const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+				const float q = __builtin_amdgcn_fmed3f(
+				    __builtin_amdgcn_rsqf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))), 0.0f, 1.0f);
+				const float w = (s.w * q) * (q * q);
+				px[k] = fmaf(dx, w, px[k]);
+				py[k] = fmaf(dy, w, py[k]);
+				pz[k] = fmaf(dz, w, pz[k]);
+			}
+		}
+		const float4 t = s4[0];
+		s4[0] = s4[1];
+		s4[1] = s4[2];
+		s4[2] = s4[3];
+		s4[3] = t;
+	}
+	float s = 0.0f;
+	for(int k = 0; k < 4; ++k) { s += px[k] + py[k] + pz[k]; }
+	out[blockIdx.x * kThreads + threadIdx.x] = s;
+}
+
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// Sum over sources as a contraction: acc[k] += q^3 (this lane's target k) x B,
+// B = one component of the source per lane (as m x, m y, m z, m would be).
+template <bool kVary>
+__global__ void __launch_bounds__(kThreads) k_pair_mfma_t(float* out, float a, float b)
+{
+	float xi[4], yi[4], zi[4];
+	mfma_f4 acc[4];
+	for(int k = 0; k < 4; ++k)
+	{
+		xi[k] = threadIdx.x * 1e-3f + k;
+		yi[k] = kVary ? threadIdx.x * 0.5f * k : 0.5f * k;
+		zi[k] = kVary ? -0.25f * k * threadIdx.x : -0.25f * k;
+		acc[k] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
+	}
+	float4 s4[4];
+	float bb[4];
+	for(int j = 0; j < 4; ++j)
+	{
+		s4[j] = make_float4(a * (j + 1), b * j - 1.0f, 0.5f * j, 0.1f * (j + 1));
+		bb[j] = 0.3f * j + threadIdx.x * 1e-4f;
+	}
+	for(int i = 0; i < kPairIters; ++i)
+	{
+#pragma unroll
+		for(int j = 0; j < 4; ++j)
+		{
+			const float4 s = s4[j];
+#pragma unroll
+			for(int k = 0; k < 4; ++k)
+			{
+				const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+				const float q = __builtin_amdgcn_fmed3f(
+				    __builtin_amdgcn_rsqf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))), 0.0f, 1.0f);
+				acc[k] = __builtin_amdgcn_mfma_f32_4x4x1f32((q * q) * q, bb[j], acc[k], 0, 0, 0);
+			}
+		}
+		const float4 t = s4[0];
+		s4[0] = s4[1];
+		s4[1] = s4[2];
+		s4[2] = s4[3];
+		s4[3] = t;
+		const float tb = bb[0];
+		bb[0] = bb[1];
+		bb[1] = bb[2];
+		bb[2] = bb[3];
+		bb[3] = tb;
+	}
+	float s = 0.0f;
+	for(int k = 0; k < 4; ++k) { s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3]; }
+	out[blockIdx.x * kThreads + threadIdx.x] = s;
+}
+
+// Work-group w (one wave): A = 1 on lane w only, B = lane + 1; out[w][lane][4] = D.
+__global__ void __launch_bounds__(64) k_mfma_layout(float* out)
+{
+	const int l = threadIdx.x;
+	const int w = blockIdx.x;
+	const mfma_f4 d = __builtin_amdgcn_mfma_f32_4x4x1f32(l == w ? 1.0f : 0.0f, static_cast<float>(l + 1),
+	                                                     mfma_f4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+	for(int v = 0; v < 4; ++v) { out[(w * 64 + l) * 4 + v] = d[v]; }
+}
+
 } // namespace
 
 int main()
@@ -172,6 +284,9 @@ int main()
 	    {"mul_vop2", k_mul_vop2, 8.0 * kIters},
 	    {"rsq", k_rsq, 8.0 * kIters},
 	    {"pair_regs", k_pair_regs, 16.0 * kPairIters * 13.0},
+	    {"pair_regs_v", k_pair_regs_t<true>, 16.0 * kPairIters * 13.0},
+	    {"pair_mfma", k_pair_mfma_t<false>, 16.0 * kPairIters * 13.0},
+	    {"pair_mfma_v", k_pair_mfma_t<true>, 16.0 * kPairIters * 13.0},
 	};
 	std::printf("device %s, %d CUs, clock %d MHz\n", prop.gcnArchName, cus, prop.clockRate / 1000);
 	for(const Probe& p : probes)
@@ -193,6 +308,27 @@ int main()
 		std::printf("%-10s %8.3f ms  %.3e wave-instr/s  %.3f per SIMD per clock at 2.4 GHz\n", p.name, best, rate,
 		            rate / (4.0 * cus) / 2.4e9);
 	}
+	// the pair_* lines are normalised to 13 instructions per pair for every form:
+	// compare their times (equal pair counts)
+	static float lay[64 * 64 * 4];
+	hipLaunchKernelGGL(k_mfma_layout, dim3(64), dim3(64), 0, 0, out);
+	PROBE_CHECK(hipGetLastError());
+	PROBE_CHECK(hipMemcpy(lay, out, sizeof(lay), hipMemcpyDeviceToHost));
+	int bad = 0;
+	for(int w = 0; w < 64; ++w)
+	{
+		for(int l = 0; l < 64; ++l)
+		{
+			for(int v = 0; v < 4; ++v)
+			{
+				const bool hit = l / 4 == w / 4 && v == w % 4;
+				const float want = hit ? static_cast<float>(l + 1) : 0.0f;
+				if(lay[(w * 64 + l) * 4 + v] != want) { ++bad; }
+			}
+		}
+	}
+	std::printf("mfma_f32_4x4x1 layout (A lane 4b+i row i, B lane 4b+j column j, D lane 4b+j register i): %s\n",
+	            bad == 0 ? "ok" : "MISMATCH");
 	PROBE_CHECK(hipFree(out));
-	return 0;
+	return bad == 0 ? 0 : 2;
 }
