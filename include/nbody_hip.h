@@ -164,7 +164,7 @@ nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const
                                        float near_k);
 /* nbh_bh_fcompute_group_local in two levels: the tree is walked once per
    super-group of 1024 bodies (16 groups) with the super-group's box; accepted
-   cells at least near_k half-diagonals of that box from its centre form one
+   cells at least max(near_k, 5) half-diagonals of that box from its centre form one
    expansion about it (monopole to third order, quadrupole to first), and the
    other accepted nodes a list that each group takes instead of walking (leaves
    and nearer cells per body, farther ones into the group's own expansion).  A
@@ -174,6 +174,11 @@ nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const
    walk counter out[2]).  The first call allocates n * 128 bytes of lists.
    Not in the reference. */
 nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* tree, const double* y, double* f, float near_k);
+/* List capacity per super-group of nbh_bh_fcompute_group_local2 (default 32768
+   entries; lists take n / 1024 * cap * 4 bytes).  Super-groups with longer lists
+   walk in one level.  Frees lists already allocated; cap in [64, 2^24].  No
+   counterpart in the reference: it bounds the two-level walk's memory. */
+nbh_status nbh_tree_set_list_cap(nbh_ctx* ctx, nbh_tree* tree, int cap);
 /* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may
    be NULL.  cm/bmin/bmax are xyz-interleaved [2n*3]; body_n holds -1 on
    internal nodes.  Used to check the layout against the reference heap

@@ -519,7 +519,9 @@ def test_two_level_local_walk_matches_oracle(ctx):
         sg = (n >> 10) + s
         scen = 0.5 * (a["bmin"][sg] + a["bmax"][sg])
         shd = 0.5 * np.linalg.norm(a["bmax"][sg] - a["bmin"][sg])
-        far_s, listed = mo.super_band(n, s, a["cm"], a["crit"], a["bmin"], a["bmax"], near_k, ratio, shift=4)
+        # the super-group level uses max(near_k, 5) super-group half-diagonals
+        far_s, listed = mo.super_band(n, s, a["cm"], a["crit"], a["bmin"], a["bmax"], max(near_k, 5.0), ratio,
+                                      shift=4)
         shared += len(far_s)
         for gsub in range(0, 16, 2):
             g = 16 * s + gsub
@@ -537,9 +539,10 @@ def test_two_level_local_walk_matches_oracle(ctx):
 
 
 def test_two_level_walk_overflow_fallback(ctx):
-    """At 2^22 Plummer bodies a few super-groups list more nodes than the list
-    holds; their groups walk from the root instead.  The whole walk keeps the
-    one-level local walk's accuracy against the FP64 direct sum."""
+    """At 2^22 Plummer bodies with the default list capacity no super-group
+    falls back; with a capacity of 2048 entries some lists overflow and their
+    groups walk from the root instead.  Both keep the one-level local walk's
+    accuracy against the FP64 direct sum."""
     n, ratio, near_k = 1 << 22, 1.0 / 0.6, 3.0
     y, m = _state("plummer", n, 4)
     t = Tree(ctx, y, m, ratio)
@@ -549,12 +552,17 @@ def test_two_level_walk_overflow_fallback(ctx):
     native.check(ctx.lib.nbh_bh_fcompute_group_local(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local")
     e1 = _rms_rel(df.read(), a0, n, idx)
     native.check(ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local2")
-    overflowed = t.counters()[2]
+    full = t.counters()[2]
     e2 = _rms_rel(df.read(), a0, n, idx)
-    print(f"\n[two-level walk, 2^22] {overflowed} super-groups fell back; rms one-level {e1[0]:.3e}, "
-          f"two-level {e2[0]:.3e}")
-    assert overflowed > 0
-    assert e2[0] < 1.1 * e1[0] and e2[0] < 1.2e-3
+    native.check(ctx.lib.nbh_tree_set_list_cap(ctx.h, t.h, 2048), "cap")
+    native.check(ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local2, cap 2048")
+    overflowed = t.counters()[2]
+    e3 = _rms_rel(df.read(), a0, n, idx)
+    print(f"\n[two-level walk, 2^22] fell back: {full} (cap 32768), {overflowed} of {n >> 10} (cap 2048); "
+          f"rms one-level {e1[0]:.3e}, two-level {e2[0]:.3e}, cap 2048 {e3[0]:.3e}")
+    assert full == 0 and overflowed > 0
+    assert e2[0] < 1.1 * e1[0] and e3[0] < 1.1 * e1[0] and e3[0] < 1.2e-3
+    assert ctx.lib.nbh_tree_set_list_cap(ctx.h, t.h, 8) != 0      # below the minimum: rejected
     t.close()
 
 
