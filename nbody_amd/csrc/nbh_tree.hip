@@ -1179,10 +1179,10 @@ __device__ inline void far_batch(const float4* fq, const int* fn, const nbh_walk
 // walk does; each body adds both expansions.  A list longer than `cap`, or a
 // stack spill in the super walk, makes the groups of that super-group walk from
 // the root instead.  The super-group's near distance is max(near_k,
-// kSuperNearMin) half-diagonals of its box: at 2^22 Plummer bodies and near_k
-// 3.5-4 the longer lists this gives cost less than the expansion terms they
-// replace, and the error drops (theta 0.65, near_k 4: walk 52.3 -> 51.3 ms, RMS
-// 1.0e-3 -> 9.8e-4).
+// kSuperNearMin) half-diagonals of its box: rejected nodes are then listed
+// higher up the tree, for each group to open with its own test.  Measured at
+// 2^22 Plummer bodies, theta 0.65, near_k 4: walk 52.7 -> 51.8 ms, RMS 1.0e-3 ->
+// 9.9e-4; at theta 0.6, near_k 3 no list overflows where near_k alone had some.
 constexpr int kSuperShift = 4;
 constexpr float kSuperNearMin = 5.0f;
 
