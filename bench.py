@@ -555,8 +555,8 @@ def main(argv=None) -> int:
                     ("quadrupole theta=0.6, near 2", 1.0 / 0.6, "quadrupole", {"bh_quad_near": 2.0}),
                     ("quadrupole theta=0.8", 1.25, "quadrupole"),
                     ("quadrupole theta=0.6, local far field", 1.0 / 0.6, "quadrupole", local),
-                    ("quadrupole theta=0.62, local far field, near 3.5", 1.0 / 0.62, "quadrupole",
-                     {**local, "bh_quad_near": 3.5}),
+                    ("quadrupole theta=0.62, local far field, near 3", 1.0 / 0.62, "quadrupole",
+                     {**local, "bh_quad_near": 3.0}),
                     ("quadrupole theta=0.65, local far field, near 4", 1.0 / 0.65, "quadrupole",
                      {**local, "bh_quad_near": 4.0}),
                     ("quadrupole theta=0.65, local far field, near 5", 1.0 / 0.65, "quadrupole",
