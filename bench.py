@@ -62,32 +62,113 @@ def cpu_baseline(y, m, n, budget_s: float, threads: int) -> dict:
                       f"{t:.1f} s on {threads} host threads"}
 
 
+def _state_y(data):
+    import numpy as np
+    pos, vel, mass, _ = data.bodies()
+    return np.ascontiguousarray(np.concatenate([pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1], vel[:, 2]])), \
+        mass
+
+
+def bh_reference_accels(y, mass, idx) -> "np.ndarray":
+    """FP64 direct-sum accelerations of the bodies idx (nbh_direct_sample: the
+    oracle's operations and source order) of the state y."""
+    import numpy as np
+
+    from nbody_amd import native
+    hip = native.hip_lib()
+    n = len(mass)
+    ctx = C.c_void_p()
+    native.check(hip.nbh_ctx_create(0, C.byref(ctx)))
+    ptrs = []
+
+    def put(a):
+        a = np.ascontiguousarray(a)
+        ptr = C.c_void_p()
+        native.check(hip.nbh_malloc(ctx, max(8, a.nbytes), C.byref(ptr)))
+        native.check(hip.nbh_memcpy_h2d(ctx, ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        ptrs.append(ptr)
+        return ptr
+
+    dy, dm, di = put(y), put(mass), put(np.asarray(idx, np.uint32))
+    out = np.empty(3 * len(idx))
+    da = put(out)
+    native.check(hip.nbh_direct_sample(ctx, dy, dm, n, di, len(idx), da))
+    native.check(hip.nbh_memcpy_d2h(ctx, out.ctypes.data_as(C.c_void_p), da, out.nbytes))
+    for ptr in ptrs:
+        native.check(hip.nbh_free(ctx, ptr))
+    native.check(hip.nbh_ctx_destroy(ctx))
+    return out.reshape(-1, 3)
+
+
+def bh_cpu_baseline(y, mass, ratio: float, budget_s: float, threads: int) -> dict:
+    """Euler steps/s of the CPU restatement of the reference's `simple_bh`
+    engine (heap_stackless layout, nested_tree targets; nbody/nbody_engine_simple_bh.cpp:16-93)
+    on the host cores: one full heap build of the state (OpenMP tasks, as
+    nbody_space_heap::build_p) plus stackless walks of a growing sample of
+    targets, extrapolated to all n targets."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    n = len(mass)
+    h = oracle_lib.Heap(n)
+    t0 = time.perf_counter()
+    h.build_omp(y, mass, ratio, threads)
+    t_build = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    k = 1024
+    while True:
+        idx = rng.choice(n, k, replace=False)
+        t0 = time.perf_counter()
+        h.accel_targets(y, mass, idx, 2, threads)
+        t_walk = time.perf_counter() - t0
+        if t_build + t_walk >= budget_s or k >= n:
+            break
+        k = min(n, int(k * max(2.0, min(16.0, (budget_s - t_build) / max(t_walk, 1e-3)))))
+    step_s = t_build + t_walk * n / k
+    return {"value": 1.0 / step_s, "unit": "euler steps/s", "cores": threads, "kind": "port",
+            "sample": f"theta={1.0 / ratio:g} heap_stackless walk, FP64: one full heap build of the N={n} state "
+                      f"({t_build:.2f} s) + {k} sampled targets ({t_walk:.2f} s), extrapolated to N targets "
+                      f"({step_s:.1f} s per force evaluation) on {threads} host threads"}
+
+
 def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kind: str = "plummer",
-             seed: int = 4, rk4_steps: int = 2) -> dict:
+             seed: int = 4, rk4_steps: int = 2, error_seeds=None) -> dict:
     """Barnes-Hut euler steps/s (one fcompute = tree build + group walk, plus one
-    fmadd_inplace, as the reference bench's step) on a Plummer sphere of n bodies,
-    with the relative force error on `samples` random bodies against FP64 direct
-    sums.  variants: (label, ratio, kind[, engine keys]) -- kind False: monopole
-    walk; "quadrupole": monopole + quadrupole walk (keys bh_quad_near,
-    bh_far_field); True: learned correction, trained first on another Plummer
-    realisation of the same size (seed 5)."""
+    fmadd_inplace, as the reference bench's step) on a Plummer sphere of n bodies
+    (seed `seed`), with the relative force error on `samples` random bodies
+    against FP64 direct sums on the initial states of the seeds `error_seeds`
+    (default: `seed` only).  variants: (label, ratio, kind[, engine keys]) --
+    kind False: monopole walk; "quadrupole": monopole + quadrupole walk (keys
+    bh_quad_near, bh_far_field); True: learned correction, trained first on
+    another Plummer realisation of the same size (seed 7)."""
     import tempfile
 
     import numpy as np
 
     from nbody_amd import correction, native, nbody_create_engine, nbody_create_solver, nbody_data
     hip = native.hip_lib()
-    data = nbody_data()
-    data.set_quiet(True)
-    log(f"BH: generating {kind} N={n}")
-    data.generate(kind, n, seed)
-    _, _, mass, _ = data.bodies()
+    error_seeds = list(error_seeds or [seed])
+    if seed not in error_seeds:
+        error_seeds.insert(0, seed)
+    states = {}
+    for sd in error_seeds:
+        log(f"BH: generating {kind} N={n} seed {sd}")
+        d = nbody_data()
+        d.set_quiet(True)
+        d.generate(kind, n, sd)
+        states[sd] = (d,) + _state_y(d)
+    data = states[seed][0]
+    mass = states[seed][2]
     desc = {"plummer": "Plummer sphere (a=1, M=1)",
-            "two_plummer": "two Plummer spheres of n/2 (a=1, M=1/2 each), centres 10 apart, approaching at 0.5"}
+            "two_plummer": "two Plummer spheres of n/2 bodies (a=1, M=1 each), centres 10 apart, approaching at 0.5"}
     out = {"workload": f"Barnes-Hut heap kd-tree, FP32 forces / FP64 state, {desc.get(kind, kind)} N={n}, "
-                       f"seed {seed}, euler dt=1e-4 (and rk4 steps/s), tree rebuilt every force evaluation",
-           "n": n, "runs": []}
+                       f"seed {seed}, euler dt=1e-4 (and rk4 steps/s), tree rebuilt every force evaluation; "
+                       f"force error on {samples} sampled bodies of the initial states of seeds {error_seeds}",
+           "n": n, "error_seeds": error_seeds, "runs": []}
     idx = np.sort(np.random.default_rng(0).choice(n, samples, replace=False)).astype(np.uint32)
+    refs = {}
+    for sd in error_seeds:
+        refs[sd] = bh_reference_accels(states[sd][1], states[sd][2], idx)
     tmpdir = tempfile.mkdtemp(prefix="nbh_corr_")
     for label, ratio, learned, *extra in variants:
         params = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
@@ -101,9 +182,8 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
         if learned:
             tdata = nbody_data()
             tdata.set_quiet(True)
-            tdata.generate("plummer", n, 5)
-            tp, tv, tm, _ = tdata.bodies()
-            ty = np.concatenate([tp[:, 0], tp[:, 1], tp[:, 2], tv[:, 0], tv[:, 1], tv[:, 2]])
+            tdata.generate("plummer", n, 7)
+            ty, tm = _state_y(tdata)
             log(f"BH: training the correction at ratio {ratio}")
             t0 = time.perf_counter()
             # far cells beyond 8 group half-diagonals with radius / distance > 0.5 go
@@ -111,7 +191,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             # the most accurate and cheapest of the thresholds tried)
             corr = correction.train(ty, tm, ratio, rho_min=0.5, near_k=8.0, max_samples=2_000_000, epochs=20)
             train_info = dict(corr.stats, seconds=time.perf_counter() - t0, rho_min=corr.rho_min,
-                              near_k=corr.near_k, train_state="Plummer seed 5")
+                              near_k=corr.near_k, train_state="Plummer seed 7")
             path = os.path.join(tmpdir, f"corr_{ratio:g}.nbhmlp")
             correction.save(path, corr)
             params["learned_correction"] = path
@@ -139,23 +219,23 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
         native.check(hip.nbh_timing_enable(ctx, 0))
         cnt = (C.c_uint64 * 5)()
         native.check(hip.nbh_bh_walk_counters(ctx, e.hip_tree(), cnt))
-        # error of the last force evaluation's state: recompute f, compare sampled bodies
-        y = e.get_y()
+        walk_counts = list(cnt)
+        # force error on the initial state of every seed (same masses: m = 1/n)
+        errs = {}
         f = e.create_buffer(48 * n)
-        e.fcompute(0.0, y, f)
-        fa = e.read_buffer(f)[3 * n:].reshape(3, n).T[idx.astype(np.int64)]
-        dm = e.create_buffer(8 * n)
-        e.write_buffer(dm, mass)
-        di = e.create_buffer(8 * ((samples + 1) // 2))
-        hbuf = np.zeros(2 * ((samples + 1) // 2), dtype=np.uint32)
-        hbuf[:samples] = idx
-        e.write_buffer(di, hbuf.view(np.float64))
-        da = e.create_buffer(24 * samples)
-        native.check(hip.nbh_direct_sample(ctx, y.device_ptr(), dm.device_ptr(), n, di.device_ptr(), samples,
-                                           da.device_ptr()))
-        a0 = e.read_buffer(da).reshape(-1, 3)
-        rel = np.linalg.norm(fa - a0, axis=1) / np.linalg.norm(a0, axis=1)
-        e.free_buffers([f, dm, di, da])
+        yb = e.create_buffer(48 * n)
+        spills = walk_counts[2]
+        for sd in error_seeds:
+            e.write_buffer(yb, states[sd][1])
+            e.fcompute(0.0, yb, f)
+            native.check(hip.nbh_bh_walk_counters(ctx, e.hip_tree(), cnt))
+            spills = max(spills, cnt[2])
+            fa = e.read_buffer(f)[3 * n:].reshape(3, n).T[idx.astype(np.int64)]
+            a0 = refs[sd]
+            rel = np.linalg.norm(fa - a0, axis=1) / np.linalg.norm(a0, axis=1)
+            errs[sd] = {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
+                        "max": float(rel.max()), "samples": samples}
+        e.free_buffers([f, yb])
         rk4_rate = None
         if rk4_steps > 0:
             s4 = nbody_create_solver({"solver": "rk4", "max_step": 1e-4})
@@ -168,6 +248,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             native.check(hip.nbh_ctx_synchronize(ctx))
             rk4_rate = rk4_steps / (time.perf_counter() - t0)
             s4.destroy()
+        walk_ms = tw.value / max(1, cw.value)
         run = {
             "variant": label, "ratio": ratio, "theta": 1.0 / ratio,
             "multipole": params.get("bh_multipole", "monopole"),
@@ -175,50 +256,64 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             "far_field": params.get("bh_far_field", "per_body") if "bh_multipole" in params else None,
             "steps_per_s": steps / wall,
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
-            "walk_ms": tw.value / max(1, cw.value), "interactions_per_body": cnt[0] / n,
-            "node_visits_per_group": cnt[1] / max(1, n // 64),
-            # node records read by the walk: 32 B (cm, crit_r2) + 8 B mass per visit
-            "node_GB_per_s": 40.0 * cnt[1] / (tw.value / max(1, cw.value) * 1e-3) / 1e9,
+            "walk_ms": walk_ms, "interactions_per_body": walk_counts[0] / n,
+            "node_visits_per_group": walk_counts[1] / max(1, n // 64),
+            # node records read by the group walks: one 64-byte record per visit
+            "node_GB_per_s": 64.0 * walk_counts[1] / (walk_ms * 1e-3) / 1e9,
+            # nodes taken as monopoles because a walk stack was full, plus the
+            # two-level walk's super-groups whose list overflowed (they walk in
+            # one level); the largest count over the timed and the error runs
+            "spills_or_fallbacks": int(spills),
             "rk4_steps_per_s": rk4_rate,
-            "force_rel_err": {"rms": float(np.sqrt(np.mean(rel ** 2))), "p99": float(np.percentile(rel, 99)),
-                              "max": float(rel.max()), "samples": samples},
+            "force_rel_err": errs[seed],
+            "force_rel_err_by_seed": {str(k): v for k, v in errs.items()},
+            "rms_max_over_seeds": max(v["rms"] for v in errs.values()),
+            "p99_max_over_seeds": max(v["p99"] for v in errs.values()),
         }
         if learned or "bh_multipole" in params:
-            run["far_cells_per_group"] = cnt[3] / (n // 64)
-            run["near_cells_per_group"] = cnt[4] / (n // 64)
+            run["far_cells_per_group"] = walk_counts[3] / (n // 64)
+            run["near_cells_per_group"] = walk_counts[4] / (n // 64)
         if learned:
             run["training"] = train_info
         out["runs"].append(run)
-        log(f"BH {label}: {steps / wall:.2f} steps/s, rms err {run['force_rel_err']['rms']:.2e}")
+        log(f"BH {label}: {steps / wall:.2f} steps/s, rms err {run['force_rel_err']['rms']:.2e} "
+            f"(max over seeds {run['rms_max_over_seeds']:.2e}), spills/fallbacks {spills}")
         s.destroy()
         e.destroy()
     for name in os.listdir(tmpdir):
         os.remove(os.path.join(tmpdir, name))
     os.rmdir(tmpdir)
-    ok = [r for r in out["runs"] if r["force_rel_err"]["rms"] <= 1e-3]
+    ok = [r for r in out["runs"] if r["rms_max_over_seeds"] <= 1e-3]
     if ok:
         best = max(ok, key=lambda r: r["steps_per_s"])
         out["best_at_rms_1e-3"] = {"variant": best["variant"], "steps_per_s": best["steps_per_s"],
-                                   "rms": best["force_rel_err"]["rms"]}
+                                   "rms": best["force_rel_err"]["rms"],
+                                   "rms_max_over_seeds": best["rms_max_over_seeds"],
+                                   "walk_ms": best["walk_ms"], "tree_build_ms": best["tree_build_ms"]}
+        base = [r for r in out["runs"] if r["variant"] == "plain theta=0.5"]
+        if base:
+            out["best_at_rms_1e-3"]["speedup_vs_plain_theta_0.5"] = best["steps_per_s"] / base[0]["steps_per_s"]
     # equal-error comparison: for each plain monopole run, the fastest run of another
-    # kind whose RMS and 99th-percentile errors are both no larger (the RMS over the
-    # samples alone can swing with one body near a force cancellation), and its speed
-    # relative to the plain run
+    # kind whose RMS and 99th-percentile errors are both no larger on every seed
+    # (the RMS over the samples alone can swing with one body near a force
+    # cancellation), and its speed relative to the plain run
     def plain(r):
         return r["multipole"] == "monopole" and not r["variant"].startswith("learned")
     eq = []
     for p in out["runs"]:
         if not plain(p):
             continue
-        pe = p["force_rel_err"]
-        cand = [r for r in out["runs"] if not plain(r) and r["force_rel_err"]["rms"] <= pe["rms"]
-                and r["force_rel_err"]["p99"] <= pe["p99"]]
+        cand = [r for r in out["runs"] if not plain(r)
+                and all(r["force_rel_err_by_seed"][k]["rms"] <= p["force_rel_err_by_seed"][k]["rms"]
+                        and r["force_rel_err_by_seed"][k]["p99"] <= p["force_rel_err_by_seed"][k]["p99"]
+                        for k in p["force_rel_err_by_seed"])]
         if cand:
             b = max(cand, key=lambda r: r["steps_per_s"])
-            eq.append({"plain": p["variant"], "plain_rms": pe["rms"], "plain_p99": pe["p99"],
+            eq.append({"plain": p["variant"], "plain_rms": p["force_rel_err"]["rms"],
+                       "plain_p99": p["force_rel_err"]["p99"], "plain_rms_max_over_seeds": p["rms_max_over_seeds"],
                        "plain_steps_per_s": p["steps_per_s"], "fastest": b["variant"],
                        "rms": b["force_rel_err"]["rms"], "p99": b["force_rel_err"]["p99"],
-                       "steps_per_s": b["steps_per_s"],
+                       "rms_max_over_seeds": b["rms_max_over_seeds"], "steps_per_s": b["steps_per_s"],
                        "speedup": b["steps_per_s"] / p["steps_per_s"]})
     if eq:
         out["at_equal_error"] = eq
@@ -365,20 +460,10 @@ def pmc_probe(n: int) -> int:
         native.check(hip.nbh_fcompute_direct(ctx, dy, dm, df, n, 1, 0))
     native.check(hip.nbh_ctx_synchronize(ctx))
     log("direct launches done")
-    m = 1 << 26
-    a, b, c = dev(8 * m), dev(8 * m), dev(8 * m)
-    native.check(hip.nbh_fill(ctx, b, 1.0, m))
-    native.check(hip.nbh_fill(ctx, c, 2.0, m))
-    srcs = (C.c_void_p * 1)(c.value)
-    coef = (C.c_double * 1)(1.0)
-    for _ in range(3):
-        native.check(hip.nbh_fmaddn(ctx, a, b, srcs, coef, 1, m))
-    native.check(hip.nbh_ctx_synchronize(ctx))
-    log("copy launches done")
     for ptr in bufs:
         native.check(hip.nbh_free(ctx, ptr))
     native.check(hip.nbh_ctx_destroy(ctx))
-    return 0
+    return copy_probe()
 
 
 def traffic_from_profile(n: int):
@@ -392,6 +477,94 @@ def traffic_from_profile(n: int):
         if line.strip().startswith(key):
             return float(line.split(":", 1)[1].strip())
     return None
+
+
+def bh_variants(learned: bool = True):
+    """Barnes-Hut variants of the 4M Plummer block: (label, ratio, kind[, keys])."""
+    local = {"bh_far_field": "local"}
+    v = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False),
+         ("plain theta=0.25", 4.0, False),
+         ("quadrupole theta=0.6, all cells per body", 1.0 / 0.6, "quadrupole", {"bh_quad_near": "inf"}),
+         ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
+         ("quadrupole theta=0.8", 1.25, "quadrupole"),
+         ("quadrupole theta=0.6, local far field", 1.0 / 0.6, "quadrupole", local),
+         ("quadrupole theta=0.62, local far field, near 3", 1.0 / 0.62, "quadrupole",
+          {**local, "bh_quad_near": 3.0}),
+         ("quadrupole theta=0.65, local far field, near 4", 1.0 / 0.65, "quadrupole",
+          {**local, "bh_quad_near": 4.0}),
+         ("quadrupole theta=0.65, local far field, near 5", 1.0 / 0.65, "quadrupole",
+          {**local, "bh_quad_near": 5.0}),
+         ("quadrupole theta=0.7, local far field, near 4", 1.0 / 0.7, "quadrupole",
+          {**local, "bh_quad_near": 4.0}),
+         ("quadrupole theta=0.8, local far field, near 5", 1.25, "quadrupole",
+          {**local, "bh_quad_near": 5.0}),
+         ("quadrupole theta=0.9, local far field, near 4", 1.0 / 0.9, "quadrupole",
+          {**local, "bh_quad_near": 4.0})]
+    if learned:
+        v.append(("learned theta=0.8", 1.25, True))
+    return v
+
+
+def bh_probe(n: int, label: str) -> int:
+    """Launches for the hardware-counter runs of one Barnes-Hut variant: three
+    force evaluations (tree build + walk) of the 4M Plummer state (seed 4), then
+    the calibration copy of pmc_probe."""
+    from nbody_amd import native, nbody_create_engine, nbody_data
+    hip = native.hip_lib()
+    v = [x for x in bh_variants(False) if x[0] == label]
+    if not v:
+        log(f"unknown variant {label!r}; known: {[x[0] for x in bh_variants(False)]}")
+        return 2
+    lab, ratio, kind, *extra = v[0]
+    params = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group",
+              "distance_to_node_radius_ratio": ratio, "tree_layout": "heap_stackless"}
+    if kind == "quadrupole":
+        params["bh_multipole"] = "quadrupole"
+    if extra:
+        params.update(extra[0])
+    data = nbody_data()
+    data.set_quiet(True)
+    data.generate("plummer", n, 4)
+    e = nbody_create_engine(params)
+    if e is None or not e.init(data):
+        raise RuntimeError("hip_bh engine init failed")
+    f = e.create_buffer(48 * n)
+    for _ in range(3):
+        e.fcompute(0.0, e.get_y(), f)
+    native.check(hip.nbh_ctx_synchronize(e.hip_context()))
+    e.free_buffer(f)
+    e.destroy()
+    log(f"BH launches done: {lab}")
+    return copy_probe()
+
+
+def copy_probe() -> int:
+    """A plain streaming kernel of known traffic (nbh_fmaddn a = b + c over 2^26
+    doubles: 1 GiB read, 512 MiB written), three times, to calibrate
+    FETCH_SIZE / WRITE_SIZE."""
+    from nbody_amd import native
+    hip = native.hip_lib()
+    ctx = C.c_void_p()
+    native.check(hip.nbh_ctx_create(0, C.byref(ctx)))
+    m = 1 << 26
+    bufs = []
+    for _ in range(3):
+        ptr = C.c_void_p()
+        native.check(hip.nbh_malloc(ctx, 8 * m, C.byref(ptr)))
+        bufs.append(ptr)
+    a, b, c = bufs
+    native.check(hip.nbh_fill(ctx, b, 1.0, m))
+    native.check(hip.nbh_fill(ctx, c, 2.0, m))
+    srcs = (C.c_void_p * 1)(c.value)
+    coef = (C.c_double * 1)(1.0)
+    for _ in range(3):
+        native.check(hip.nbh_fmaddn(ctx, a, b, srcs, coef, 1, m))
+    native.check(hip.nbh_ctx_synchronize(ctx))
+    log("copy launches done")
+    for ptr in bufs:
+        native.check(hip.nbh_free(ctx, ptr))
+    native.check(hip.nbh_ctx_destroy(ctx))
+    return 0
 
 
 def main(argv=None) -> int:
@@ -415,6 +588,9 @@ def main(argv=None) -> int:
                     help="also run the cold-cube split-precision workload (N=65536) for STEPS steps")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch the kernels measured by the hardware-counter runs, then exit")
+    ap.add_argument("--bh-probe", metavar="VARIANT",
+                    help="only launch three force evaluations of this Barnes-Hut variant at --bh-n (for the "
+                         "hardware-counter runs) and the calibration copy, then exit")
     a = ap.parse_args(argv)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -432,6 +608,8 @@ def main(argv=None) -> int:
         return 2
     if a.pmc_probe:
         return pmc_probe(a.n)
+    if a.bh_probe:
+        return bh_probe(a.bh_n, a.bh_probe)
     hip = native.hip_lib()
     n = a.n
     log(f"generating Plummer N={n}")
@@ -547,29 +725,18 @@ def main(argv=None) -> int:
         log(f"cpu baseline ({threads} threads, ~{a.cpu_seconds:g} s)")
         out["cpu_baseline"] = cpu_baseline(yh, mass, n, a.cpu_seconds, threads)
     if not a.no_bh:
-        local = {"bh_far_field": "local"}
-        variants = [("plain theta=0.5", 2.0, False), ("plain theta=0.8", 1.25, False),
-                    ("plain theta=0.25", 4.0, False),
-                    ("quadrupole theta=0.6, all cells per body", 1.0 / 0.6, "quadrupole", {"bh_quad_near": "inf"}),
-                    ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
-                    ("quadrupole theta=0.6, near 2", 1.0 / 0.6, "quadrupole", {"bh_quad_near": 2.0}),
-                    ("quadrupole theta=0.8", 1.25, "quadrupole"),
-                    ("quadrupole theta=0.6, local far field", 1.0 / 0.6, "quadrupole", local),
-                    ("quadrupole theta=0.62, local far field, near 3", 1.0 / 0.62, "quadrupole",
-                     {**local, "bh_quad_near": 3.0}),
-                    ("quadrupole theta=0.65, local far field, near 4", 1.0 / 0.65, "quadrupole",
-                     {**local, "bh_quad_near": 4.0}),
-                    ("quadrupole theta=0.65, local far field, near 5", 1.0 / 0.65, "quadrupole",
-                     {**local, "bh_quad_near": 5.0}),
-                    ("quadrupole theta=0.7, local far field, near 4", 1.0 / 0.7, "quadrupole",
-                     {**local, "bh_quad_near": 4.0}),
-                    ("quadrupole theta=0.8, local far field, near 5", 1.25, "quadrupole",
-                     {**local, "bh_quad_near": 5.0}),
-                    ("quadrupole theta=0.9, local far field, near 4", 1.0 / 0.9, "quadrupole",
-                     {**local, "bh_quad_near": 4.0})]
-        if not a.no_learned:
-            variants.append(("learned theta=0.8", 1.25, True))
-        out["barnes_hut"] = bh_bench(a.bh_n, a.bh_steps, 1, variants)
+        variants = bh_variants(not a.no_learned)
+        bh = bh_bench(a.bh_n, a.bh_steps, 1, variants, error_seeds=[4, 5, 6])
+        if not a.no_cpu_baseline:
+            threads = min(16, os.cpu_count() or 1)
+            log(f"BH cpu baseline ({threads} threads, ~{a.cpu_seconds:g} s)")
+            from nbody_amd import nbody_data as _nd
+            d4 = _nd()
+            d4.set_quiet(True)
+            d4.generate("plummer", a.bh_n, 4)
+            y4, m4 = _state_y(d4)
+            bh["cpu_baseline"] = bh_cpu_baseline(y4, m4, 2.0, a.cpu_seconds, threads)
+        out["barnes_hut"] = bh
     if not a.no_bh and not a.no_two_plummer:
         out["two_plummer"] = bh_bench(1 << 20, 3, 1, [("monopole theta=0.5, group walk", 2.0, False),
                                                       ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),

@@ -171,14 +171,22 @@ nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const
    node accepted for the super-group's box is accepted for every group box
    inside it, so the groups keep the acceptance rule.  Lists longer than 32768
    entries fall back to the one-level walk for that super-group (counted in
-   walk counter out[2]).  The first call allocates n * 128 bytes of lists.
+   walk counter out[2]).  The lists (n * 128 bytes at the default capacity) are
+   allocated by nbh_tree_set_list_cap, or by the first call when that was not
+   called; a failed allocation leaves none of them allocated.
    Not in the reference. */
 nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* tree, const double* y, double* f, float near_k);
 /* List capacity per super-group of nbh_bh_fcompute_group_local2 (default 32768
    entries; lists take n / 1024 * cap * 4 bytes).  Super-groups with longer lists
-   walk in one level.  Frees lists already allocated; cap in [64, 2^24].  No
-   counterpart in the reference: it bounds the two-level walk's memory. */
+   walk in one level.  Frees the lists already allocated and allocates them at
+   the new capacity (all of them or, on failure, none: NBH_ERR_NOMEM); cap in
+   [64, 2^24].  No counterpart in the reference: it bounds the two-level walk's
+   memory and lets a caller allocate it outside the force evaluation. */
 nbh_status nbh_tree_set_list_cap(nbh_ctx* ctx, nbh_tree* tree, int cap);
+/* Device bytes held by the tree: its build buffers and node records, plus the
+   two-level walk's lists when they are allocated.  No counterpart in the
+   reference (memory accounting). */
+nbh_status nbh_tree_bytes(nbh_ctx* ctx, const nbh_tree* tree, size_t* bytes);
 /* Copy the tree to host arrays of tree size 2n (index 0 unused); any pointer may
    be NULL.  cm/bmin/bmax are xyz-interleaved [2n*3]; body_n holds -1 on
    internal nodes.  Used to check the layout against the reference heap

@@ -2,7 +2,6 @@
 
     nbody_amd/libnbody_hip.so   gfx950 kernels + C ABI (include/nbody_hip.h), hipcc
     nbody_amd/libnbody.so       host layer: engines, solvers, data (include/nbody_host.h), g++
-    nbody_amd/probe/valu_issue  FP32 instruction issue-rate probe (executable), hipcc
     oracle/liboracle.so         CPU restatement used by the tests as the checker, gcc
     oracle/liboracle_engine.so  the restatement behind the nbody_engine interface (tests only)
 
@@ -93,11 +92,6 @@ def build(jobs: int = 8, oracle: bool = True, verbose: bool = False) -> dict:
           f"-L{PKG}", "-lnbody_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(str(lib_host) + ".tmp", lib_host)
     out = {"libnbody_hip": lib_hip, "libnbody": lib_host}
-    # issue-rate probe behind the force kernels' roofline figures
-    probe = PKG / "probe" / "valu_issue"
-    _run([HIPCC, *HIP_FLAGS, "-o", str(probe) + ".tmp", str(PKG / "probe" / "valu_issue.hip")])
-    os.replace(str(probe) + ".tmp", probe)
-    out["valu_issue"] = probe
     if oracle:
         lib_or = ORACLE / "liboracle.so"
         _run(["gcc", "-shared", "-fopenmp", "-o", str(lib_or) + ".tmp", str(BUILD / "nbody_oracle.o"), "-lm"])

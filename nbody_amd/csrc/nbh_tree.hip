@@ -22,6 +22,7 @@
 // coherent), nodes as 32-byte AoS records {cm.xyz, crit_r2} plus mass.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "nbh_common.h"
@@ -2065,6 +2066,48 @@ nbh_status sort_pairs(nbh_ctx* ctx, uint64_t* k0, uint32_t* v0, uint64_t* k1, ui
 
 extern "C" {
 
+// The buffers a tree allocates at creation, with their sizes in bytes.
+constexpr int kTreeBuffers = 17;
+static void tree_buffers(nbh_tree* t, void** ptrs[kTreeBuffers], size_t sizes[kTreeBuffers])
+{
+	const size_t n = t->n;
+	const size_t nb = (n + kSortTile - 1) / kSortTile;
+	const size_t ncounts = std::max(nb * 256, (n + kPartTile - 1) / kPartTile) + 1;
+	int k = 0;
+	auto add = [&](void* p, size_t bytes) {
+		ptrs[k] = static_cast<void**>(p);
+		sizes[k] = bytes;
+		++k;
+	};
+	for(int a = 0; a < 3; ++a) { add(&t->list[a], n * sizeof(uint32_t)); }
+	add(&t->tmp, n * sizeof(uint32_t));
+	add(&t->side, n);
+	add(&t->keys, 2 * n * sizeof(uint64_t));
+	add(&t->vals, 2 * n * sizeof(uint32_t));
+	add(&t->counts, ncounts * sizeof(uint32_t));
+	add(&t->scan_part, kScanParts * sizeof(uint32_t));
+	add(&t->xyzr, 2 * n * sizeof(double4));
+	add(&t->mass, 2 * n * sizeof(double));
+	add(&t->bmin, 2 * n * 3 * sizeof(double));
+	add(&t->bmax, 2 * n * 3 * sizeof(double));
+	add(&t->quad, 2 * n * 6 * sizeof(double));
+	add(&t->wrec, 2 * n * sizeof(nbh_walk_rec));
+	add(&t->body_n, 2 * n * sizeof(int32_t));
+	add(&t->counters, 5 * sizeof(unsigned long long));
+}
+
+static size_t tree_base_bytes(size_t n)
+{
+	nbh_tree probe;
+	probe.n = n;
+	void** ptrs[kTreeBuffers];
+	size_t sizes[kTreeBuffers];
+	tree_buffers(&probe, ptrs, sizes);
+	size_t total = 0;
+	for(size_t b : sizes) { total += b; }
+	return total;
+}
+
 nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 {
 	NBH_TRY(nbh::enter(ctx));
@@ -2076,27 +2119,14 @@ nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 	t->ctx = ctx;
 	t->n = n;
 	while((static_cast<size_t>(1) << t->levels) < n) { ++t->levels; }
-	size_t nb = (n + kSortTile - 1) / kSortTile;
-	size_t ncounts = std::max(nb * 256, (n + kPartTile - 1) / kPartTile) + 1;
+	void** ptrs[kTreeBuffers];
+	size_t sizes[kTreeBuffers];
+	tree_buffers(t, ptrs, sizes);
 	hipError_t e = hipSuccess;
-	auto alloc = [&](void** p, size_t bytes) {
-		if(e == hipSuccess) { e = hipMalloc(p, bytes); }
-	};
-	for(int a = 0; a < 3; ++a) { alloc(reinterpret_cast<void**>(&t->list[a]), n * sizeof(uint32_t)); }
-	alloc(reinterpret_cast<void**>(&t->tmp), n * sizeof(uint32_t));
-	alloc(reinterpret_cast<void**>(&t->side), n);
-	alloc(reinterpret_cast<void**>(&t->keys), 2 * n * sizeof(uint64_t));
-	alloc(reinterpret_cast<void**>(&t->vals), 2 * n * sizeof(uint32_t));
-	alloc(reinterpret_cast<void**>(&t->counts), ncounts * sizeof(uint32_t));
-	alloc(reinterpret_cast<void**>(&t->scan_part), kScanParts * sizeof(uint32_t));
-	alloc(reinterpret_cast<void**>(&t->xyzr), 2 * n * sizeof(double4));
-	alloc(reinterpret_cast<void**>(&t->mass), 2 * n * sizeof(double));
-	alloc(reinterpret_cast<void**>(&t->bmin), 2 * n * 3 * sizeof(double));
-	alloc(reinterpret_cast<void**>(&t->bmax), 2 * n * 3 * sizeof(double));
-	alloc(reinterpret_cast<void**>(&t->quad), 2 * n * 6 * sizeof(double));
-	alloc(reinterpret_cast<void**>(&t->wrec), 2 * n * sizeof(nbh_walk_rec));
-	alloc(reinterpret_cast<void**>(&t->body_n), 2 * n * sizeof(int32_t));
-	alloc(reinterpret_cast<void**>(&t->counters), 5 * sizeof(unsigned long long));
+	for(int k = 0; k < kTreeBuffers && e == hipSuccess; ++k)
+	{
+		e = hipMalloc(ptrs[k], sizes[k]);
+	}
 	if(e == hipSuccess) { e = hipMemsetAsync(t->xyzr, 0, 2 * n * sizeof(double4), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->mass, 0, 2 * n * sizeof(double), ctx->stream); }
 	if(e == hipSuccess) { e = hipMemsetAsync(t->quad, 0, 2 * n * 6 * sizeof(double), ctx->stream); }
@@ -2285,6 +2315,78 @@ nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* t, const do
 // List capacity per super-group of the two-level walk (4 bytes per entry).
 constexpr int kTwoCap = 32768;
 
+// Bytes of the two-level walk's buffers: per super-group a list of `cap` node
+// indices, its length and the 34 coefficients of its expansion.
+static size_t two_bytes(size_t n, int cap)
+{
+	const size_t supers = n >> (6 + kSuperShift);
+	return supers * (static_cast<size_t>(cap) * sizeof(int) + sizeof(int) + 34 * sizeof(float));
+}
+
+// Frees the two-level walk's buffers.  Every pointer is null afterwards, also
+// when a hipFree fails (the first failure is reported).
+static nbh_status two_free(nbh_ctx* ctx, nbh_tree* t)
+{
+	if(t->two_lists == nullptr && t->two_count == nullptr && t->two_coef == nullptr)
+	{
+		return NBH_OK;
+	}
+	hipError_t e = hipStreamSynchronize(ctx->stream);
+	void* ptrs[3] = { t->two_lists, t->two_count, t->two_coef };
+	t->two_lists = nullptr;
+	t->two_count = nullptr;
+	t->two_coef = nullptr;
+	for(void* p : ptrs)
+	{
+		if(p != nullptr)
+		{
+			hipError_t ef = hipFree(p);
+			if(e == hipSuccess) { e = ef; }
+		}
+	}
+	if(e != hipSuccess)
+	{
+		return nbh::hip_fail(e, "freeing the two-level walk lists", __FILE__, __LINE__);
+	}
+	return NBH_OK;
+}
+
+// Allocates the two-level walk's buffers all together or not at all: on any
+// failure the ones already allocated are freed and every pointer stays null,
+// so a later call allocates afresh instead of launching with a missing buffer.
+// NBH_TEST_FAIL_ALLOC=k (k = 1, 2, 3) in the environment makes the k-th of the
+// three allocations fail as out of memory (a test hook for this path).
+static nbh_status two_alloc(nbh_ctx* ctx, nbh_tree* t, int cap)
+{
+	NBH_TRY(two_free(ctx, t));
+	const size_t supers = t->n >> (6 + kSuperShift);
+	const size_t sizes[3] = { supers * static_cast<size_t>(cap) * sizeof(int), supers * sizeof(int),
+	                          supers * 34 * sizeof(float) };
+	int fail_at = 0;
+	if(const char* env = getenv("NBH_TEST_FAIL_ALLOC"))
+	{
+		fail_at = atoi(env);
+	}
+	void* p[3] = { nullptr, nullptr, nullptr };
+	hipError_t e = hipSuccess;
+	for(int k = 0; k < 3 && e == hipSuccess; ++k)
+	{
+		e = (k + 1 == fail_at) ? hipErrorOutOfMemory : hipMalloc(&p[k], sizes[k]);
+	}
+	if(e != hipSuccess)
+	{
+		for(void* q : p)
+		{
+			if(q != nullptr) { (void)hipFree(q); }
+		}
+		return nbh::hip_fail(e, "two-level walk list allocation", __FILE__, __LINE__);
+	}
+	t->two_lists = static_cast<int*>(p[0]);
+	t->two_count = static_cast<int*>(p[1]);
+	t->two_coef = static_cast<float*>(p[2]);
+	return NBH_OK;
+}
+
 nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* t, const double* y, double* f, float near_k)
 {
 	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1 (the expansion needs the cell outside the group)");
@@ -2299,11 +2401,9 @@ nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* t, const double*
 	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
 	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
 	const int cap = t->two_cap > 0 ? t->two_cap : kTwoCap;
-	if(t->two_lists == nullptr)
+	if(t->two_lists == nullptr || t->two_count == nullptr || t->two_coef == nullptr)
 	{
-		NBH_HIP(hipMalloc(reinterpret_cast<void**>(&t->two_lists), supers * static_cast<size_t>(cap) * sizeof(int)));
-		NBH_HIP(hipMalloc(reinterpret_cast<void**>(&t->two_count), supers * sizeof(int)));
-		NBH_HIP(hipMalloc(reinterpret_cast<void**>(&t->two_coef), supers * 34 * sizeof(float)));
+		NBH_TRY(two_alloc(ctx, t, cap));
 	}
 	TwoLevel two;
 	two.lists = t->two_lists;
@@ -2329,17 +2429,21 @@ nbh_status nbh_tree_set_list_cap(nbh_ctx* ctx, nbh_tree* t, int cap)
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(cap >= 64 && cap <= (1 << 24), "list capacity %d: must be in [64, 2^24]", cap);
-	if(t->two_lists != nullptr)
-	{
-		NBH_HIP(hipStreamSynchronize(ctx->stream));
-		NBH_HIP(hipFree(t->two_lists));
-		t->two_lists = nullptr;
-		NBH_HIP(hipFree(t->two_count));
-		t->two_count = nullptr;
-		NBH_HIP(hipFree(t->two_coef));
-		t->two_coef = nullptr;
-	}
+	NBH_TRY(two_free(ctx, t));
 	t->two_cap = cap;
+	if((t->n >> (6 + kSuperShift)) < 1)
+	{
+		return NBH_OK;   // no super-group: the walk runs in one level and needs no lists
+	}
+	return two_alloc(ctx, t, cap);
+}
+
+nbh_status nbh_tree_bytes(nbh_ctx* ctx, const nbh_tree* t, size_t* bytes)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(bytes != nullptr, "null output");
+	*bytes = tree_base_bytes(t->n) + (t->two_lists != nullptr ? two_bytes(t->n, t->two_cap > 0 ? t->two_cap : kTwoCap) : 0);
 	return NBH_OK;
 }
 

@@ -730,6 +730,13 @@ bool nbody_engine_hip_bh::init(nbody_data* data)
 	{
 		return false;
 	}
+	// the two-level walk's lists (default capacity, 32768 entries per 1024 bodies)
+	// are allocated here, not inside the first force evaluation
+	if(m_group_walk && m_quadrupole && m_local_far && m_precision == NBH_FP32 &&
+	   !check(nbh_tree_set_list_cap(m_ctx, m_tree, 32768), "two-level walk list allocation"))
+	{
+		return false;
+	}
 	if(!m_net_params.empty() && m_net == nullptr)
 	{
 		if(!check(nbh_mlp_create(m_ctx, 0, &m_net), "correction network allocation") ||

@@ -60,6 +60,7 @@ HIP_API = {
     "nbh_bh_fcompute_group_local": (i32, [vp, vp, vp, vp, C.c_float]),
     "nbh_bh_fcompute_group_local2": (i32, [vp, vp, vp, vp, C.c_float]),
     "nbh_tree_set_list_cap": (i32, [vp, vp, C.c_int]),
+    "nbh_tree_bytes": (i32, [vp, vp, C.POINTER(sz)]),
     "nbh_tree_get": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "nbh_bh_counters": (i32, [vp, vp, C.POINTER(u64), C.POINTER(u64)]),
     "nbh_bh_walk_counters": (i32, [vp, vp, C.POINTER(u64)]),

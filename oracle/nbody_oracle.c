@@ -449,6 +449,82 @@ static void heap_build_p(oracle_heap* h, size_t count, size_t* ind, const double
 	heap_update(h, idx, oracle_heap_left(idx), oracle_heap_right(idx));
 }
 
+/* heap_build_p with the two halves of every segment above 64 bodies as OpenMP
+   tasks, as nbody/nbody_space_heap.cpp:175-183 does; the subtrees are disjoint,
+   so the result equals the serial build's.  Used to time the CPU baseline. */
+static void heap_build_task(oracle_heap* h, size_t count, size_t* ind, const double* r[3],
+                            const double* mass, size_t idx, int dim)
+{
+	if(count <= 64)
+	{
+		heap_build_p(h, count, ind, r, mass, idx, dim);
+		return;
+	}
+	size_t left_size = count / 2;
+	select_k(ind, count, left_size, r[dim]);
+	int next = (dim + 1) % 3;
+	#pragma omp task
+	heap_build_task(h, left_size, ind, r, mass, oracle_heap_left(idx), next);
+	#pragma omp task
+	heap_build_task(h, count - left_size, ind + left_size, r, mass, oracle_heap_right(idx), next);
+	#pragma omp taskwait
+	heap_update(h, idx, oracle_heap_left(idx), oracle_heap_right(idx));
+}
+
+int oracle_heap_build_omp(oracle_heap* h, const double* rx, const double* ry, const double* rz,
+                          const double* mass, double ratio, int threads)
+{
+	size_t n = h->count;
+	size_t* ind = (size_t*)malloc(n * sizeof(size_t));
+	if(ind == NULL)
+	{
+		return -1;
+	}
+	for(size_t i = 0; i < n; ++i)
+	{
+		ind[i] = i;
+	}
+	for(size_t i = 0; i < h->size; ++i)
+	{
+		h->body_n[i] = ORACLE_NO_BODY;
+	}
+	h->ratio_sqr = ratio * ratio;
+	const double* r[3] = { rx, ry, rz };
+	#pragma omp parallel num_threads(threads > 0 ? threads : omp_get_max_threads())
+	#pragma omp single
+	heap_build_task(h, n, ind, r, mass, ORACLE_HEAP_ROOT, 0);
+	free(ind);
+	return 0;
+}
+
+/* The Barnes-Hut accelerations (force / m) of the bodies idx[0..m) with the
+   layout's walk (1 = heap stack walk, 2 = stackless), OpenMP over targets:
+   the per-target work of oracle_bh_fcompute on a sample. */
+void oracle_bh_accel_targets(const oracle_heap* h, int layout, size_t n, const double* y, const double* mass,
+                             const size_t* idx, size_t m, double* out, int threads)
+{
+	long long mm = (long long)m;
+	#pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : omp_get_max_threads())
+	for(long long t = 0; t < mm; ++t)
+	{
+		size_t b = idx[t];
+		double v1[3] = { y[b], y[n + b], y[2 * n + b] };
+		double tot[3];
+		if(layout == 2)
+		{
+			oracle_heap_traverse_stackless(h, v1, mass[b], tot);
+		}
+		else
+		{
+			oracle_heap_traverse(h, v1, mass[b], tot);
+		}
+		for(int d = 0; d < 3; ++d)
+		{
+			out[3 * t + d] = tot[d] / mass[b];
+		}
+	}
+}
+
 /* nbody/nbody_space_heap.cpp:13-37 */
 int oracle_heap_build(oracle_heap* h, const double* rx, const double* ry, const double* rz,
                       const double* mass, double ratio)

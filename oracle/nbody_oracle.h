@@ -97,6 +97,10 @@ void oracle_heap_free(oracle_heap* h);
 /* nbody_space_heap.cpp:13-37 + build_p :136-191 + update :119-134 */
 int  oracle_heap_build(oracle_heap* h, const double* rx, const double* ry, const double* rz,
                        const double* mass, double ratio);
+/* The same build with OpenMP tasks for the segments above 64 bodies
+   (nbody/nbody_space_heap.cpp:175-183); identical result.  For timing. */
+int  oracle_heap_build_omp(oracle_heap* h, const double* rx, const double* ry, const double* rz,
+                           const double* mass, double ratio, int threads);
 /* nbody_space_heap.cpp:39-62 (refit with the existing body_n) */
 void oracle_heap_rebuild(oracle_heap* h, const double* rx, const double* ry, const double* rz,
                          double ratio);
@@ -111,6 +115,12 @@ void oracle_heap_traverse_stackless(const oracle_heap* h, const double v1[3], do
    The tree must already be built / refitted for y. */
 void oracle_bh_fcompute(const oracle_heap* h, int layout, int traverse, size_t n,
                         const double* y, const double* mass, double* f, int threads);
+
+/* Accelerations of the bodies idx[0..m) by the walk of `layout` (the
+   per-target work of oracle_bh_fcompute), out[3m]; for sampled checks and the
+   CPU baseline. */
+void oracle_bh_accel_targets(const oracle_heap* h, int layout, size_t n, const double* y, const double* mass,
+                             const size_t* idx, size_t m, double* out, int threads);
 
 /* ---- conservation checks (nbody/nbody_data.cpp:57-145, summation.h:19-45,
         summation_proxy.h:6-87) ----

@@ -58,6 +58,9 @@ def lib():
             "oracle_heap_free": (None, [C.POINTER(heap_t)]),
             "oracle_heap_build": (C.c_int, [C.POINTER(heap_t), PD, PD, PD, PD, C.c_double]),
             "oracle_heap_rebuild": (None, [C.POINTER(heap_t), PD, PD, PD, C.c_double]),
+            "oracle_heap_build_omp": (C.c_int, [C.POINTER(heap_t), PD, PD, PD, PD, C.c_double, C.c_int]),
+            "oracle_bh_accel_targets": (None, [C.POINTER(heap_t), C.c_int, sz, PD, PD, C.POINTER(sz), sz, PD,
+                                               C.c_int]),
             "oracle_heap_traverse": (None, [C.POINTER(heap_t), PD, C.c_double, PD]),
             "oracle_heap_traverse_stackless": (None, [C.POINTER(heap_t), PD, C.c_double, PD]),
             "oracle_bh_fcompute": (None, [C.POINTER(heap_t), C.c_int, C.c_int, sz, PD, PD, PD, C.c_int]),
@@ -150,6 +153,20 @@ class Heap:
         n = self.n
         rx = np.ascontiguousarray(y[:n]); ry = np.ascontiguousarray(y[n:2 * n]); rz = np.ascontiguousarray(y[2 * n:3 * n])
         lib().oracle_heap_build(C.byref(self.h), p(rx), p(ry), p(rz), p(np.ascontiguousarray(mass)), ratio)
+
+    def build_omp(self, y: np.ndarray, mass: np.ndarray, ratio: float, threads: int = 0):
+        n = self.n
+        rx = np.ascontiguousarray(y[:n]); ry = np.ascontiguousarray(y[n:2 * n]); rz = np.ascontiguousarray(y[2 * n:3 * n])
+        assert lib().oracle_heap_build_omp(C.byref(self.h), p(rx), p(ry), p(rz), p(np.ascontiguousarray(mass)),
+                                           ratio, threads) == 0
+
+    def accel_targets(self, y, mass, idx, layout: int = 2, threads: int = 0) -> np.ndarray:
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        out = np.empty(3 * len(idx))
+        lib().oracle_bh_accel_targets(C.byref(self.h), layout, self.n, p(np.ascontiguousarray(y)),
+                                      p(np.ascontiguousarray(mass)), idx.ctypes.data_as(C.POINTER(sz)), len(idx),
+                                      p(out), threads)
+        return out.reshape(-1, 3)
 
     def rebuild(self, y: np.ndarray, ratio: float):
         n = self.n

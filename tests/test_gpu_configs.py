@@ -7,6 +7,10 @@
   rkfeagin10 and Bulirsch-Stoer (max_level 8) with FP32 force tiles and the
   solvers' compensated FP64 combination against all-FP64; the FP64 run is
   checked bit-exact against the oracle, the FP32 run by its energy drift.
+* Plummer sphere, N = 2^18 (seed 2), the direct-sum throughput workload at its
+  full size: FP32 tiles against the oracle's FP64 direct sum on 8192 sampled
+  targets, and two rkdp steps (error_threshold 1e-9, one recursion level) with
+  FP32 tiles against the same run in FP64.
 * Two Plummer spheres approaching (the Barnes-Hut workload, scaled down),
   theta = 0.5, rk4: the tree is rebuilt each step; node order and moments of
   the final state bit-exact with the oracle's heap, forces near FP64 direct.
@@ -86,7 +90,9 @@ def _energy(d, e):
 def test_cold_cube_split_precision(solver):
     n, dt, steps = 4096, 1e-3, 20 if solver["solver"] == "rkfeagin10" else 8
     runs = {}
-    for label, prec, corr in (("fp64", "fp64", False), ("fp32", "fp32", True), ("oracle", None, False)):
+    counts = {}
+    for label, prec, corr in (("fp64", "fp64", False), ("fp32", "fp32", True), ("fp32_corr", "fp32_corr", True),
+                              ("oracle", None, False)):
         d = _data("cube", n, 5)
         _, v, m, _ = d.bodies()
         assert np.all(v == 0) and abs(m.sum() - 1.0) < 1e-12
@@ -102,25 +108,102 @@ def test_cold_cube_split_precision(solver):
         for _ in range(steps):
             s.advise(dt)
         e.get_data(d)
+        counts[label] = e.get_compute_count()
         runs[label] = (d, e_start, _energy(d, e if prec is not None else None))
         s.destroy()
         e.destroy()
     p64, v64, _, _ = runs["fp64"][0].bodies()
     po, vo, _, _ = runs["oracle"][0].bodies()
     assert np.array_equal(p64, po) and np.array_equal(v64, vo)
+    assert counts["fp64"] == counts["oracle"]
     drift64 = runs["fp64"][2] - runs["fp64"][1]
     drift32 = runs["fp32"][2] - runs["fp32"][1]
+    driftc = runs["fp32_corr"][2] - runs["fp32_corr"][1]
     e0 = abs(runs["fp64"][1])
-    # the split-precision run follows the FP64 energy history to the FP32
-    # rounding of close pairs (measured for rkfeagin10, 20 steps: 6.9e-6 of |E0|)
-    print(f"\n[split precision] {solver['solver']}: energy drift FP64 {drift64 / e0:.3e}, "
-          f"FP32 tiles {drift32 / e0:.3e} (relative to |E0|)")
-    assert abs(drift32 - drift64) < 3e-5 * e0, (drift32, drift64, e0)
     p32, _, _, _ = runs["fp32"][0].bodies()
+    pc, _, _, _ = runs["fp32_corr"][0].bodies()
     dev = np.linalg.norm(p32 - p64, axis=1)
+    devc = np.linalg.norm(pc - p64, axis=1)
+    print(f"\n[split precision] {solver['solver']}: energy drift / |E0|: FP64 {drift64 / e0:.3e}, "
+          f"FP32 tiles {drift32 / e0:.3e}, FP32 + FP64 correction pass {driftc / e0:.3e}; "
+          f"position deviation from FP64 max {dev.max():.2e} / {devc.max():.2e}, p99 "
+          f"{np.percentile(dev, 99):.2e} / {np.percentile(devc, 99):.2e}; force evaluations {counts}")
+    # the split-precision runs follow the FP64 energy history to the FP32
+    # rounding of close pairs (measured for rkfeagin10, 20 steps: 6.9e-6 of |E0|)
+    for drift in (drift32, driftc):
+        assert abs(drift - drift64) < 3e-5 * e0, (drift, drift64, e0)
     # most bodies agree to ~1e-12; close pairs carry the FP32 offset rounding
     # (measured max 3.5e-6 for rkfeagin10, 20 steps)
-    assert np.percentile(dev, 99) < 1e-9 and dev.max() < 1e-4, (np.percentile(dev, 99), dev.max())
+    for d in (dev, devc):
+        assert np.percentile(d, 99) < 1e-9 and d.max() < 1e-4, (np.percentile(d, 99), d.max())
+    if solver["solver"] == "rkfeagin10":
+        # fixed-step rkfeagin10: the correction pass takes the close pairs' FP32
+        # rounding out, so the run stays at least as close to FP64 as the tiles
+        assert counts["fp32"] == counts["fp32_corr"] == counts["fp64"]
+        assert devc.max() <= dev.max() and abs(driftc - drift64) <= abs(drift32 - drift64)
+
+
+def test_plummer_2e18_fp32_tiles():
+    """The direct-sum workload at full size: N = 2^18 Plummer bodies (seed 2).
+
+    (a) Force evaluation as the reference's engine harness does it
+    (test/engine/test_nbody_engine.cpp:472-550: the derivative buffer is
+    filled with a sentinel, fcompute, read back): the `hip` engine's FP32 tiles
+    against the oracle's FP64 direct sum on 8192 sampled targets, RMS and p99
+    relative error below 1e-5, maximum below 1e-3; the velocity half is an exact copy.
+    (b) Two rkdp steps of dt = 1e-3 (error_threshold 1e-9, max_recursion 1)
+    with FP32 tiles against the same run in FP64: the FP32 noise floor of the
+    error estimate subdivides every step into 8 substeps (7 + 8 * 7 = 63 force
+    evaluations); the states stay within the force error times the elapsed time."""
+    n, seed, dt = 1 << 18, 2, 1e-3
+    d = _data("plummer", n, seed)
+    pos, vel, mass, _ = d.bodies()
+    y = np.concatenate([pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1], vel[:, 2]])
+    e = nbody_create_engine({"engine": "hip", "force_precision": "fp32"})
+    assert e.init(d)
+    f = e.create_buffer(48 * n)
+    e.fill_buffer(f, -1e10)
+    e.fcompute(0.0, e.get_y(), f)
+    out = e.read_buffer(f)
+    e.free_buffer(f)
+    assert np.array_equal(out[:3 * n], y[3 * n:])
+    idx = np.random.default_rng(2).choice(n, 8192, replace=False)
+    a0 = oracle_lib.accel_targets(y, mass, idx.astype(np.uint64), 0)
+    a = out[3 * n:].reshape(3, n).T[idx]
+    rel = np.linalg.norm(a - a0, axis=1) / np.linalg.norm(a0, axis=1)
+    rms, p99 = float(np.sqrt(np.mean(rel ** 2))), float(np.percentile(rel, 99))
+    print(f"\n[plummer 2^18] FP32 tiles vs oracle on 8192 targets: rms {rms:.2e} p99 {p99:.2e} max {rel.max():.2e}")
+    assert rms < 1e-5 and p99 < 1e-5 and rel.max() < 1e-3
+    e.destroy()
+
+    finals, counts = {}, {}
+    for prec in ("fp32", "fp64"):
+        d = _data("plummer", n, seed)
+        e = nbody_create_engine({"engine": "hip", "force_precision": prec})
+        assert e.init(d)
+        s = nbody_create_solver({"solver": "rkdp", "max_step": dt, "error_threshold": 1e-9, "max_recursion": 1})
+        s.set_engine(e)
+        per_step = []
+        for _ in range(2):
+            c0 = e.get_compute_count()
+            s.advise(dt)
+            per_step.append(e.get_compute_count() - c0)
+        assert not e.has_error()
+        e.get_data(d)
+        finals[prec] = d.bodies()
+        counts[prec] = per_step
+        s.destroy()
+        e.destroy()
+    (p32, v32, _, _), (p64, v64, _, _) = finals["fp32"], finals["fp64"]
+    dv = np.linalg.norm(v32 - v64, axis=1)
+    dx = np.linalg.norm(p32 - p64, axis=1)
+    print(f"[plummer 2^18] rkdp 2 steps, force evaluations per step FP32 {counts['fp32']} FP64 {counts['fp64']}; "
+          f"|dv| max {dv.max():.2e} p99 {np.percentile(dv, 99):.2e}, |dx| max {dx.max():.2e}")
+    assert counts["fp32"] == [63, 63]
+    assert all(c in (7, 63) for c in counts["fp64"])
+    # |a| < 1 for these bodies; the FP32 force error (max 1e-3, RMS 1e-5 relative)
+    # over t = 2e-3 bounds the velocity deviation, t times that the positions
+    assert dv.max() < 2e-6 and np.percentile(dv, 99) < 1e-7 and dx.max() < 4e-9
 
 
 def test_two_plummer_bh_rk4():

@@ -566,6 +566,43 @@ def test_two_level_walk_overflow_fallback(ctx):
     t.close()
 
 
+def _tree_bytes(ctx, t):
+    b = C.c_size_t()
+    native.check(ctx.lib.nbh_tree_bytes(ctx.h, t.h, C.byref(b)), "tree_bytes")
+    return b.value
+
+
+def test_two_level_list_allocation_is_all_or_nothing(ctx, monkeypatch):
+    """A failed allocation of the two-level walk's lists (forced through the
+    library's NBH_TEST_FAIL_ALLOC hook at each of its three allocations, in
+    nbh_tree_set_list_cap and in the walk's own first-use allocation) leaves
+    none of them allocated and launches nothing; the next call allocates them
+    afresh and the walk equals that of a tree that never failed."""
+    n, ratio, near_k, cap = 1 << 14, 1.0 / 0.6, 3.0, 4096
+    y, m = _state("plummer", n, 31)
+    t = Tree(ctx, y, m, ratio)
+    base = _tree_bytes(ctx, t)
+    lists = (n >> 10) * (cap * 4 + 4 + 34 * 4)
+    df = ctx.alloc(6 * n)
+    for k in (1, 2, 3):
+        monkeypatch.setenv("NBH_TEST_FAIL_ALLOC", str(k))
+        assert ctx.lib.nbh_tree_set_list_cap(ctx.h, t.h, cap) == 3          # NBH_ERR_NOMEM
+        assert _tree_bytes(ctx, t) == base
+        assert ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t.h, t.dy.ptr, df.ptr, near_k) == 3
+        assert _tree_bytes(ctx, t) == base
+        monkeypatch.delenv("NBH_TEST_FAIL_ALLOC")
+    native.check(ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local2")
+    assert _tree_bytes(ctx, t) == base + lists
+    got = df.read()
+    t2 = Tree(ctx, y, m, ratio)
+    native.check(ctx.lib.nbh_tree_set_list_cap(ctx.h, t2.h, cap), "cap")
+    assert _tree_bytes(ctx, t2) == base + lists
+    native.check(ctx.lib.nbh_bh_fcompute_group_local2(ctx.h, t2.h, t2.dy.ptr, df.ptr, near_k), "local2")
+    assert np.array_equal(got, df.read())
+    t2.close()
+    t.close()
+
+
 def test_local_far_field_walk_accuracy(ctx):
     """At theta 0.6 the local far field keeps the quadrupole walk's accuracy
     against the FP64 direct sum (every cell per body as the reference point)."""
