@@ -25,6 +25,8 @@ typedef struct nbs_data nbs_data;
 typedef struct nbs_engine nbs_engine;
 typedef struct nbs_memory nbs_memory;
 typedef struct nbs_solver nbs_solver;
+typedef struct nbs_stream nbs_stream;
+typedef struct nbs_stream_reader nbs_stream_reader;
 
 /* ---- diagnostics ---- */
 size_t      nbs_log_count(void);
@@ -115,7 +117,10 @@ void        nbs_solver_destroy(nbs_solver* s);
 const char* nbs_solver_type_name(const nbs_solver* s);
 void        nbs_solver_set_engine(nbs_solver* s, nbs_engine* e);
 void        nbs_solver_set_time_step(nbs_solver* s, double min_step, double max_step);
-int         nbs_solver_run(nbs_solver* s, nbs_data* d, double max_time, double dump_dt, double check_dt);
+/* nbody_solver::run (nbody/nbody_solver.cpp:55-105); stream may be NULL
+   (no dumps).  Frames go to the stream from a writer thread. */
+int         nbs_solver_run(nbs_solver* s, nbs_data* d, nbs_stream* stream, double max_time, double dump_dt,
+                           double check_dt);
 void        nbs_solver_advise(nbs_solver* s, double dt);
 void        nbs_solver_print_info(const nbs_solver* s);
 /* Runge-Kutta (Butcher table) solvers only; return 0 for other solvers */
@@ -130,6 +135,32 @@ size_t      nbs_solver_rk_substeps(const nbs_solver* s);
 nbs_solver* nbs_solver_create_rk_butcher(const char* table);
 /* Adams-Bashforth of the given rank with an euler starter */
 nbs_solver* nbs_solver_create_adams(size_t rank, int correction);
+
+/* ---- frame streams (nbody/nbody_data_stream.h, nbody_data_stream_reader.h):
+        <base>.idx, <base>K.dat, <base>.col, <base>.mass; 0 = success, -1 = failure ---- */
+nbs_stream* nbs_stream_create(void);
+void        nbs_stream_destroy(nbs_stream* s);
+/* append_to: a loaded reader whose stream is continued, or NULL */
+int         nbs_stream_open(nbs_stream* s, const char* base, int64_t max_part_size,
+                            const nbs_stream_reader* append_to);
+int         nbs_stream_write(nbs_stream* s, const nbs_data* d);
+void        nbs_stream_close(nbs_stream* s);
+nbs_stream_reader* nbs_stream_reader_create(void);
+void        nbs_stream_reader_destroy(nbs_stream_reader* r);
+int         nbs_stream_reader_load(nbs_stream_reader* r, const char* base);
+void        nbs_stream_reader_close(nbs_stream_reader* r);
+size_t      nbs_stream_reader_frame_count(const nbs_stream_reader* r);
+size_t      nbs_stream_reader_steps_count(const nbs_stream_reader* r);
+double      nbs_stream_reader_max_time(const nbs_stream_reader* r);
+int         nbs_stream_reader_seek(nbs_stream_reader* r, size_t frame);
+size_t      nbs_stream_reader_current_frame(const nbs_stream_reader* r);
+size_t      nbs_stream_reader_body_count(const nbs_stream_reader* r);
+size_t      nbs_stream_reader_coord_size(const nbs_stream_reader* r);
+double      nbs_stream_reader_box_size(const nbs_stream_reader* r);
+size_t      nbs_stream_reader_last_file_n(const nbs_stream_reader* r);
+int         nbs_stream_reader_read(nbs_stream_reader* r, nbs_data* d);
+/* Colours of the bodies, RGBA float [4n] */
+void        nbs_data_get_color(const nbs_data* d, float* rgba);
 
 /* ---- extrapolators (nbody/nbody_extrapolator.h) ---- */
 typedef struct nbs_extrapolator nbs_extrapolator;

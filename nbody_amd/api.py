@@ -91,6 +91,25 @@ class nbody_data:
         self._lib.nbs_data_get(self.handle, _pd(pos), _pd(vel), _pd(m), _pd(r))
         return pos, vel, m, r
 
+    def resize(self, n: int) -> None:
+        """n bodies at the origin with zero mass (nbody_data::resize)."""
+        self._lib.nbs_data_resize(self.handle, n)
+
+    def copy(self) -> "nbody_data":
+        """A new nbody_data with the same bodies, time and step."""
+        d = nbody_data()
+        pos, vel, m, r = self.bodies()
+        d.set_bodies(pos, vel, m, r)
+        d.set_time(self.get_time())
+        d.set_step(self.get_step())
+        return d
+
+    def colors(self) -> np.ndarray:
+        """RGBA float colours [n][4]."""
+        c = np.empty((self.get_count(), 4), np.float32)
+        self._lib.nbs_data_get_color(self.handle, c.ctypes.data_as(C.POINTER(C.c_float)))
+        return c
+
     def get_time(self) -> float:
         return self._lib.nbs_data_time(self.handle)
 
@@ -363,8 +382,10 @@ class nbody_solver:
     def set_time_step(self, min_step: float, max_step: float) -> None:
         self._lib.nbs_solver_set_time_step(self.handle, min_step, max_step)
 
-    def run(self, data: nbody_data, max_time: float, dump_dt: float = 0.0, check_dt: float = 0.0) -> int:
-        return self._lib.nbs_solver_run(self.handle, data.handle, max_time, dump_dt, check_dt)
+    def run(self, data: nbody_data, max_time: float, dump_dt: float = 0.0, check_dt: float = 0.0,
+            stream: Optional["nbody_data_stream"] = None) -> int:
+        return self._lib.nbs_solver_run(self.handle, data.handle, stream.handle if stream is not None else None,
+                                        max_time, dump_dt, check_dt)
 
     def advise(self, dt: float) -> None:
         self._lib.nbs_solver_advise(self.handle, dt)
@@ -395,3 +416,79 @@ def nbody_create_solver(params: dict) -> Optional[nbody_solver]:
     """Solver factory; None for an unknown solver or an invalid parameter."""
     h = native.host_lib().nbs_solver_create(_params(params))
     return nbody_solver(h) if h else None
+
+
+class nbody_data_stream:
+    """Frame-file writer (nbody/nbody_data_stream.h): <base>.idx, <base>K.dat,
+    <base>.col, <base>.mass.  Methods return 0 on success, -1 on failure."""
+
+    def __init__(self):
+        self._lib = native.host_lib()
+        self.handle = self._lib.nbs_stream_create()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            self._lib.nbs_stream_destroy(h)
+            self.handle = None
+
+    def open(self, base: str, max_part_size: int = 1 << 30,
+             append_to: Optional["nbody_data_stream_reader"] = None) -> int:
+        return self._lib.nbs_stream_open(self.handle, base.encode(), max_part_size,
+                                         append_to.handle if append_to is not None else None)
+
+    def write(self, data: Optional[nbody_data]) -> int:
+        return self._lib.nbs_stream_write(self.handle, data.handle if data is not None else None)
+
+    def close(self) -> None:
+        self._lib.nbs_stream_close(self.handle)
+
+
+class nbody_data_stream_reader:
+    """Frame-file reader (nbody/nbody_data_stream_reader.h)."""
+
+    def __init__(self):
+        self._lib = native.host_lib()
+        self.handle = self._lib.nbs_stream_reader_create()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            self._lib.nbs_stream_reader_destroy(h)
+            self.handle = None
+
+    def load(self, base: str) -> int:
+        return self._lib.nbs_stream_reader_load(self.handle, base.encode())
+
+    def close(self) -> None:
+        self._lib.nbs_stream_reader_close(self.handle)
+
+    def get_frame_count(self) -> int:
+        return self._lib.nbs_stream_reader_frame_count(self.handle)
+
+    def get_steps_count(self) -> int:
+        return self._lib.nbs_stream_reader_steps_count(self.handle)
+
+    def get_max_time(self) -> float:
+        return self._lib.nbs_stream_reader_max_time(self.handle)
+
+    def seek(self, frame: int) -> int:
+        return self._lib.nbs_stream_reader_seek(self.handle, frame)
+
+    def get_current_frame(self) -> int:
+        return self._lib.nbs_stream_reader_current_frame(self.handle)
+
+    def get_body_count(self) -> int:
+        return self._lib.nbs_stream_reader_body_count(self.handle)
+
+    def get_coord_size(self) -> int:
+        return self._lib.nbs_stream_reader_coord_size(self.handle)
+
+    def get_box_size(self) -> float:
+        return self._lib.nbs_stream_reader_box_size(self.handle)
+
+    def get_last_file_n(self) -> int:
+        return self._lib.nbs_stream_reader_last_file_n(self.handle)
+
+    def read(self, data: Optional[nbody_data]) -> int:
+        return self._lib.nbs_stream_reader_read(self.handle, data.handle if data is not None else None)

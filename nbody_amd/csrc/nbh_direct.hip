@@ -1,6 +1,6 @@
 // nbh_direct.hip -- direct all-pairs accelerations on gfx950.
 //
-// FP64 kernel: one target per work-item, sources staged in LDS tiles, the
+// FP64 kernel: two targets per work-item, sources staged in LDS tiles, the
 // source loop runs in index order and each pair is evaluated with exactly the
 // operations of nbody_data::force followed by the division by m1 of
 // nbody_engine_simple::fcompute (nbody/nbody_data.cpp:35-44,
@@ -8,13 +8,15 @@
 // -ffp-contract=off, so the FP64 result equals the CPU restatement bit for bit
 // (IEEE sqrt and division are correctly rounded on both sides).
 //
-// FP32 kernel (the throughput path): each work-group owns 256*TGT targets and a
-// slice of the sources.  Sources are converted once per work-group from FP64 to
-// FP32 coordinates relative to the work-group's origin (the FP64 position of
-// its first target), staged in LDS as float4 (x, y, z, m) and read back as
-// wave-wide broadcasts; each work-item reuses every source for TGT targets
-// held in registers.  Per pair: 3 sub, 3 mul/fma for r^2, 1 max (the
-// reference's r^2 >= 1e-8 clamp), 1 v_rsq_f32, 3 mul, 3 fma.  Partial sums of
+// FP32 kernel (the throughput path): the bodies are sorted along a Morton curve
+// and copied once into records (x, y, z, m) in that order; each work-group owns
+// 256*TGT consecutive targets and a slice of the sources.  Sources are converted
+// once per work-group from FP64 to FP32 coordinates relative to the
+// work-group's origin (the FP64 position of its first target), staged in LDS as
+// float4 (x, y, z, m) and read back as wave-wide broadcasts; each work-item
+// reuses every source for TGT targets held in registers.  Per pair: 3 sub,
+// 3 mul/fma for r^2, 1 v_rsq_f32 whose output clamp is the reference's
+// r^2 >= 1e-8 clamp (in scaled units), 3 mul, 3 fma.  Partial sums of
 // one LDS tile stay in FP32 and are folded into a Kahan (summation_k) FP32
 // accumulator per tile; the source slices are then added in FP64 in a fixed
 // order by a second kernel, which also copies the velocities into f[0:3n].
@@ -30,65 +32,155 @@ constexpr double kMinDistance = 1e-8;   // nbody/nbtype.h:79
 // ---------------------------------------------------------------------------
 // FP64, reference operation order
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024)
-k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass,
-              double* __restrict__ f, size_t n)
+// Correctly rounded binary64 square root and quotient without the range steps
+// of the general sequences.  The compiler's sqrt is rsq + two Goldschmidt
+// refinements + two Newton corrections, wrapped in a scaling by 2^+-256 for
+// inputs below 2^-767 and a pass-through for 0 and inf; its division is rcp +
+// two Newton refinements of the reciprocal, the quotient and one fma
+// correction, wrapped in v_div_scale / v_div_fmas / v_div_fixup, which scale
+// operands whose exponents are near the ends of the range and handle 0, inf and
+// NaN.  For the operands of the pair term -- r2 in [1e-8, 1.3e13] (coordinates of
+// magnitude <= 1e6) and numerators -m1 m2 with masses of magnitude in [1e-60,
+// 1e60] -- those wrappers are identities, so the bare sequences below give the
+// same bits (checked against the CPU restatement by the tests).  Tiles holding
+// any other operand take the general sequences.
+__device__ inline double sqrt_rn(double x)
 {
-	extern __shared__ double sh[];
+	const double y = __builtin_amdgcn_rsq(x);
+	double g = x * y;
+	double h = y * 0.5;
+	const double r = fma(-h, g, 0.5);
+	g = fma(g, r, g);
+	double d = fma(-g, g, x);
+	h = fma(h, r, h);
+	g = fma(d, h, g);
+	d = fma(-g, g, x);
+	return fma(d, h, g);
+}
+
+__device__ inline double div_rn(double a, double b)
+{
+	double r = __builtin_amdgcn_rcp(b);
+	double e = fma(-b, r, 1.0);
+	r = fma(r, e, r);
+	e = fma(-b, r, 1.0);
+	r = fma(r, e, r);
+	const double q = a * r;
+	const double rem = fma(-b, q, a);
+	return fma(rem, r, q);
+}
+
+__device__ inline bool coord_in_range(double v) { return fabs(v) <= 1e6; }
+__device__ inline bool mass_in_range(double m) { return fabs(m) >= 1e-60 && fabs(m) <= 1e60; }
+
+// One pair term of nbody_data::force (nbody/nbody_data.cpp:35-44) with the
+// mass factor -m1 precomputed: t += d * ((gm1 * m2) / (r2 * sqrt(r2))).
+template <bool kFast>
+__device__ inline void pair64(double x1, double y1, double z1, double gm1, const double4& s, double& tx, double& ty,
+                              double& tz)
+{
+	const double dx = x1 - s.x;
+	const double dy = y1 - s.y;
+	const double dz = z1 - s.z;
+	double r2 = dx * dx + dy * dy + dz * dz;
+	double coeff;
+	if constexpr(kFast)
+	{
+		r2 = fmax(r2, kMinDistance);     // r2 is finite here: the same as the branch below
+		coeff = div_rn(gm1 * s.w, r2 * sqrt_rn(r2));
+	}
+	else
+	{
+		if(r2 < kMinDistance)
+		{
+			r2 = kMinDistance;
+		}
+		coeff = (gm1 * s.w) / (r2 * sqrt(r2));
+	}
+	tx += dx * coeff;
+	ty += dy * coeff;
+	tz += dz * coeff;
+}
+
+// kF64Tgt targets per work-item (i0 + k * blockDim.x), sources staged in LDS as
+// double4 (x, y, z, m) tiles of blockDim.x and read back as wave-wide
+// broadcasts, each reused for every target of the work-item.  The source loop
+// runs in index order.  The self pair is skipped in the reference; here, on
+// the fast path, it is summed: dx = dy = dz = +0 there, so its term is -0 and
+// adding -0 leaves every value (+0 included) unchanged.
+constexpr int kF64Tgt = 2;
+
+__global__ void __launch_bounds__(1024)
+k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass, double* __restrict__ f, size_t n)
+{
+	extern __shared__ double4 src[];
 	const int bs = blockDim.x;
-	double* sx = sh;
-	double* sy = sh + bs;
-	double* sz = sh + 2 * bs;
-	double* sm = sh + 3 * bs;
-
-	const size_t i = static_cast<size_t>(blockIdx.x) * bs + threadIdx.x;
-	const bool active = i < n;
-	const double x1 = active ? y[i] : 0.0;
-	const double y1 = active ? y[n + i] : 0.0;
-	const double z1 = active ? y[2 * n + i] : 0.0;
-	const double m1 = active ? mass[i] : 1.0;
-	const double gm1 = -1.0 * m1;            // (-GravityConst * mass1), G = 1
-	double tx = 0.0, ty = 0.0, tz = 0.0;
-
+	const size_t i0 = static_cast<size_t>(blockIdx.x) * (kF64Tgt * bs) + threadIdx.x;
+	double x1[kF64Tgt], y1[kF64Tgt], z1[kF64Tgt], m1[kF64Tgt], gm1[kF64Tgt];
+	double tx[kF64Tgt], ty[kF64Tgt], tz[kF64Tgt];
+	bool safe = true;
+	for(int k = 0; k < kF64Tgt; ++k)
+	{
+		const size_t i = i0 + static_cast<size_t>(k) * bs;
+		const bool active = i < n;
+		x1[k] = active ? y[i] : 0.0;
+		y1[k] = active ? y[n + i] : 0.0;
+		z1[k] = active ? y[2 * n + i] : 0.0;
+		m1[k] = active ? mass[i] : 1.0;
+		gm1[k] = -1.0 * m1[k];            // (-GravityConst * mass1), G = 1
+		tx[k] = ty[k] = tz[k] = 0.0;
+		safe = safe && coord_in_range(x1[k]) && coord_in_range(y1[k]) && coord_in_range(z1[k]) &&
+		       (!active || mass_in_range(m1[k]));
+	}
 	for(size_t base = 0; base < n; base += bs)
 	{
 		const size_t j = base + threadIdx.x;
 		__syncthreads();
-		sx[threadIdx.x] = j < n ? y[j] : 0.0;
-		sy[threadIdx.x] = j < n ? y[n + j] : 0.0;
-		sz[threadIdx.x] = j < n ? y[2 * n + j] : 0.0;
-		sm[threadIdx.x] = j < n ? mass[j] : 0.0;
-		__syncthreads();
-		const int cnt = n - base < static_cast<size_t>(bs) ? static_cast<int>(n - base) : bs;
-		const int self = (active && i >= base && i < base + cnt) ? static_cast<int>(i - base) : -1;
-		for(int jj = 0; jj < cnt; ++jj)
+		double4 s = make_double4(0.0, 0.0, 0.0, 0.0);
+		bool ok = true;
+		if(j < n)
 		{
-			if(jj == self)
+			s = make_double4(y[j], y[n + j], y[2 * n + j], mass[j]);
+			ok = coord_in_range(s.x) && coord_in_range(s.y) && coord_in_range(s.z) && mass_in_range(s.w);
+		}
+		src[threadIdx.x] = s;
+		const bool fast = __syncthreads_and(ok && safe) != 0;
+		const int cnt = n - base < static_cast<size_t>(bs) ? static_cast<int>(n - base) : bs;
+		if(fast)
+		{
+			for(int jj = 0; jj < cnt; ++jj)
 			{
-				continue;
+				const double4 sj = src[jj];
+				for(int k = 0; k < kF64Tgt; ++k) { pair64<true>(x1[k], y1[k], z1[k], gm1[k], sj, tx[k], ty[k], tz[k]); }
 			}
-			const double dx = x1 - sx[jj];
-			const double dy = y1 - sy[jj];
-			const double dz = z1 - sz[jj];
-			double r2 = dx * dx + dy * dy + dz * dz;
-			if(r2 < kMinDistance)
+		}
+		else
+		{
+			for(int jj = 0; jj < cnt; ++jj)
 			{
-				r2 = kMinDistance;
+				const double4 sj = src[jj];
+				for(int k = 0; k < kF64Tgt; ++k)
+				{
+					if(base + jj != i0 + static_cast<size_t>(k) * bs)
+					{
+						pair64<false>(x1[k], y1[k], z1[k], gm1[k], sj, tx[k], ty[k], tz[k]);
+					}
+				}
 			}
-			const double coeff = (gm1 * sm[jj]) / (r2 * sqrt(r2));
-			tx += dx * coeff;
-			ty += dy * coeff;
-			tz += dz * coeff;
 		}
 	}
-	if(active)
+	for(int k = 0; k < kF64Tgt; ++k)
 	{
-		f[i] = y[3 * n + i];
-		f[n + i] = y[4 * n + i];
-		f[2 * n + i] = y[5 * n + i];
-		f[3 * n + i] = tx / m1;
-		f[4 * n + i] = ty / m1;
-		f[5 * n + i] = tz / m1;
+		const size_t i = i0 + static_cast<size_t>(k) * bs;
+		if(i < n)
+		{
+			f[i] = y[3 * n + i];
+			f[n + i] = y[4 * n + i];
+			f[2 * n + i] = y[5 * n + i];
+			f[3 * n + i] = tx[k] / m1[k];
+			f[4 * n + i] = ty[k] / m1[k];
+			f[5 * n + i] = tz[k] / m1[k];
+		}
 	}
 }
 
@@ -177,21 +269,37 @@ __device__ inline void kahan_f32(float& s, float& c, float v)
 	s = ns;
 }
 
-// part: [nsplit][3][n] doubles, slice blockIdx.y of the sources.  Targets are
-// taken in the order `perm` (bodies sorted along a space-filling curve), so a
-// work-group's 1024 targets are close together and their offsets from its
-// origin -- and those of the sources near them -- are small in FP32.
+// The state in Morton order as one record per body: sorted[p] = (x, y, z, m) of
+// body perm[p].  Written once per force evaluation, so that the tile kernel
+// reads its targets and sources as contiguous 32-byte records instead of
+// gathering three coordinates through the permutation.
+__global__ void __launch_bounds__(256)
+k_sorted_copy(const double* __restrict__ y, const double* __restrict__ mass, const uint32_t* __restrict__ perm,
+              double4* __restrict__ sorted, size_t n)
+{
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
+	{
+		const size_t b = perm[p];
+		sorted[p] = make_double4(y[b], y[n + b], y[2 * n + b], mass[b]);
+	}
+}
+
+// part: [nsplit][3][n] doubles by Morton position, slice blockIdx.y of the
+// sources (also in Morton order).  A work-group's 1024 targets are consecutive
+// along the curve, so they are close together and their offsets from its
+// origin (its first target) -- and those of the sources near them -- are
+// small in FP32.
 __global__ void __launch_bounds__(kF32Threads)
-k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, const uint32_t* __restrict__ perm,
-              double* __restrict__ part, size_t n, size_t jchunk)
+k_direct_fp32(const double4* __restrict__ sorted, double* __restrict__ part, size_t n, size_t jchunk)
 {
 	__shared__ float4 tile[kF32Tile];
 
 	const size_t tbase = static_cast<size_t>(blockIdx.x) * kF32TgtPerGroup;
-	const size_t o = perm[tbase];
-	const double ox = y[o];
-	const double oy = y[n + o];
-	const double oz = y[2 * n + o];
+	const double4 org = sorted[tbase];
+	const double ox = org.x;
+	const double oy = org.y;
+	const double oz = org.z;
 
 	float xi[kF32Tgt], yi[kF32Tgt], zi[kF32Tgt];
 	float sx[kF32Tgt], sy[kF32Tgt], sz[kF32Tgt];
@@ -201,10 +309,10 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, con
 	{
 		size_t i = tbase + static_cast<size_t>(k) * kF32Threads + threadIdx.x;
 		bool ok = i < n;
-		const size_t b = ok ? perm[i] : 0;
-		xi[k] = ok ? static_cast<float>((y[b] - ox) * kF32Len) : 0.0f;
-		yi[k] = ok ? static_cast<float>((y[n + b] - oy) * kF32Len) : 0.0f;
-		zi[k] = ok ? static_cast<float>((y[2 * n + b] - oz) * kF32Len) : 0.0f;
+		const double4 t = ok ? sorted[i] : org;
+		xi[k] = ok ? static_cast<float>((t.x - ox) * kF32Len) : 0.0f;
+		yi[k] = ok ? static_cast<float>((t.y - oy) * kF32Len) : 0.0f;
+		zi[k] = ok ? static_cast<float>((t.z - oz) * kF32Len) : 0.0f;
 		sx[k] = sy[k] = sz[k] = 0.0f;
 		cx[k] = cy[k] = cz[k] = 0.0f;
 	}
@@ -218,10 +326,11 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, con
 		__syncthreads();
 		if(j < jend)
 		{
-			tile[threadIdx.x] = make_float4(static_cast<float>((y[j] - ox) * kF32Len),
-			                                static_cast<float>((y[n + j] - oy) * kF32Len),
-			                                static_cast<float>((y[2 * n + j] - oz) * kF32Len),
-			                                static_cast<float>(mass[j] * kF32Mass));
+			const double4 sj = sorted[j];
+			tile[threadIdx.x] = make_float4(static_cast<float>((sj.x - ox) * kF32Len),
+			                                static_cast<float>((sj.y - oy) * kF32Len),
+			                                static_cast<float>((sj.z - oz) * kF32Len),
+			                                static_cast<float>(sj.w * kF32Mass));
 		}
 		else
 		{
@@ -281,15 +390,16 @@ k_direct_fp32(const double* __restrict__ y, const double* __restrict__ mass, con
 constexpr int kCorrWindow = 32;
 
 __global__ void __launch_bounds__(256)
-k_direct_window(const double* __restrict__ y, const double* __restrict__ mass, const uint32_t* __restrict__ perm,
-                double* __restrict__ f, size_t n, int w)
+k_direct_window(const double4* __restrict__ sorted, const uint32_t* __restrict__ perm, double* __restrict__ f,
+                size_t n, int w)
 {
 	const size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if(p >= n) { return; }
 	const uint32_t i = perm[p];
-	const uint32_t o = perm[p / kF32TgtPerGroup * kF32TgtPerGroup];     // the tile kernel's origin
-	const double ox = y[o], oy = y[n + o], oz = y[2 * n + o];
-	const double xi = y[i], yi = y[n + i], zi = y[2 * n + i];
+	const double4 org = sorted[p / kF32TgtPerGroup * kF32TgtPerGroup];     // the tile kernel's origin
+	const double ox = org.x, oy = org.y, oz = org.z;
+	const double4 me = sorted[p];
+	const double xi = me.x, yi = me.y, zi = me.z;
 	const float txf = static_cast<float>((xi - ox) * kF32Len);
 	const float tyf = static_cast<float>((yi - oy) * kF32Len);
 	const float tzf = static_cast<float>((zi - oz) * kF32Len);
@@ -299,8 +409,8 @@ k_direct_window(const double* __restrict__ y, const double* __restrict__ mass, c
 	for(size_t q = q0; q <= q1; ++q)
 	{
 		if(q == p) { continue; }
-		const uint32_t j = perm[q];
-		const double xj = y[j], yj = y[n + j], zj = y[2 * n + j], mj = mass[j];
+		const double4 sj = sorted[q];
+		const double xj = sj.x, yj = sj.y, zj = sj.z, mj = sj.w;
 		const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
 		double r2 = dx * dx + dy * dy + dz * dz;
 		if(r2 < kMinDistance) { r2 = kMinDistance; }
@@ -474,9 +584,9 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 		int bs = block_size == 0 ? 256 : block_size;
 		NBH_CHECK(bs >= 64 && bs <= 1024 && bs % 64 == 0,
 		          "block_size %d: must be a multiple of 64 in [64, 1024]", block_size);
-		unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
+		unsigned grid = static_cast<unsigned>((n + kF64Tgt * bs - 1) / (kF64Tgt * bs));
 		nbh::scoped_timer timer(ctx, "direct_fp64");
-		hipLaunchKernelGGL(k_direct_fp64, dim3(grid), dim3(bs), 4 * bs * sizeof(double),
+		hipLaunchKernelGGL(k_direct_fp64, dim3(grid), dim3(bs), bs * sizeof(double4),
 		                   ctx->stream, y, mass, f, n);
 		NBH_LAUNCHED();
 		return NBH_OK;
@@ -498,7 +608,8 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	// scan chunk sums, bounding-box partials and the box
 	const unsigned gbox = nbh::grid_for(ctx, n, 256);
 	auto align = [](size_t b) { return (b + 255) / 256 * 256; };
-	const size_t off_keys = align(nsplit * 3 * n * sizeof(double));
+	const size_t off_sorted = align(nsplit * 3 * n * sizeof(double));
+	const size_t off_keys = off_sorted + align(n * sizeof(double4));
 	const size_t off_vals = off_keys + align(2 * n * sizeof(uint64_t));
 	const size_t off_counts = off_vals + align(2 * n * sizeof(uint32_t));
 	const size_t off_scan = off_counts + align(nbh::sort_counts_size(n) * sizeof(uint32_t));
@@ -508,6 +619,7 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	NBH_TRY(nbh::scratch(ctx, off_box + align(6 * sizeof(double)), &sp));
 	char* base = static_cast<char*>(sp);
 	double* part = reinterpret_cast<double*>(base);
+	double4* sorted = reinterpret_cast<double4*>(base + off_sorted);
 	uint64_t* keys = reinterpret_cast<uint64_t*>(base + off_keys);
 	uint32_t* vals = reinterpret_cast<uint32_t*>(base + off_vals);
 	uint32_t* counts = reinterpret_cast<uint32_t*>(base + off_counts);
@@ -522,10 +634,12 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	NBH_LAUNCHED();
 	uint32_t* perm = nullptr;
 	NBH_TRY(nbh::sort_pairs(ctx, keys, vals, keys + n, vals + n, n, 30, counts, scanp, &perm));
+	hipLaunchKernelGGL(k_sorted_copy, dim3(gbox), dim3(256), 0, ctx->stream, y, mass, perm, sorted, n);
+	NBH_LAUNCHED();
 	{
 		nbh::scoped_timer timer(ctx, "direct_fp32");
 		hipLaunchKernelGGL(k_direct_fp32, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(nsplit)),
-		                   dim3(kF32Threads), 0, ctx->stream, y, mass, perm, part, n, jchunk);
+		                   dim3(kF32Threads), 0, ctx->stream, sorted, part, n, jchunk);
 		NBH_LAUNCHED();
 	}
 	hipLaunchKernelGGL(k_direct_fp32_finish, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0,
@@ -534,7 +648,7 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	if(precision == NBH_FP32_CORR)
 	{
 		hipLaunchKernelGGL(k_direct_window, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream,
-		                   y, mass, perm, f, n, kCorrWindow);
+		                   sorted, perm, f, n, kCorrWindow);
 		NBH_LAUNCHED();
 	}
 	return NBH_OK;

@@ -255,9 +255,58 @@ const char* nbs_solver_type_name(const nbs_solver* s) { return S(s)->type_name()
 void nbs_solver_set_engine(nbs_solver* s, nbs_engine* e) { S(s)->set_engine(E(e)); }
 void nbs_solver_set_time_step(nbs_solver* s, double min_step, double max_step) { S(s)->set_time_step(min_step, max_step); }
 
-int nbs_solver_run(nbs_solver* s, nbs_data* d, double max_time, double dump_dt, double check_dt)
+int nbs_solver_run(nbs_solver* s, nbs_data* d, nbs_stream* stream, double max_time, double dump_dt, double check_dt)
 {
-	return S(s)->run(D(d), max_time, dump_dt, check_dt);
+	return S(s)->run(D(d), reinterpret_cast<nbody_data_stream*>(stream), max_time, dump_dt, check_dt);
+}
+
+nbs_stream* nbs_stream_create(void) { return reinterpret_cast<nbs_stream*>(new nbody_data_stream()); }
+void nbs_stream_destroy(nbs_stream* s) { delete reinterpret_cast<nbody_data_stream*>(s); }
+int nbs_stream_open(nbs_stream* s, const char* base, int64_t max_part_size, const nbs_stream_reader* append_to)
+{
+	return reinterpret_cast<nbody_data_stream*>(s)->open(base != nullptr ? base : "", max_part_size,
+	                                                     reinterpret_cast<const nbody_data_stream_reader*>(append_to));
+}
+int nbs_stream_write(nbs_stream* s, const nbs_data* d)
+{
+	return reinterpret_cast<nbody_data_stream*>(s)->write(reinterpret_cast<const nbody_data*>(d));
+}
+void nbs_stream_close(nbs_stream* s) { reinterpret_cast<nbody_data_stream*>(s)->close(); }
+
+static nbody_data_stream_reader* R(nbs_stream_reader* r) { return reinterpret_cast<nbody_data_stream_reader*>(r); }
+static const nbody_data_stream_reader* RC(const nbs_stream_reader* r)
+{
+	return reinterpret_cast<const nbody_data_stream_reader*>(r);
+}
+nbs_stream_reader* nbs_stream_reader_create(void)
+{
+	return reinterpret_cast<nbs_stream_reader*>(new nbody_data_stream_reader());
+}
+void nbs_stream_reader_destroy(nbs_stream_reader* r) { delete R(r); }
+int nbs_stream_reader_load(nbs_stream_reader* r, const char* base) { return R(r)->load(base != nullptr ? base : ""); }
+void nbs_stream_reader_close(nbs_stream_reader* r) { R(r)->close(); }
+size_t nbs_stream_reader_frame_count(const nbs_stream_reader* r) { return RC(r)->get_frame_count(); }
+size_t nbs_stream_reader_steps_count(const nbs_stream_reader* r) { return RC(r)->get_steps_count(); }
+double nbs_stream_reader_max_time(const nbs_stream_reader* r) { return RC(r)->get_max_time(); }
+int nbs_stream_reader_seek(nbs_stream_reader* r, size_t frame) { return R(r)->seek(frame); }
+size_t nbs_stream_reader_current_frame(const nbs_stream_reader* r) { return RC(r)->get_current_frame(); }
+size_t nbs_stream_reader_body_count(const nbs_stream_reader* r) { return RC(r)->get_body_count(); }
+size_t nbs_stream_reader_coord_size(const nbs_stream_reader* r) { return RC(r)->get_coord_size(); }
+double nbs_stream_reader_box_size(const nbs_stream_reader* r) { return RC(r)->get_box_size(); }
+size_t nbs_stream_reader_last_file_n(const nbs_stream_reader* r) { return RC(r)->get_last_file_n(); }
+int nbs_stream_reader_read(nbs_stream_reader* r, nbs_data* d) { return R(r)->read(reinterpret_cast<nbody_data*>(d)); }
+
+void nbs_data_get_color(const nbs_data* d, float* rgba)
+{
+	const nbody_data* x = reinterpret_cast<const nbody_data*>(d);
+	const nbcolor_t* c = x->get_color();
+	for(size_t i = 0; i < x->get_count(); ++i)
+	{
+		rgba[4 * i] = c[i].r;
+		rgba[4 * i + 1] = c[i].g;
+		rgba[4 * i + 2] = c[i].b;
+		rgba[4 * i + 3] = c[i].a;
+	}
 }
 
 void nbs_solver_advise(nbs_solver* s, double dt) { S(s)->advise(dt); }

@@ -210,16 +210,19 @@ bool nbody_data::resize(size_t s)
 	m_velosites.resize(s);
 	m_mass.resize(s);
 	m_radius.resize(s);
+	m_color.resize(s);
 	m_count = s;
 	return true;
 }
 
-void nbody_data::add_body(const nbvertex_t& r, const nbvertex_t& v, const nbcoord_t& m, const nbcoord_t& radius)
+void nbody_data::add_body(const nbvertex_t& r, const nbvertex_t& v, const nbcoord_t& m, const nbcoord_t& radius,
+                          const nbcolor_t& color)
 {
 	m_vertites.push_back(r);
 	m_velosites.push_back(v);
 	m_mass.push_back(m);
 	m_radius.push_back(radius);
+	m_color.push_back(color);
 	++m_count;
 }
 
@@ -241,6 +244,8 @@ nbcoord_t* nbody_data::get_mass() { return m_mass.data(); }
 const nbcoord_t* nbody_data::get_mass() const { return m_mass.data(); }
 nbcoord_t* nbody_data::get_radius() { return m_radius.data(); }
 const nbcoord_t* nbody_data::get_radius() const { return m_radius.data(); }
+nbcolor_t* nbody_data::get_color() { return m_color.data(); }
+const nbcolor_t* nbody_data::get_color() const { return m_color.data(); }
 size_t nbody_data::get_box_size() const { return m_box_size; }
 void nbody_data::set_box_size(size_t s) { m_box_size = s; }
 size_t nbody_data::get_count() const { return m_count; }
@@ -256,7 +261,7 @@ static size_t round_up(size_t n, size_t factor)
 // nbody/nbody_data.cpp:263-303: a disk galaxy around a 99.9% central mass,
 // drawn with the default-seeded std::mt19937_64.
 void nbody_data::add_galaxy(const nbvertex_t& center, const nbvertex_t& velosity, nbcoord_t radius,
-                            nbcoord_t total_mass, size_t count)
+                            nbcoord_t total_mass, size_t count, const nbcolor_t& color)
 {
 	std::mt19937_64 rng;
 	std::uniform_real_distribution<double> uniform(-radius, radius);
@@ -269,7 +274,7 @@ void nbody_data::add_galaxy(const nbvertex_t& center, const nbvertex_t& velosity
 	nbcoord_t star_mass = (total_mass - black_hole_mass) / static_cast<nbcoord_t>(count);
 	nbcoord_t all_stars_mass = static_cast<nbcoord_t>(count) * star_mass;
 
-	add_body(center, velosity, black_hole_mass);
+	add_body(center, velosity, black_hole_mass, 0, nbcolor_t(0, 1, 0, 1));
 
 	for(size_t n = 0; n != count; ++n)
 	{
@@ -292,7 +297,7 @@ void nbody_data::add_galaxy(const nbvertex_t& center, const nbvertex_t& velosity
 		v.normalize();
 		nbcoord_t effective_mass = std::pow(rlen / radius, 3.0) * all_stars_mass + black_hole_mass;
 		v *= std::sqrt(force(r, center, star_mass, effective_mass).length() * (r - center).length() / star_mass);
-		add_body(r, v + velosity, star_mass);
+		add_body(r, v + velosity, star_mass, 0, color);
 	}
 }
 
@@ -306,8 +311,11 @@ void nbody_data::make_universe(size_t star_count, nbcoord_t sx, nbcoord_t sy, nb
 	nbvertex_t base(radius, 0, 0);
 	nbvertex_t velosity(0, std::sqrt(force(nbvertex_t(), base, galaxy_mass, galaxy_mass).length() *
 	                                 (base).length() / (2 * galaxy_mass)), 0);
-	add_galaxy(center - base, velosity / 3, radius, galaxy_mass, star_count);
-	add_galaxy(center + base, -velosity / 3, radius, galaxy_mass, star_count);
+	const float intensity = 0.8f;
+	add_galaxy(center - base, velosity / 3, radius, galaxy_mass, star_count,
+	           nbcolor_t(intensity * 0.5f, intensity * 0.5f, intensity, 1.0f));
+	add_galaxy(center + base, -velosity / 3, radius, galaxy_mass, star_count,
+	           nbcolor_t(intensity, intensity * 0.5f, intensity * 0.5f, 1.0f));
 	m_box_size = static_cast<size_t>(std::max(std::max(static_cast<nbcoord_t>(1), sx), std::max(sy, sz)));
 }
 
@@ -371,6 +379,7 @@ void nbody_data::clear()
 	m_velosites.clear();
 	m_mass.clear();
 	m_radius.clear();
+	m_color.clear();
 	m_count = 0;
 }
 

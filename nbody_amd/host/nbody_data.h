@@ -25,6 +25,7 @@ class nbody_data
 	std::vector<nbvertex_t> m_velosites;
 	std::vector<nbcoord_t>  m_mass;
 	std::vector<nbcoord_t>  m_radius;
+	std::vector<nbcolor_t>  m_color;
 
 	nbvertex_t  m_initial_impulce;
 	nbvertex_t  m_initial_impulce_moment;
@@ -56,7 +57,8 @@ public:
 	void compute_statistics(nbody_engine* engine, double out[12]);
 
 	bool resize(size_t s);
-	void add_body(const nbvertex_t& r, const nbvertex_t& v, const nbcoord_t& m, const nbcoord_t& radius = 0);
+	void add_body(const nbvertex_t& r, const nbvertex_t& v, const nbcoord_t& m, const nbcoord_t& radius = 0,
+	              const nbcolor_t& color = nbcolor_t());
 	void advise_time(nbcoord_t dt);
 	nbcoord_t get_time() const;
 	void set_time(nbcoord_t t);
@@ -70,6 +72,8 @@ public:
 	const nbcoord_t* get_mass() const;
 	nbcoord_t* get_radius();
 	const nbcoord_t* get_radius() const;
+	nbcolor_t* get_color();
+	const nbcolor_t* get_color() const;
 	size_t get_box_size() const;
 	void set_box_size(size_t s);
 	size_t get_count() const;
@@ -78,7 +82,7 @@ public:
 	void set_quiet(bool quiet);
 
 	void add_galaxy(const nbvertex_t& center, const nbvertex_t& velosity, nbcoord_t radius,
-	                nbcoord_t total_mass, size_t count);
+	                nbcoord_t total_mass, size_t count, const nbcolor_t& color = nbcolor_t());
 	void make_universe(size_t star_count, nbcoord_t sx, nbcoord_t sy, nbcoord_t sz);
 
 	nbvertex_t get_initial_impulce() const;

@@ -108,6 +108,29 @@ bool nbody_engine::compute_statistics(bool, double*)
 	return false;
 }
 
+bool nbody_engine::snapshot_begin(int slot)
+{
+	memory* y = get_y();
+	if(slot < 0 || slot > 1 || y == nullptr)
+	{
+		nbody_log("snapshot_begin: bad slot %d or engine not initialised", slot);
+		return false;
+	}
+	m_snapshot[slot].resize(y->size() / sizeof(nbcoord_t));
+	read_buffer(m_snapshot[slot].data(), y);
+	return true;
+}
+
+const nbcoord_t* nbody_engine::snapshot_state(int slot)
+{
+	if(slot < 0 || slot > 1 || m_snapshot[slot].empty())
+	{
+		nbody_log("snapshot_state: slot %d holds no snapshot", slot);
+		return nullptr;
+	}
+	return m_snapshot[slot].data();
+}
+
 void nbody_engine::advise_compute_count()
 {
 	m_compute_count++;

@@ -20,6 +20,7 @@ class nbody_engine
 {
 	size_t      m_compute_count;
 	e_ode_order m_ode_order;
+	std::vector<nbcoord_t> m_snapshot[2];
 	nbody_engine(const nbody_engine&) = delete;
 	nbody_engine& operator=(const nbody_engine&) = delete;
 public:
@@ -78,6 +79,17 @@ public:
 	//! the current state and return true, or return false to let the caller
 	//! compute them on the host.  No counterpart in the reference interface.
 	virtual bool compute_statistics(bool with_energy, double out[12]);
+
+	//! Frame snapshots for the run loop's dumps (no counterpart in the
+	//! reference interface, whose run loop copies with get_data):
+	//! snapshot_begin(slot) takes a copy of the state y for host slot 0 or 1;
+	//! snapshot_state(slot) returns it -- problem_size() doubles
+	//! [rx ry rz vx vy vz] -- once the copy is complete, or NULL on failure.
+	//! The defaults copy at once with read_buffer; a device engine lets the
+	//! copy to the host overlap the steps that follow.  The two calls of one
+	//! slot may come from different threads (snapshot_state from a writer).
+	virtual bool snapshot_begin(int slot);
+	virtual const nbcoord_t* snapshot_state(int slot);
 
 	void advise_compute_count();
 	size_t get_compute_count() const;

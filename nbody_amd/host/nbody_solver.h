@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "nbody_butcher_table.h"
+#include "nbody_data_stream.h"
 #include "nbody_engine.h"
 #include "nbody_params.h"
 
@@ -35,9 +36,15 @@ public:
 	nbcoord_t get_max_step() const;
 	void set_clamp_to_box(bool clamp);
 	void add_check_visitor(std::function<void(nbody_data*)> v);
-	//! Integrate to max_time in steps of max_step (nbody/nbody_solver.cpp:55-105);
-	//! statistics every check_dt.  Dumps are not part of this build (dump_dt ignored).
-	int run(nbody_data* data, nbcoord_t max_time, nbcoord_t dump_dt, nbcoord_t check_dt);
+	//! Integrate to max_time in steps of max_step (nbody/nbody_solver.cpp:55-105):
+	//! statistics every check_dt; with a stream and dump_dt > 0 a frame at the
+	//! start (when the time is <= 0) and every dump_dt.  Frames are written by a
+	//! second thread from snapshots the engine takes between steps, so the
+	//! steps go on while a frame is copied and written; on return every frame
+	//! is written and `data` holds the last one.  -1 when a frame could not be
+	//! taken or written.
+	int run(nbody_data* data, nbody_data_stream* stream, nbcoord_t max_time, nbcoord_t dump_dt,
+	        nbcoord_t check_dt);
 
 	virtual const char* type_name() const = 0;
 	virtual void advise(nbcoord_t dt) = 0;

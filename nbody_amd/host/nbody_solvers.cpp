@@ -3,9 +3,14 @@
 // reference method cited beside it (paths relative to the drons/nbody root).
 #include "nbody_solver.h"
 
+#include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
 #include <limits>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "nbody_log.h"
 
@@ -35,11 +40,159 @@ void nbody_solver::set_clamp_to_box(bool clamp) { m_clamp_to_box = clamp; }
 void nbody_solver::add_check_visitor(std::function<void(nbody_data*)> v) { m_check_visitors.push_back(v); }
 
 // nbody/nbody_solver.cpp:55-105 (stream dumps are outside this build)
-int nbody_solver::run(nbody_data* data, nbcoord_t max_time, nbcoord_t dump_dt, nbcoord_t check_dt)
+namespace {
+
+// Frames of a run's dumps, written by a second thread: the run loop has the
+// engine take a snapshot of the state (nbody_engine::snapshot_begin) and queues
+// it; the writer waits for the copy, turns the SoA state into the AoS frame of
+// the stream format and appends it, while the loop goes on with the next steps.
+// The two snapshot slots alternate; a dump waits until the frame that last
+// used its slot is written.
+class frame_writer
 {
-	(void)dump_dt;
+	struct job
+	{
+		int       slot;
+		size_t    step;
+		nbcoord_t time;
+	};
+	nbody_engine*           m_engine;
+	nbody_data_stream*      m_stream;
+	const nbody_data*       m_data;
+	std::mutex              m_mu;
+	std::condition_variable m_cv;
+	std::deque<job>         m_queue;
+	bool                    m_busy[2] = {false, false};
+	bool                    m_stop = false;
+	bool                    m_failed = false;
+	int                     m_next_slot = 0;
+	bool                    m_written = false;
+	std::vector<nbvertex_t> m_pos, m_vel;       // the last frame written
+	std::thread             m_thread;
+
+	void loop()
+	{
+		const size_t n = m_data->get_count();
+		for(;;)
+		{
+			job j;
+			{
+				std::unique_lock<std::mutex> lock(m_mu);
+				m_cv.wait(lock, [this] { return m_stop || !m_queue.empty(); });
+				if(m_queue.empty())
+				{
+					return;
+				}
+				j = m_queue.front();
+				m_queue.pop_front();
+			}
+			bool ok = false;
+			const nbcoord_t* y = m_engine->snapshot_state(j.slot);
+			if(y != nullptr)
+			{
+				m_pos.resize(n);
+				m_vel.resize(n);
+				for(size_t i = 0; i < n; ++i)
+				{
+					m_pos[i] = nbvertex_t(y[i], y[n + i], y[2 * n + i]);
+					m_vel[i] = nbvertex_t(y[3 * n + i], y[4 * n + i], y[5 * n + i]);
+				}
+				ok = m_stream->write_frame(m_data, j.step, j.time, m_pos.data(), m_vel.data()) == 0;
+			}
+			std::lock_guard<std::mutex> lock(m_mu);
+			m_busy[j.slot] = false;
+			m_failed = m_failed || !ok;
+			m_written = m_written || ok;
+			m_cv.notify_all();
+		}
+	}
+
+public:
+	frame_writer(nbody_engine* e, nbody_data_stream* stream, const nbody_data* data) :
+		m_engine(e), m_stream(stream), m_data(data)
+	{
+		m_thread = std::thread([this] { loop(); });
+	}
+
+	~frame_writer()
+	{
+		stop();
+	}
+
+	void stop()
+	{
+		{
+			std::lock_guard<std::mutex> lock(m_mu);
+			m_stop = true;
+		}
+		m_cv.notify_all();
+		if(m_thread.joinable())
+		{
+			m_thread.join();
+		}
+	}
+
+	//! Queue a frame of the current state; -1 if an earlier frame failed or the
+	//! snapshot cannot be taken.
+	int dump(size_t step, nbcoord_t time)
+	{
+		const int slot = m_next_slot;
+		m_next_slot ^= 1;
+		{
+			std::unique_lock<std::mutex> lock(m_mu);
+			m_cv.wait(lock, [this, slot] { return m_failed || !m_busy[slot]; });
+			if(m_failed)
+			{
+				return -1;
+			}
+		}
+		if(!m_engine->snapshot_begin(slot))
+		{
+			return -1;
+		}
+		std::lock_guard<std::mutex> lock(m_mu);
+		m_busy[slot] = true;
+		m_queue.push_back(job{slot, step, time});
+		m_cv.notify_all();
+		return 0;
+	}
+
+	//! Write what is queued, stop the thread and put the last frame into data.
+	int finish(nbody_data* data)
+	{
+		stop();
+		if(m_written)
+		{
+			std::copy(m_pos.begin(), m_pos.end(), data->get_vertites());
+			std::copy(m_vel.begin(), m_vel.end(), data->get_velosites());
+		}
+		return m_failed ? -1 : 0;
+	}
+};
+
+}  // namespace
+
+int nbody_solver::run(nbody_data* data, nbody_data_stream* stream, nbcoord_t max_time, nbcoord_t dump_dt,
+                      nbcoord_t check_dt)
+{
 	nbcoord_t dt = get_max_step();
 	nbcoord_t last_check = data->get_time();
+	nbcoord_t last_dump = last_check;
+	std::unique_ptr<frame_writer> writer;
+	if(stream != nullptr && dump_dt > 0)
+	{
+		writer.reset(new frame_writer(m_engine, stream, data));
+		if(last_dump <= 0)
+		{
+			if(writer->dump(data->get_step(), data->get_time()) != 0)
+			{
+				writer->finish(data);
+				nbody_log("run: can't write the first frame");
+				return -1;
+			}
+			last_dump = data->get_time();
+		}
+	}
 	while(data->get_time() < max_time)
 	{
 		if(m_clamp_to_box)
@@ -57,6 +210,21 @@ int nbody_solver::run(nbody_data* data, nbcoord_t max_time, nbcoord_t dump_dt, n
 			}
 			last_check = t;
 		}
+		if(writer && t >= last_dump + dump_dt - dt * 0.1)
+		{
+			if(writer->dump(data->get_step(), t) != 0)
+			{
+				writer->finish(data);
+				nbody_log("run: can't write the frame of step %zu", data->get_step());
+				return -1;
+			}
+			last_dump = t;
+		}
+	}
+	if(writer && writer->finish(data) != 0)
+	{
+		nbody_log("run: a frame could not be written");
+		return -1;
 	}
 	return 0;
 }

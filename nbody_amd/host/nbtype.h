@@ -38,6 +38,16 @@ struct nbvertex_t
 	void normalize() { *this = *this / length(); }
 };
 
+// Display colour of a body, RGBA in [0, 1] (nbcolor_t of the reference,
+// vertex4<float>, nbody/nbtype.h:31): 16 bytes, as the .col stream file holds it.
+struct nbcolor_t
+{
+	float r, g, b, a;
+
+	nbcolor_t() : r(1), g(1), b(1), a(1) {}
+	nbcolor_t(float r_, float g_, float b_, float a_) : r(r_), g(g_), b(b_), a(a_) {}
+};
+
 #define NBODY_DATA_BLOCK_SIZE 64
 
 namespace nbody {

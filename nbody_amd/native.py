@@ -145,7 +145,27 @@ HOST_API = {
     "nbs_solver_type_name": (cstr, [vp]),
     "nbs_solver_set_engine": (None, [vp, vp]),
     "nbs_solver_set_time_step": (None, [vp, dbl, dbl]),
-    "nbs_solver_run": (i32, [vp, vp, dbl, dbl, dbl]),
+    "nbs_solver_run": (i32, [vp, vp, vp, dbl, dbl, dbl]),
+    "nbs_stream_create": (vp, []),
+    "nbs_stream_destroy": (None, [vp]),
+    "nbs_stream_open": (i32, [vp, cstr, C.c_int64, vp]),
+    "nbs_stream_write": (i32, [vp, vp]),
+    "nbs_stream_close": (None, [vp]),
+    "nbs_stream_reader_create": (vp, []),
+    "nbs_stream_reader_destroy": (None, [vp]),
+    "nbs_stream_reader_load": (i32, [vp, cstr]),
+    "nbs_stream_reader_close": (None, [vp]),
+    "nbs_stream_reader_frame_count": (sz, [vp]),
+    "nbs_stream_reader_steps_count": (sz, [vp]),
+    "nbs_stream_reader_max_time": (dbl, [vp]),
+    "nbs_stream_reader_seek": (i32, [vp, sz]),
+    "nbs_stream_reader_current_frame": (sz, [vp]),
+    "nbs_stream_reader_body_count": (sz, [vp]),
+    "nbs_stream_reader_coord_size": (sz, [vp]),
+    "nbs_stream_reader_box_size": (dbl, [vp]),
+    "nbs_stream_reader_last_file_n": (sz, [vp]),
+    "nbs_stream_reader_read": (i32, [vp, vp]),
+    "nbs_data_get_color": (None, [vp, C.POINTER(C.c_float)]),
     "nbs_solver_advise": (None, [vp, dbl]),
     "nbs_solver_print_info": (None, [vp]),
     "nbs_solver_rk_configure": (i32, [vp, sz, sz, sz, dbl]),
