@@ -49,6 +49,7 @@ typedef enum nbh_tree_layout {
 typedef struct nbh_ctx nbh_ctx;     /* one device + one stream */
 typedef struct nbh_tree nbh_tree;   /* device-resident heap kd-tree */
 typedef struct nbh_mlp nbh_mlp;     /* correction network (parameters + Adam state) */
+typedef struct nbh_snapshot nbh_snapshot;   /* staging for asynchronous state copies */
 
 /* ---- context, errors ---------------------------------------------------- */
 const char* nbh_last_error(void);
@@ -70,6 +71,24 @@ nbh_status nbh_free(nbh_ctx* ctx, void* dptr);
 nbh_status nbh_memcpy_h2d(nbh_ctx* ctx, void* dst, const void* src, size_t bytes);
 nbh_status nbh_memcpy_d2h(nbh_ctx* ctx, void* dst, const void* src, size_t bytes);
 nbh_status nbh_memcpy_d2d(nbh_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- asynchronous state snapshots for frame dumps ------------------------
+   The reference dumps a frame by copying the state to the host before the next
+   step (nbody/nbody_solver.cpp:93-103 through nbody_engine_cuda::get_data,
+   nbody_engine_cuda.cpp:141-165).  A snapshot copies `bytes` from a device
+   buffer to a device staging slot on the context's stream (so it sees the
+   state of the work queued so far, and later work may overwrite the source),
+   then from staging to pinned host memory on a second stream of default
+   priority, overlapping what the context's stream runs next.  slots: 1 or 2
+   independent slots.  nbh_snapshot_wait blocks until the host copy of `slot`
+   is complete and returns its host address (valid until the next
+   nbh_snapshot_begin of that slot or nbh_snapshot_destroy); it may be called
+   from another host thread than nbh_snapshot_begin, but not concurrently with
+   calls on the same slot. */
+nbh_status nbh_snapshot_create(nbh_ctx* ctx, size_t bytes, int slots, nbh_snapshot** snap);
+nbh_status nbh_snapshot_destroy(nbh_snapshot* snap);
+nbh_status nbh_snapshot_begin(nbh_ctx* ctx, nbh_snapshot* snap, int slot, const void* src);
+nbh_status nbh_snapshot_wait(nbh_ctx* ctx, nbh_snapshot* snap, int slot, const void** host);
 
 /* ---- FP64 vector ops over n doubles ------------------------------------- */
 /* fill_buffer (nbody_engine_cuda_impl.cu:716-721) */

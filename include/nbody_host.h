@@ -33,6 +33,11 @@ size_t      nbs_log_count(void);
 const char* nbs_log_last(void);
 void        nbs_log_quiet(int quiet);
 
+/* ---- front-end arguments (nbody/nbody_arg_parser.cpp:5-28) ----
+   The --key=value arguments of argv[1..argc) as "key=value;..." in key order
+   into out (truncated to out_len - 1 characters); returns the full length. */
+size_t      nbs_parse_arguments(int argc, const char* const* argv, char* out, size_t out_len);
+
 /* ---- bodies (nbody_data) ---- */
 nbs_data* nbs_data_create(void);
 void      nbs_data_destroy(nbs_data* d);

@@ -2,6 +2,7 @@
 
     nbody_amd/libnbody_hip.so   gfx950 kernels + C ABI (include/nbody_hip.h), hipcc
     nbody_amd/libnbody.so       host layer: engines, solvers, data (include/nbody_host.h), g++
+    nbody_amd/nbody-simulation  command-line front end over libnbody.so, g++
     oracle/liboracle.so         CPU restatement used by the tests as the checker, gcc
     oracle/liboracle_engine.so  the restatement behind the nbody_engine interface (tests only)
 
@@ -92,6 +93,13 @@ def build(jobs: int = 8, oracle: bool = True, verbose: bool = False) -> dict:
           f"-L{PKG}", "-lnbody_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(str(lib_host) + ".tmp", lib_host)
     out = {"libnbody_hip": lib_hip, "libnbody": lib_host}
+    # command-line front ends, one executable per source
+    for src in sorted((PKG / "cli").glob("*.cpp")):
+        exe = PKG / src.stem.replace("_", "-")
+        _run(["g++", *[f for f in HOST_FLAGS if f != "-fPIC"], "-o", str(exe) + ".tmp", str(src), f"-L{PKG}",
+              "-lnbody", "-lnbody_hip", "-lpthread", "-Wl,-rpath,$ORIGIN"])
+        os.replace(str(exe) + ".tmp", exe)
+        out[src.stem] = exe
     if oracle:
         lib_or = ORACLE / "liboracle.so"
         _run(["gcc", "-shared", "-fopenmp", "-o", str(lib_or) + ".tmp", str(BUILD / "nbody_oracle.o"), "-lm"])

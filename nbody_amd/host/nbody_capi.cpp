@@ -4,6 +4,7 @@
 #include <cstring>
 #include <vector>
 
+#include "nbody_arg_parser.h"
 #include "nbody_engine_hip.h"
 #include "nbody_engines.h"
 #include "nbody_generators.h"
@@ -258,6 +259,18 @@ void nbs_solver_set_time_step(nbs_solver* s, double min_step, double max_step) {
 int nbs_solver_run(nbs_solver* s, nbs_data* d, nbs_stream* stream, double max_time, double dump_dt, double check_dt)
 {
 	return S(s)->run(D(d), reinterpret_cast<nbody_data_stream*>(stream), max_time, dump_dt, check_dt);
+}
+
+size_t nbs_parse_arguments(int argc, const char* const* argv, char* out, size_t out_len)
+{
+	const std::string text = nbody_parse_arguments(argc, argv).str();
+	if(out != nullptr && out_len > 0)
+	{
+		const size_t k = text.size() < out_len - 1 ? text.size() : out_len - 1;
+		std::memcpy(out, text.data(), k);
+		out[k] = '\0';
+	}
+	return text.size();
 }
 
 nbs_stream* nbs_stream_create(void) { return reinterpret_cast<nbs_stream*>(new nbody_data_stream()); }

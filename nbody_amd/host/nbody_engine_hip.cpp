@@ -38,12 +38,14 @@ nbody_engine_hip::nbody_engine_hip() :
 	m_data(nullptr),
 	m_mass(nullptr),
 	m_y(nullptr),
+	m_snap(nullptr),
 	m_error(false)
 {
 }
 
 nbody_engine_hip::~nbody_engine_hip()
 {
+	nbh_snapshot_destroy(m_snap);
 	delete m_mass;
 	delete m_y;
 	if(m_ctx != nullptr)
@@ -109,6 +111,8 @@ bool nbody_engine_hip::init(nbody_data* data)
 	}
 	m_data = data;
 	size_t count = data->get_count();
+	nbh_snapshot_destroy(m_snap);
+	m_snap = nullptr;
 	delete m_mass;
 	delete m_y;
 	m_mass = create_buffer(sizeof(nbcoord_t) * count);
@@ -162,6 +166,33 @@ size_t nbody_engine_hip::problem_size() const
 nbody_engine::memory* nbody_engine_hip::get_y()
 {
 	return m_y;
+}
+
+bool nbody_engine_hip::snapshot_begin(int slot)
+{
+	if(m_y == nullptr || slot < 0 || slot > 1)
+	{
+		nbody_log("hip: snapshot_begin: bad slot %d or engine not initialised", slot);
+		return false;
+	}
+	if(m_snap == nullptr && !check(nbh_snapshot_create(m_ctx, m_y->size(), 2, &m_snap), "snapshot allocation"))
+	{
+		return false;
+	}
+	return check(nbh_snapshot_begin(m_ctx, m_snap, slot, m_y->data()), "snapshot");
+}
+
+// Called from the run loop's writer thread: no error latching here, the
+// failure reaches the run loop as a NULL state.
+const nbcoord_t* nbody_engine_hip::snapshot_state(int slot)
+{
+	const void* host = nullptr;
+	if(m_snap == nullptr || nbh_snapshot_wait(m_ctx, m_snap, slot, &host) != NBH_OK)
+	{
+		nbody_log("hip: snapshot of slot %d not available: %s", slot, nbh_last_error());
+		return nullptr;
+	}
+	return static_cast<const nbcoord_t*>(host);
 }
 
 void nbody_engine_hip::advise_time(const nbcoord_t& dt)

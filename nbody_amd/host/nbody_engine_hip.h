@@ -60,6 +60,7 @@ protected:
 	nbody_data*   m_data;
 	smemory*      m_mass;
 	smemory*      m_y;
+	nbh_snapshot* m_snap;       // frame snapshots (created on the first dump)
 	bool          m_error;
 
 	bool check(nbh_status s, const char* what);
@@ -90,6 +91,10 @@ public:
 	void set_step(size_t s) override;
 	void fcompute(const nbcoord_t& t, const memory* y, memory* f) override;
 	void clamp(memory* y, nbcoord_t b) override;
+	//! Device-side snapshots: a staging copy on the engine's stream, then a
+	//! copy to pinned host memory that overlaps the following steps.
+	bool snapshot_begin(int slot) override;
+	const nbcoord_t* snapshot_state(int slot) override;
 
 	smemory* create_buffer(size_t bytes) override;
 	void free_buffer(memory* m) override;

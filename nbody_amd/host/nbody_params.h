@@ -38,6 +38,17 @@ public:
 	}
 
 	void set(const std::string& key, const std::string& value) { m_values[key] = value; }
+	// "key=value" pairs joined by ';' in key order (the text parse() reads).
+	std::string str() const
+	{
+		std::string out;
+		for(const auto& kv : m_values)
+		{
+			if(!out.empty()) { out += ';'; }
+			out += kv.first + '=' + kv.second;
+		}
+		return out;
+	}
 	bool contains(const std::string& key) const { return m_values.count(key) != 0; }
 
 	std::string value(const std::string& key, const std::string& def) const

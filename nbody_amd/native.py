@@ -47,6 +47,10 @@ HIP_API = {
     "nbh_fmaddn": (i32, [vp, vp, vp, PPD, PD, sz, sz]),
     "nbh_fmaddn_corr": (i32, [vp, vp, vp, PPD, PD, sz, sz]),
     "nbh_maxabs": (i32, [vp, vp, sz, PD]),
+    "nbh_snapshot_create": (i32, [vp, sz, i32, C.POINTER(vp)]),
+    "nbh_snapshot_destroy": (i32, [vp]),
+    "nbh_snapshot_begin": (i32, [vp, vp, i32, vp]),
+    "nbh_snapshot_wait": (i32, [vp, vp, i32, C.POINTER(vp)]),
     "nbh_clamp": (i32, [vp, vp, sz, dbl]),
     "nbh_fcompute_direct": (i32, [vp, vp, vp, vp, sz, i32, i32]),
     "nbh_direct_sample": (i32, [vp, vp, vp, sz, vp, sz, vp]),
@@ -146,6 +150,7 @@ HOST_API = {
     "nbs_solver_set_engine": (None, [vp, vp]),
     "nbs_solver_set_time_step": (None, [vp, dbl, dbl]),
     "nbs_solver_run": (i32, [vp, vp, vp, dbl, dbl, dbl]),
+    "nbs_parse_arguments": (sz, [i32, C.POINTER(C.c_char_p), C.c_char_p, sz]),
     "nbs_stream_create": (vp, []),
     "nbs_stream_destroy": (None, [vp]),
     "nbs_stream_open": (i32, [vp, cstr, C.c_int64, vp]),
