@@ -1,5 +1,9 @@
 // nbody_butcher_table.h -- Runge-Kutta tableau (a, b1, b2, c) with the
 // accessors of the reference's nbody_butcher_table (nbody/nbody_butcher_table.h:7-22).
+//
+// Licence: this file restates code of drons/nbody (the reference files cited
+// in it), which is distributed under the GNU Lesser General Public License,
+// version 3; this file is distributed under the same licence.
 #pragma once
 
 #include <string>

@@ -1,5 +1,9 @@
 // nbody_data.cpp -- see nbody_data.h.  Reference counterparts are cited per
 // function (paths relative to the drons/nbody repository root).
+//
+// Licence: this file restates code of drons/nbody (the reference files cited
+// in it), which is distributed under the GNU Lesser General Public License,
+// version 3; this file is distributed under the same licence.
 #include "nbody_data.h"
 
 #include <algorithm>

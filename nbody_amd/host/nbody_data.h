@@ -4,6 +4,10 @@
 // that the engines, solvers and tests use: state arrays, time/step counters,
 // the force law, the P/L/E/V checks, the G1/SI/ADK/Zeno loaders and the G1
 // writer, and make_universe.  Qt types are replaced by the standard library.
+//
+// Licence: this file restates code of drons/nbody (the reference files cited
+// in it), which is distributed under the GNU Lesser General Public License,
+// version 3; this file is distributed under the same licence.
 #pragma once
 
 #include <cstdint>

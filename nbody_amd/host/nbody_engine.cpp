@@ -2,6 +2,10 @@
 // decompositions of the multi-term ops into single-term ops, with the
 // semantics of nbody/nbody_engine.cpp:21-113 (zero coefficients skipped;
 // fmaddn with b == NULL starts from 0; fmaddn_corr ignores corr by default).
+//
+// Licence: this file restates code of drons/nbody (the reference files cited
+// in it), which is distributed under the GNU Lesser General Public License,
+// version 3; this file is distributed under the same licence.
 #include "nbody_engine.h"
 
 #include "nbody_log.h"

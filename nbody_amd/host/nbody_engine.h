@@ -4,6 +4,10 @@
 // reference's nbody_engine (nbody/nbody_engine.h:16-96): solvers see only
 // opaque byte-sized `memory` buffers; a wrong buffer type, a null coefficient
 // array or csize > b.size() is logged and the call returns without effect.
+//
+// Licence: this file restates code of drons/nbody (the reference files cited
+// in it), which is distributed under the GNU Lesser General Public License,
+// version 3; this file is distributed under the same licence.
 #pragma once
 
 #include <vector>

@@ -5,6 +5,10 @@
 // factory nbody/nbody_solvers.cpp:17-110).  They call the engine only through
 // the nbody_engine interface, so the `hip` and `hip_bh` engines run under
 // them unchanged.
+//
+// Licence: this file restates code of drons/nbody (the reference files cited
+// in it), which is distributed under the GNU Lesser General Public License,
+// version 3; this file is distributed under the same licence.
 #pragma once
 
 #include <functional>

@@ -1,6 +1,10 @@
 // nbody_solvers.cpp -- solver implementations (see nbody_solver.h).  Each
 // advise() issues the same engine calls, with the same coefficients, as the
 // reference method cited beside it (paths relative to the drons/nbody root).
+//
+// Licence: this file restates code of drons/nbody (the reference files cited
+// in it), which is distributed under the GNU Lesser General Public License,
+// version 3; this file is distributed under the same licence.
 #include "nbody_solver.h"
 
 #include <algorithm>
