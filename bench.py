@@ -179,7 +179,25 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             params["bh_multipole"] = "quadrupole"
             learned = False
         train_info = None
-        if learned:
+        if learned == "two_level":
+            # the two-level local walk with the network's correction of the
+            # super-groups' far fields, trained on another realisation (seed 7)
+            params.update({"bh_multipole": "quadrupole", "bh_far_field": "local"})
+            tdata = nbody_data()
+            tdata.set_quiet(True)
+            tdata.generate("plummer", n, 7)
+            ty, tm = _state_y(tdata)
+            near = float(params.get("bh_quad_near", 5.0))
+            params["bh_quad_near"] = near
+            log(f"BH: training the two-level correction at ratio {ratio}, near {near}")
+            t0 = time.perf_counter()
+            corr = correction.train_two_level(ty, tm, ratio, near_k=near, max_samples=2_000_000, epochs=40)
+            train_info = dict(corr.stats, seconds=time.perf_counter() - t0, near_k=near,
+                              train_state="Plummer seed 7")
+            path = os.path.join(tmpdir, f"corr2_{ratio:g}.nbhmlp")
+            correction.save(path, corr)
+            params["learned_correction"] = path
+        elif learned:
             tdata = nbody_data()
             tdata.set_quiet(True)
             tdata.generate("plummer", n, 7)
@@ -270,10 +288,10 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             "rms_max_over_seeds": max(v["rms"] for v in errs.values()),
             "p99_max_over_seeds": max(v["p99"] for v in errs.values()),
         }
-        if learned or "bh_multipole" in params:
+        if train_info is not None or "bh_multipole" in params:
             run["far_cells_per_group"] = walk_counts[3] / (n // 64)
             run["near_cells_per_group"] = walk_counts[4] / (n // 64)
-        if learned:
+        if train_info is not None:
             run["training"] = train_info
         out["runs"].append(run)
         log(f"BH {label}: {steps / wall:.2f} steps/s, rms err {run['force_rel_err']['rms']:.2e} "
@@ -299,6 +317,28 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
     # cancellation), and its speed relative to the plain run
     def plain(r):
         return r["multipole"] == "monopole" and not r["variant"].startswith("learned")
+
+    learned_runs = [r for r in out["runs"] if r["variant"].startswith("learned")]
+    base_runs = [r for r in out["runs"] if r["variant"] == "plain theta=0.5"]
+    if learned_runs and base_runs:
+        lr_, b_ = learned_runs[0], base_runs[0]
+        same = [r for r in out["runs"] if not r["variant"].startswith("learned") and r["ratio"] == lr_["ratio"]
+                and r["far_field"] == "local" and r["quad_near"] == lr_["quad_near"]]
+        out["learned_gate"] = {
+            "learned": lr_["variant"], "steps_per_s": lr_["steps_per_s"],
+            "speedup_vs_plain_theta_0.5": lr_["steps_per_s"] / b_["steps_per_s"],
+            "rms_max_over_seeds": lr_["rms_max_over_seeds"], "p99_max_over_seeds": lr_["p99_max_over_seeds"],
+            "plain_theta_0.5_rms_max_over_seeds": b_["rms_max_over_seeds"],
+            "plain_theta_0.5_p99_max_over_seeds": b_["p99_max_over_seeds"],
+            "meets_gate": bool(lr_["steps_per_s"] >= 2.0 * b_["steps_per_s"]
+                               and lr_["rms_max_over_seeds"] <= b_["rms_max_over_seeds"]
+                               and lr_["p99_max_over_seeds"] <= b_["p99_max_over_seeds"]),
+            "without_network": ({"variant": same[0]["variant"], "steps_per_s": same[0]["steps_per_s"],
+                                 "rms_max_over_seeds": same[0]["rms_max_over_seeds"],
+                                 "rms_by_seed": {k: v["rms"] for k, v in same[0]["force_rel_err_by_seed"].items()}}
+                                if same else None),
+            "rms_by_seed": {k: v["rms"] for k, v in lr_["force_rel_err_by_seed"].items()},
+        }
     eq = []
     for p in out["runs"]:
         if not plain(p):
@@ -501,7 +541,7 @@ def bh_variants(learned: bool = True):
          ("quadrupole theta=0.9, local far field, near 4", 1.0 / 0.9, "quadrupole",
           {**local, "bh_quad_near": 4.0})]
     if learned:
-        v.append(("learned theta=0.8", 1.25, True))
+        v.append(("learned theta=0.8, two-level local walk, near 5", 1.25, "two_level", {"bh_quad_near": 5.0}))
     return v
 
 

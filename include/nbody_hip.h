@@ -195,6 +195,31 @@ nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const
    called; a failed allocation leaves none of them allocated.
    Not in the reference. */
 nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* tree, const double* y, double* f, float near_k);
+/* nbh_bh_fcompute_group_local2 with a learned correction of the super-groups'
+   shared far fields.  The super-group walk also lists each super-group's far
+   cells (up to 8192; further ones keep the plain expansion), and one wave per
+   super-group evaluates the network (nbh_mlp_*, BF16 on the matrix cores) on
+   them, 16 cells per tile: from the invariants of (super box centre p, cell)
+   it predicts the cell's field remainder after the monopole and quadrupole
+   terms -- R(p) and J = dR/dp through the basis of nbh_mlp_forward, with
+   R / w and kJac J d / w as outputs, w = M rho^3 / d^2 -- and adds them to the
+   super-group's expansion (field and Jacobian), which every body of the
+   super-group applies to first order in its offset from p.  The far-cell lists
+   take n / 1024 * 8192 * 4 bytes, allocated on first use.  Not in the reference. */
+nbh_status nbh_bh_fcompute_group_local2_mlp(nbh_ctx* ctx, nbh_tree* tree, const nbh_mlp* net, const double* y,
+                                            double* f, float near_k);
+/* Training samples of nbh_bh_fcompute_group_local2_mlp: runs the super-group
+   walk (near_k as for the walk) and, for the super-groups listed in the host
+   array supers[n_supers], writes every listed far cell's features x[16],
+   basis[16] and exact target t[9] (FP64 field of the cell's bodies at the
+   super box centre minus its monopole and quadrupole terms, and the same for
+   the Jacobian, scaled as above, rounded to FP32) into device arrays of
+   `capacity` rows; cells (device, may be NULL) receives the (super-group,
+   heap node) pair of each row; *count = rows found (rows beyond capacity are
+   not written).  Synchronous.  Not in the reference. */
+nbh_status nbh_bh_super_samples(nbh_ctx* ctx, nbh_tree* tree, float near_k, const uint32_t* supers,
+                                size_t n_supers, float* x, float* basis, float* target, uint32_t* cells,
+                                size_t capacity, size_t* count);
 /* List capacity per super-group of nbh_bh_fcompute_group_local2 (default 32768
    entries; lists take n / 1024 * cap * 4 bytes).  Super-groups with longer lists
    walk in one level.  Frees the lists already allocated and allocates them at

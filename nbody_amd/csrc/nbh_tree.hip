@@ -83,6 +83,9 @@ struct nbh_tree {
 	int*      two_lists = nullptr;
 	int*      two_count = nullptr;
 	float*    two_coef = nullptr;
+	// the super-groups' far cells for the learned correction (allocated on first use)
+	int*      far_list = nullptr;
+	int*      far_count = nullptr;
 	double    ratio = 0.0;         // opening ratio of the last build / refit
 	bool      built = false;
 };
@@ -1192,6 +1195,10 @@ struct TwoLevel {
 	int*   count = nullptr;   // [super] list length (> cap: overflowed)
 	float* coef = nullptr;    // [super][34] expansion about the super box centre (local_expansion layout)
 	int    cap = 0;
+	// optional: the super-group's far cells, for the learned correction
+	int*   far_list = nullptr;   // [super][far_cap] far nodes, in walk order
+	int*   far_count = nullptr;  // [super] far cells found (> far_cap: the rest are not listed)
+	int    far_cap = 0;
 };
 
 // Next 64 listed nodes from position pos, and their records.
@@ -1468,6 +1475,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 	for(int k = 0; k < 34; ++k) { acc[k] = 0.0f; }
 	unsigned long long vis = 0, nfar = 0;
 	int lcount = 0;
+	int nfar_listed = 0;
 	bool overflow = false;
 	int sp = 1;
 	int fhead = 0, ftail = 0;
@@ -1525,7 +1533,13 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 			const int slot = (ftail + __popcll(bfar & lt)) & (kFarRing - 1);
 			farq[slot] = make_float4(ux, uy, uz, static_cast<float>(m));
 			farn[slot] = node;
+			if(two.far_list != nullptr)
+			{
+				const int fslot = nfar_listed + __popcll(bfar & lt);
+				if(fslot < two.far_cap) { two.far_list[static_cast<size_t>(si) * two.far_cap + fslot] = node; }
+			}
 		}
+		nfar_listed += __popcll(bfar);
 		if(tolist)
 		{
 			const int slot = lcount + __popcll(blist & lt);
@@ -1553,6 +1567,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 		float* c = two.coef + 34 * static_cast<size_t>(si);
 		for(int k = 0; k < 34; ++k) { c[k] = acc[k]; }
 		two.count[si] = overflow || lcount > two.cap ? two.cap + 1 : lcount;
+		if(two.far_count != nullptr) { two.far_count[si] = nfar_listed; }
 	}
 	for(int off = 32; off > 0; off >>= 1)
 	{
@@ -1659,20 +1674,55 @@ k_band_collect(const double4* __restrict__ xyzr, const double* __restrict__ bmin
 	}
 }
 
-// One wave per sample: the exact FP64 field of the cell's bodies at the group
-// centre p and its Jacobian, minus those of the monopole, written as features
-// x[16], basis b[16] and target t[9] = [R / scale, kJac J d / scale] (J as
-// xx yy zz xy xz yz).
+// Quadrupole term of a cell's field at p and its Jacobian, FP64: Q the second
+// central moments (xx yy zz xy xz yz) about the mass centre cm; with
+// r = cm - p, d = |r|, Q' = 3 Q - tr(Q) I and s = r.Q'.r,
+//   g = (2.5 s / d^2 r - Q' r) / d^5,
+//   J = dg/dp = -(5 (r (Q'r)' + (Q'r) r') + 2.5 s I) / d^7 + 17.5 s r r' / d^9 + Q' / d^5
+// (the terms quad_expansion evaluates in FP32); out[9] = g (3), J (6).
+__device__ inline void quad_field64(double rx, double ry, double rz, const double* Q, double* out)
+{
+	const double tr = Q[0] + Q[1] + Q[2];
+	const double qxx = 3.0 * Q[0] - tr, qyy = 3.0 * Q[1] - tr, qzz = 3.0 * Q[2] - tr;
+	const double qxy = 3.0 * Q[3], qxz = 3.0 * Q[4], qyz = 3.0 * Q[5];
+	const double Qx = qxx * rx + qxy * ry + qxz * rz;
+	const double Qy = qxy * rx + qyy * ry + qyz * rz;
+	const double Qz = qxz * rx + qyz * ry + qzz * rz;
+	const double sq = rx * Qx + ry * Qy + rz * Qz;
+	const double d2 = rx * rx + ry * ry + rz * rz;
+	const double d = sqrt(d2);
+	const double i5 = 1.0 / (d2 * d2 * d);
+	const double i7 = i5 / d2;
+	const double c = 2.5 * sq * i7;
+	out[0] = c * rx - Qx * i5;
+	out[1] = c * ry - Qy * i5;
+	out[2] = c * rz - Qz * i5;
+	const double a = 5.0 * i7;
+	const double e = 17.5 * sq * i7 / d2;
+	out[3] = -(a * (2.0 * rx * Qx) + c - e * rx * rx - qxx * i5);
+	out[4] = -(a * (2.0 * ry * Qy) + c - e * ry * ry - qyy * i5);
+	out[5] = -(a * (2.0 * rz * Qz) + c - e * rz * rz - qzz * i5);
+	out[6] = -(a * (rx * Qy + ry * Qx) - e * rx * ry - qxy * i5);
+	out[7] = -(a * (rx * Qz + rz * Qx) - e * rx * rz - qxz * i5);
+	out[8] = -(a * (ry * Qz + rz * Qy) - e * ry * rz - qyz * i5);
+}
+
+// One wave per sample (centre node centre_base + pairs[2s], cell pairs[2s+1]):
+// the exact FP64 field of the cell's bodies at the centre node's box centre p
+// and its Jacobian, minus those of the monopole (order 1) or of the monopole
+// and the quadrupole term (order 2), written as features x[16], basis b[16] and
+// target t[9] = [R / w, kJac J d / w] (J as xx yy zz xy xz yz), where w is the
+// scale of cell_features (M rho^2 / d^2) times rho^(order - 1).
 __global__ void __launch_bounds__(64)
 k_band_residual(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
-                const double* __restrict__ bmax, const double* __restrict__ quad, int n, int levels,
-                const uint32_t* __restrict__ pairs, size_t count, float* __restrict__ x, float* __restrict__ bs,
-                float* __restrict__ t)
+                const double* __restrict__ bmax, const double* __restrict__ quad, int n, int levels, int centre_base,
+                int order, const uint32_t* __restrict__ pairs, size_t count, float* __restrict__ x,
+                float* __restrict__ bs, float* __restrict__ t)
 {
 	const int lane = threadIdx.x;
 	for(size_t s = blockIdx.x; s < count; s += gridDim.x)
 	{
-		const int g = n / kGroup + static_cast<int>(pairs[2 * s]);
+		const int g = centre_base + static_cast<int>(pairs[2 * s]);
 		const int c = static_cast<int>(pairs[2 * s + 1]);
 		const double px = 0.5 * (bmin[3 * g] + bmax[3 * g]);
 		const double py = 0.5 * (bmin[3 * g + 1] + bmax[3 * g + 1]);
@@ -1722,14 +1772,124 @@ k_band_residual(const double4* __restrict__ xyzr, const double* __restrict__ nma
 			a[6] -= w5 * dx * dy;
 			a[7] -= w5 * dx * dz;
 			a[8] -= w5 * dy * dz;
+			if(order >= 2)
+			{
+				double q[9];
+				quad_field64(dx, dy, dz, quad + 6 * c, q);
+				for(int k = 0; k < 9; ++k) { a[k] -= q[k]; }
+			}
 			float f[nbh_net::kIn];
 			float bv[nbh_net::kBasis];
-			const double scale = nbh_net::cell_features(px, py, pz, rec.x, rec.y, rec.z, M, bmin + 3 * c, bmax + 3 * c,
-			                                            quad + 6 * c, f, bv);
+			double scale = nbh_net::cell_features(px, py, pz, rec.x, rec.y, rec.z, M, bmin + 3 * c, bmax + 3 * c,
+			                                      quad + 6 * c, f, bv);
+			if(order >= 2) { scale *= static_cast<double>(f[0]); }    // f[0] = rho
 			for(int k = 0; k < nbh_net::kIn; ++k) { x[s * nbh_net::kIn + k] = f[k]; }
 			for(int k = 0; k < nbh_net::kBasis; ++k) { bs[s * nbh_net::kBasis + k] = bv[k]; }
 			for(int k = 0; k < 3; ++k) { t[s * nbh_net::kOut + k] = static_cast<float>(a[k] / scale); }
 			for(int k = 3; k < 9; ++k) { t[s * nbh_net::kOut + k] = static_cast<float>(a[k] * nbh_net::kJac * d / scale); }
+		}
+	}
+}
+
+// Learned correction of the two-level walk's shared far field: one wave per
+// super-group evaluates the network on the super-group's far cells (listed by
+// k_bh_super), 16 at a time on the matrix cores, and adds the predicted
+// remainder after the monopole and quadrupole terms -- field R and Jacobian J
+// at the super box centre, R / w = c0 uh + c1 w + c2 o and kJac J d / w =
+// (basis of nbh_mlp_dev.h), w = M rho^3 / d^2 -- to the super-group's
+// expansion coefficients (field and Jacobian slots), which every body of the
+// super-group applies.  Features are formed in FP64 and rounded once.
+__global__ void __launch_bounds__(64)
+k_super_net(const double4* __restrict__ xyzr, const double* __restrict__ nmass, const double* __restrict__ bmin,
+            const double* __restrict__ bmax, const double* __restrict__ quad, int n, TwoLevel two,
+            const uint2* __restrict__ frags, const float* __restrict__ mlp_p)
+{
+	__shared__ __align__(16) uint16_t tf[16 * nbh_net::kIn];
+	__shared__ float tb[16 * nbh_net::kBasis];
+	__shared__ float tw[2 * 16];
+	__shared__ __align__(16) uint16_t mlds[2 * 16 * nbh_net::kHid];
+	const int lane = threadIdx.x;
+	const int col = lane & 15;
+	const int si = blockIdx.x;
+	const int sg = (n >> (6 + kSuperShift)) + si;
+	const double px = 0.5 * (bmin[3 * sg] + bmax[3 * sg]);
+	const double py = 0.5 * (bmin[3 * sg + 1] + bmax[3 * sg + 1]);
+	const double pz = 0.5 * (bmin[3 * sg + 2] + bmax[3 * sg + 2]);
+	const int rows = min(two.far_count[si], two.far_cap);
+	const int* list = two.far_list + static_cast<size_t>(si) * two.far_cap;
+	float acc[nbh_net::kOut];
+	for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] = 0.0f; }
+	for(int base = 0; base < rows; base += 16)
+	{
+		if(lane < 16)
+		{
+			const int r = base + lane;
+			float f[nbh_net::kIn];
+			float bv[nbh_net::kBasis];
+			float w0 = 0.0f, w1 = 0.0f;
+			if(r < rows)
+			{
+				const int c = list[r];
+				const double4 rec = xyzr[c];
+				const double scale = nbh_net::cell_features(px, py, pz, rec.x, rec.y, rec.z, nmass[c], bmin + 3 * c,
+				                                            bmax + 3 * c, quad + 6 * c, f, bv) *
+				                     static_cast<double>(f[0]);
+				const double dx = rec.x - px, dy = rec.y - py, dz = rec.z - pz;
+				const double d = sqrt(dx * dx + dy * dy + dz * dz);
+				w0 = static_cast<float>(scale);
+				w1 = static_cast<float>(scale / (nbh_net::kJac * d));
+			}
+			else
+			{
+				for(int k = 0; k < nbh_net::kIn; ++k) { f[k] = 0.0f; }
+				for(int k = 0; k < nbh_net::kBasis; ++k) { bv[k] = 0.0f; }
+			}
+			for(int k = 0; k < nbh_net::kIn; ++k) { tf[lane * nbh_net::kIn + k] = nbh_net::to_bf16(f[k]); }
+			for(int k = 0; k < nbh_net::kBasis; ++k) { tb[lane * nbh_net::kBasis + k] = bv[k]; }
+			tw[2 * lane] = w0;
+			tw[2 * lane + 1] = w1;
+		}
+		__syncthreads();
+		const nbh_net::bf16x4 xa = nbh_net::lds_afrag(tf, nbh_net::kIn, 0, lane);
+		const nbh_net::f32x4 o = nbh_net::forward_tile(xa, frags, mlp_p, mlds, lane, nullptr, nullptr);
+		for(int e = 0; e < 4; ++e)
+		{
+			const int row = 4 * (lane >> 4) + e;
+			float v[nbh_net::kOut];
+			nbh_net::basis_col(tb + row * nbh_net::kBasis, col, v);
+			const float c0 = o[e] * tw[2 * row], c1 = o[e] * tw[2 * row + 1];
+			for(int k = 0; k < 3; ++k) { acc[k] = fmaf(c0, v[k], acc[k]); }
+			for(int k = 3; k < nbh_net::kOut; ++k) { acc[k] = fmaf(c1, v[k], acc[k]); }
+		}
+		__syncthreads();
+	}
+	for(int sh = 1; sh < 64; sh <<= 1)
+	{
+		for(int k = 0; k < nbh_net::kOut; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
+	}
+	if(lane == 0)
+	{
+		float* c = two.coef + 34 * static_cast<size_t>(si);
+		for(int k = 0; k < nbh_net::kOut; ++k) { c[k] += acc[k]; }
+	}
+}
+
+// Training pairs (super-group ordinal, far cell) of the super-groups sel[g]:
+// block g copies the far list of super-group sel[g] to pairs[off[g] ...].
+__global__ void __launch_bounds__(64)
+k_super_pairs(TwoLevel two, const uint32_t* __restrict__ sel, const unsigned long long* __restrict__ off,
+              uint32_t* __restrict__ pairs, unsigned long long capacity)
+{
+	const uint32_t si = sel[blockIdx.x];
+	const int rows = min(two.far_count[si], two.far_cap);
+	const int* list = two.far_list + static_cast<size_t>(si) * two.far_cap;
+	for(int r = threadIdx.x; r < rows; r += blockDim.x)
+	{
+		const unsigned long long slot = off[blockIdx.x] + static_cast<unsigned long long>(r);
+		if(slot < capacity)
+		{
+			pairs[2 * slot] = si;
+			pairs[2 * slot + 1] = static_cast<uint32_t>(list[r]);
 		}
 	}
 }
@@ -2156,7 +2316,7 @@ nbh_status nbh_tree_destroy(nbh_tree* t)
 	}
 	void* ptrs[] = { t->list[0], t->list[1], t->list[2], t->tmp, t->side, t->keys, t->vals, t->counts, t->scan_part,
 	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->wrec, t->body_n, t->counters,
-	                 t->two_lists, t->two_count, t->two_coef };
+	                 t->two_lists, t->two_count, t->two_coef, t->far_list, t->far_count };
 	for(void* p : ptrs)
 	{
 		if(p != nullptr) { (void)hipFree(p); }
@@ -2387,40 +2547,177 @@ static nbh_status two_alloc(nbh_ctx* ctx, nbh_tree* t, int cap)
 	return NBH_OK;
 }
 
-nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* t, const double* y, double* f, float near_k)
+// Far cells listed per super-group for the learned correction (4 bytes each);
+// cells beyond the cap are left to the plain expansion.
+constexpr int kFarCap = 8192;
+
+// The super-groups' far-cell lists, all or nothing (as two_alloc).
+static nbh_status far_alloc(nbh_ctx* ctx, nbh_tree* t)
 {
-	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1 (the expansion needs the cell outside the group)");
-	NBH_TRY(nbh::enter(ctx));
-	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
-	const size_t supers = t->n >> (6 + kSuperShift);
-	if(supers < 1)
+	if(t->far_list != nullptr && t->far_count != nullptr)
 	{
-		return group_walk(ctx, t, y, f, kGroupLocal, near_k);
+		return NBH_OK;
 	}
-	NBH_CHECK(t->built, "tree has not been built");
-	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
-	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
+	const size_t supers = t->n >> (6 + kSuperShift);
+	void* a = nullptr;
+	void* b = nullptr;
+	hipError_t e = hipMalloc(&a, supers * static_cast<size_t>(kFarCap) * sizeof(int));
+	if(e == hipSuccess) { e = hipMalloc(&b, supers * sizeof(int)); }
+	if(e != hipSuccess)
+	{
+		if(a != nullptr) { (void)hipFree(a); }
+		if(b != nullptr) { (void)hipFree(b); }
+		return nbh::hip_fail(e, "far-cell list allocation", __FILE__, __LINE__);
+	}
+	(void)ctx;
+	if(t->far_list != nullptr) { (void)hipFree(t->far_list); }
+	if(t->far_count != nullptr) { (void)hipFree(t->far_count); }
+	t->far_list = static_cast<int*>(a);
+	t->far_count = static_cast<int*>(b);
+	return NBH_OK;
+}
+
+// Super-group walk (k_bh_super) with the lists of t; with far lists when
+// want_far.  Allocates the lists when they are missing.
+static nbh_status super_walk(nbh_ctx* ctx, nbh_tree* t, float near_k, bool want_far, TwoLevel* two)
+{
+	const size_t supers = t->n >> (6 + kSuperShift);
 	const int cap = t->two_cap > 0 ? t->two_cap : kTwoCap;
 	if(t->two_lists == nullptr || t->two_count == nullptr || t->two_coef == nullptr)
 	{
 		NBH_TRY(two_alloc(ctx, t, cap));
 	}
-	TwoLevel two;
-	two.lists = t->two_lists;
-	two.count = t->two_count;
-	two.coef = t->two_coef;
-	two.cap = cap;
+	if(want_far)
+	{
+		NBH_TRY(far_alloc(ctx, t));
+	}
+	two->lists = t->two_lists;
+	two->count = t->two_count;
+	two->coef = t->two_coef;
+	two->cap = cap;
+	two->far_list = want_far ? t->far_list : nullptr;
+	two->far_count = want_far ? t->far_count : nullptr;
+	two->far_cap = want_far ? kFarCap : 0;
+	const int n = static_cast<int>(t->n);
+	const float super_near = fmaxf(near_k, kSuperNearMin);
+	hipLaunchKernelGGL(k_bh_super, dim3(static_cast<unsigned>(supers)), dim3(kGroup), 0, ctx->stream, t->wrec,
+	                   t->bmin, t->bmax, n, super_near, 1.0 / t->ratio, *two, t->counters);
+	NBH_LAUNCHED();
+	return NBH_OK;
+}
+
+// Two-level local walk, with the network's correction of the super-groups'
+// far fields when net is not null.
+static nbh_status two_level(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const double* y, double* f, float near_k)
+{
+	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1 (the expansion needs the cell outside the group)");
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(net == nullptr || nbh::mlp_ctx(net) == ctx, "network does not belong to this context");
+	const size_t supers = t->n >> (6 + kSuperShift);
+	if(supers < 1)
+	{
+		NBH_CHECK(net == nullptr, "the learned two-level walk needs at least %d bodies", kGroup << kSuperShift);
+		return group_walk(ctx, t, y, f, kGroupLocal, near_k);
+	}
+	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
+	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
 	const int n = static_cast<int>(t->n);
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
 	NBH_LAUNCHED();
 	nbh::scoped_timer timer(ctx, "bh_traverse");
-	const float super_near = fmaxf(near_k, kSuperNearMin);
-	hipLaunchKernelGGL(k_bh_super, dim3(static_cast<unsigned>(supers)), dim3(kGroup), 0, ctx->stream, t->wrec,
-	                   t->bmin, t->bmax, n, super_near, 1.0 / t->ratio, two, t->counters);
-	NBH_LAUNCHED();
+	TwoLevel two;
+	NBH_TRY(super_walk(ctx, t, near_k, net != nullptr, &two));
+	if(net != nullptr)
+	{
+		hipLaunchKernelGGL(k_super_net, dim3(static_cast<unsigned>(supers)), dim3(64), 0, ctx->stream, t->xyzr,
+		                   t->mass, t->bmin, t->bmax, t->quad, n, two, nbh::mlp_frags(net), nbh::mlp_params(net));
+		NBH_LAUNCHED();
+	}
 	hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec,
 	                   t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two);
 	NBH_LAUNCHED();
+	return NBH_OK;
+}
+
+nbh_status nbh_bh_fcompute_group_local2(nbh_ctx* ctx, nbh_tree* t, const double* y, double* f, float near_k)
+{
+	return two_level(ctx, t, nullptr, y, f, near_k);
+}
+
+nbh_status nbh_bh_fcompute_group_local2_mlp(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const double* y,
+                                            double* f, float near_k)
+{
+	NBH_CHECK(net != nullptr, "null network");
+	return two_level(ctx, t, net, y, f, near_k);
+}
+
+nbh_status nbh_bh_super_samples(nbh_ctx* ctx, nbh_tree* t, float near_k, const uint32_t* supers, size_t n_supers,
+                                float* x, float* basis, float* target, uint32_t* cells, size_t capacity,
+                                size_t* count)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1");
+	NBH_CHECK(supers != nullptr && count != nullptr, "null argument");
+	const size_t nsup = t->n >> (6 + kSuperShift);
+	NBH_CHECK(nsup >= 1, "no super-group: at least %d bodies are needed", kGroup << kSuperShift);
+	NBH_CHECK(n_supers > 0 && n_supers <= nsup, "bad super-group count %zu", n_supers);
+	NBH_CHECK(capacity == 0 || (x != nullptr && basis != nullptr && target != nullptr), "null sample buffer");
+	NBH_CHECK(capacity <= (static_cast<size_t>(1) << 31), "capacity %zu too large", capacity);
+	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
+	for(size_t i = 0; i < n_supers; ++i)
+	{
+		NBH_CHECK(supers[i] < nsup, "super-group %u out of range (%zu)", supers[i], nsup);
+	}
+	*count = 0;
+	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
+	NBH_LAUNCHED();
+	TwoLevel two;
+	NBH_TRY(super_walk(ctx, t, near_k, true, &two));
+	std::vector<int> far_count(nsup);
+	NBH_HIP(hipMemcpyAsync(far_count.data(), t->far_count, nsup * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	NBH_HIP(hipStreamSynchronize(ctx->stream));
+	std::vector<unsigned long long> off(n_supers);
+	unsigned long long total = 0;
+	for(size_t i = 0; i < n_supers; ++i)
+	{
+		off[i] = total;
+		total += static_cast<unsigned long long>(std::min(far_count[supers[i]], kFarCap));
+	}
+	*count = static_cast<size_t>(total);
+	const size_t m = std::min(static_cast<size_t>(total), capacity);
+	if(m == 0)
+	{
+		return NBH_OK;
+	}
+	// scratch: selected super-groups, their offsets, then the pairs
+	const size_t off_off = (n_supers * sizeof(uint32_t) + 255) / 256 * 256;
+	const size_t off_pairs = off_off + (n_supers * sizeof(unsigned long long) + 255) / 256 * 256;
+	void* sp = nullptr;
+	NBH_TRY(nbh::scratch(ctx, off_pairs + m * 2 * sizeof(uint32_t), &sp));
+	char* base = static_cast<char*>(sp);
+	uint32_t* dsel = reinterpret_cast<uint32_t*>(base);
+	unsigned long long* doff = reinterpret_cast<unsigned long long*>(base + off_off);
+	uint32_t* pairs = reinterpret_cast<uint32_t*>(base + off_pairs);
+	NBH_HIP(hipMemcpyAsync(dsel, supers, n_supers * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+	NBH_HIP(hipMemcpyAsync(doff, off.data(), n_supers * sizeof(unsigned long long), hipMemcpyHostToDevice,
+	                       ctx->stream));
+	hipLaunchKernelGGL(k_super_pairs, dim3(static_cast<unsigned>(n_supers)), dim3(64), 0, ctx->stream, two, dsel, doff,
+	                   pairs, static_cast<unsigned long long>(m));
+	NBH_LAUNCHED();
+	const int n = static_cast<int>(t->n);
+	hipLaunchKernelGGL(k_band_residual, dim3(nbh::grid_for(ctx, m, 1)), dim3(64), 0, ctx->stream, t->xyzr, t->mass,
+	                   t->bmin, t->bmax, t->quad, n, t->levels, n >> (6 + kSuperShift), 2, pairs, m, x, basis,
+	                   target);
+	NBH_LAUNCHED();
+	if(cells != nullptr)
+	{
+		NBH_HIP(hipMemcpyAsync(cells, pairs, m * 2 * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+	}
+	NBH_HIP(hipStreamSynchronize(ctx->stream));
 	return NBH_OK;
 }
 
@@ -2443,7 +2740,8 @@ nbh_status nbh_tree_bytes(nbh_ctx* ctx, const nbh_tree* t, size_t* bytes)
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(bytes != nullptr, "null output");
-	*bytes = tree_base_bytes(t->n) + (t->two_lists != nullptr ? two_bytes(t->n, t->two_cap > 0 ? t->two_cap : kTwoCap) : 0);
+	*bytes = tree_base_bytes(t->n) + (t->two_lists != nullptr ? two_bytes(t->n, t->two_cap > 0 ? t->two_cap : kTwoCap) : 0) +
+	         (t->far_list != nullptr ? (t->n >> (6 + kSuperShift)) * (static_cast<size_t>(kFarCap) + 1) * sizeof(int) : 0);
 	return NBH_OK;
 }
 
@@ -2492,7 +2790,8 @@ nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* 
 	if(m > 0)
 	{
 		hipLaunchKernelGGL(k_band_residual, dim3(nbh::grid_for(ctx, m, 1)), dim3(64), 0, ctx->stream, t->xyzr,
-		                   t->mass, t->bmin, t->bmax, t->quad, n, t->levels, pairs, m, x, basis, target);
+		                   t->mass, t->bmin, t->bmax, t->quad, n, t->levels, n / kGroup, 1, pairs, m, x, basis,
+		                   target);
 		NBH_LAUNCHED();
 		if(cells != nullptr)
 		{

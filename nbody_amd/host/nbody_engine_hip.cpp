@@ -678,6 +678,7 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_local_far(false),
 	m_net_rho(0.0f),
 	m_net_near(0.0f),
+	m_net_two_level(false),
 	m_net(nullptr)
 {
 }
@@ -694,9 +695,12 @@ nbody_engine_hip_bh::~nbody_engine_hip_bh()
 	}
 }
 
-// Correction file: the 8 bytes "NBHMLP01", int32 parameter count, float32
-// rho_min, float32 near_k, then the float32 parameters (little-endian), as
-// written by nbody_amd.correction.save.
+// Correction file: 8 bytes of magic, int32 parameter count, float32 rho_min,
+// float32 near_k, then the float32 parameters (little-endian), as written by
+// nbody_amd.correction.save.  "NBHMLP01": a network of the monopole group walk
+// (far cells of each 64-body group); "NBHMLP2L": a network of the two-level
+// local walk (far cells of each super-group, remainder after the quadrupole),
+// which needs bh_multipole=quadrupole and bh_far_field=local.
 bool nbody_engine_hip_bh::load_correction(const std::string& path, double rho, double near_k)
 {
 	FILE* fp = std::fopen(path.c_str(), "rb");
@@ -711,7 +715,10 @@ bool nbody_engine_hip_bh::load_correction(const std::string& path, double rho, d
 	float file_near = 0.0f;
 	int expect = 0;
 	nbh_mlp_param_count(&expect);
-	bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, "NBHMLP01", 8) == 0 &&
+	const bool read_magic = std::fread(magic, 1, 8, fp) == 8;
+	const bool group_net = read_magic && std::memcmp(magic, "NBHMLP01", 8) == 0;
+	const bool two_level_net = read_magic && std::memcmp(magic, "NBHMLP2L", 8) == 0;
+	bool ok = (group_net || two_level_net) &&
 	          std::fread(&count, sizeof(count), 1, fp) == 1 && count == expect &&
 	          std::fread(&file_rho, sizeof(file_rho), 1, fp) == 1 && file_rho >= 0.0f &&
 	          std::fread(&file_near, sizeof(file_near), 1, fp) == 1 && file_near >= 0.0f;
@@ -727,6 +734,19 @@ bool nbody_engine_hip_bh::load_correction(const std::string& path, double rho, d
 		nbody_log("learned_correction: '%s' is not a correction network file", path.c_str());
 		return false;
 	}
+	if(two_level_net && !(m_quadrupole && m_local_far))
+	{
+		nbody_log("learned_correction: '%s' corrects the two-level walk: it needs bh_multipole=quadrupole and "
+		          "bh_far_field=local", path.c_str());
+		return false;
+	}
+	if(group_net && m_quadrupole)
+	{
+		nbody_log("learned_correction: '%s' corrects the monopole group walk: it needs bh_multipole=monopole",
+		          path.c_str());
+		return false;
+	}
+	m_net_two_level = two_level_net;
 	m_net_params.swap(params);
 	m_net_rho = rho >= 0.0 ? static_cast<float>(rho) : file_rho;
 	m_net_near = near_k >= 0.0 ? static_cast<float>(near_k) : file_near;
@@ -813,6 +833,12 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 	}
 	if(m_group_walk && m_precision == NBH_FP32 && m_data->get_count() >= 64)
 	{
+		if(m_net != nullptr && m_net_two_level)
+		{
+			check(nbh_bh_fcompute_group_local2_mlp(m_ctx, m_tree, m_net, y->data(), f->data(), m_quad_near),
+			      "bh fcompute (two-level walk, learned correction)");
+			return;
+		}
 		if(m_net != nullptr)
 		{
 			check(nbh_bh_fcompute_group_mlp(m_ctx, m_tree, m_net, y->data(), f->data(), m_net_rho, m_net_near),

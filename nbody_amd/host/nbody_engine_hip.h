@@ -128,6 +128,7 @@ class nbody_engine_hip_bh : public nbody_engine_hip
 	float           m_net_rho;
 	float           m_net_near;
 	std::string     m_net_path;
+	bool            m_net_two_level;   // the network corrects the two-level walk's super-group far fields
 	nbh_mlp*        m_net;
 public:
 	nbody_engine_hip_bh(nbcoord_t distance_to_node_radius_ratio, e_traverse_type tt,

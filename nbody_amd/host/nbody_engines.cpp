@@ -156,9 +156,9 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("Invalid correction_near");
 			return nullptr;
 		}
-		if(net != "none" && (walk != "group" || multipole != "monopole"))
+		if(net != "none" && walk != "group")
 		{
-			nbody_log("learned_correction needs bh_walk=group and bh_multipole=monopole");
+			nbody_log("learned_correction needs bh_walk=group");
 			return nullptr;
 		}
 		nbody_engine_hip_bh* engine = new nbody_engine_hip_bh(ratio, tt, static_cast<size_t>(rate), tl);

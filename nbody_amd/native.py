@@ -63,6 +63,8 @@ HIP_API = {
     "nbh_bh_fcompute_group_quad": (i32, [vp, vp, vp, vp, C.c_float]),
     "nbh_bh_fcompute_group_local": (i32, [vp, vp, vp, vp, C.c_float]),
     "nbh_bh_fcompute_group_local2": (i32, [vp, vp, vp, vp, C.c_float]),
+    "nbh_bh_fcompute_group_local2_mlp": (i32, [vp, vp, vp, vp, vp, C.c_float]),
+    "nbh_bh_super_samples": (i32, [vp, vp, C.c_float, vp, sz, vp, vp, vp, vp, sz, C.POINTER(sz)]),
     "nbh_tree_set_list_cap": (i32, [vp, vp, C.c_int]),
     "nbh_tree_bytes": (i32, [vp, vp, C.POINTER(sz)]),
     "nbh_tree_get": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -14,6 +14,8 @@ Contents
   * cell_features / basis_matrix / predict: the 16 rotation-invariant inputs of
     (evaluation point, cell) and the basis the 9 outputs are coefficients of;
   * cell_residual: exact residual field and Jacobian of a cell by brute force;
+    cell_remainder: the same after the quadrupole term too (the target of the
+    two-level walk's correction), with quadrupole_jacobian;
   * group_band: the near and far cells of one 64-body group, by a plain stack
     walk of the heap tree with the group walk's acceptance rule; super_band /
     group_band_from: the two walks of the two-level local walk (super-group of
@@ -30,6 +32,9 @@ Pinning (tests/test_oracle_mlp.py)
   * bf16 rounding: exhaustive on special patterns, ties to even, vs float32;
   * cell_residual: Jacobian vs finite differences of the field; at small
     size/distance the residual tends to the textbook quadrupole field;
+  * cell_remainder: shrinking a cell about its mass centre by 2 divides the
+    remainder by about 8 (octupole order), where a dropped or doubled
+    quadrupole term would divide it by 4;
   * cell_features / basis: invariance of the features and covariance of the
     basis under a symmetry of the box; the quadrupole coefficients reproduce
     the textbook quadrupole field and its finite-difference Jacobian;
@@ -223,12 +228,37 @@ def cell_residual(p, pos, mass):
     return a - a0, J - J0
 
 
-def target_vector(p, pos, mass, bmin, bmax):
-    """Features x[16], basis b[16] and target t[9] of the cell holding bodies pos/mass."""
+def quadrupole_jacobian(p, cm, Q, h=None):
+    """d quadrupole_field / dp by central differences of the textbook field
+    (step 1e-4 of the distance: truncation error ~1e-8 relative)."""
+    p = np.asarray(p, np.float64)
+    if h is None:
+        h = 1e-4 * np.linalg.norm(np.asarray(cm, np.float64) - p)
+    cols = [(quadrupole_field(p + h * e, cm, Q) - quadrupole_field(p - h * e, cm, Q)) / (2 * h) for e in np.eye(3)]
+    return np.stack(cols, 1)
+
+
+def cell_remainder(p, pos, mass):
+    """Exact field of the bodies minus their monopole and quadrupole terms at
+    p (the octupole and higher terms), and its Jacobian."""
+    R, J = cell_residual(p, pos, mass)
+    cm, Q = second_moments(pos, mass)
+    return R - quadrupole_field(p, cm, Q), J - quadrupole_jacobian(p, cm, Q)
+
+
+def target_vector(p, pos, mass, bmin, bmax, order: int = 1):
+    """Features x[16], basis b[16] and target t[9] of the cell holding bodies
+    pos/mass: order 1, the residual after the monopole, scaled by
+    M rho^2 / d^2; order 2, the remainder after the monopole and the
+    quadrupole, scaled by M rho^3 / d^2."""
     cm, Q = second_moments(pos, mass)
     M = mass.sum()
     f, b, scale, d = cell_features(p, cm, M, bmin, bmax, Q)
-    R, J = cell_residual(p, pos, mass)
+    if order == 1:
+        R, J = cell_residual(p, pos, mass)
+    else:
+        R, J = cell_remainder(p, pos, mass)
+        scale = scale * f[0]
     Jv = np.array([J[0, 0], J[1, 1], J[2, 2], J[0, 1], J[0, 2], J[1, 2]])
     return f, b, np.concatenate([R / scale, K_JAC * Jv * d / scale])
 

@@ -204,6 +204,31 @@ class Tree:
                        db.read_raw(F32, k * 16).reshape(k, 16), dt.read_raw(F32, k * mo.K_OUT).reshape(k, mo.K_OUT),
                        dc.read_raw(np.uint32, 2 * k).reshape(k, 2))
 
+    def super_samples(self, near_k, supers, capacity):
+        sel = np.ascontiguousarray(supers, np.uint32)
+        dx = self.ctx.zeros_raw(F32, max(1, capacity) * mo.K_IN)
+        db = self.ctx.zeros_raw(F32, max(1, capacity) * 16)
+        dt = self.ctx.zeros_raw(F32, max(1, capacity) * mo.K_OUT)
+        dc = self.ctx.zeros_raw(np.uint32, max(1, capacity) * 2)
+        cnt = C.c_size_t()
+        native.check(self.ctx.lib.nbh_bh_super_samples(self.ctx.h, self.h, near_k, sel.ctypes.data_as(C.c_void_p),
+                                                       len(sel), dx.ptr, db.ptr, dt.ptr, dc.ptr, capacity,
+                                                       C.byref(cnt)), "super samples")
+        k = min(cnt.value, capacity)
+        return Samples(cnt.value, dx, db, dt, dx.read_raw(F32, k * mo.K_IN).reshape(k, mo.K_IN),
+                       db.read_raw(F32, k * 16).reshape(k, 16), dt.read_raw(F32, k * mo.K_OUT).reshape(k, mo.K_OUT),
+                       dc.read_raw(np.uint32, 2 * k).reshape(k, 2))
+
+    def local2(self, near_k, net=None):
+        df = self.ctx.alloc(6 * self.n)
+        if net is None:
+            native.check(self.ctx.lib.nbh_bh_fcompute_group_local2(self.ctx.h, self.h, self.dy.ptr, df.ptr, near_k),
+                         "local2")
+        else:
+            native.check(self.ctx.lib.nbh_bh_fcompute_group_local2_mlp(self.ctx.h, self.h, net.h, self.dy.ptr,
+                                                                        df.ptr, near_k), "local2_mlp")
+        return df.read()
+
     def group(self):
         df = self.ctx.alloc(6 * self.n)
         native.check(self.ctx.lib.nbh_bh_fcompute_group(self.ctx.h, self.h, self.dy.ptr, df.ptr), "group")
@@ -702,3 +727,119 @@ def test_trained_correction_gain(ctx, capsys):
     zero.close()
     t.close()
     t2.close()
+
+
+def _super_box(a, n, s):
+    sg = (n >> 10) + s
+    return 0.5 * (a["bmin"][sg] + a["bmax"][sg]), sg
+
+
+def test_super_samples_match_oracle(ctx):
+    """nbh_bh_super_samples: the rows of a super-group are exactly the far cells
+    of the oracle's super-group walk (mo.super_band), and each row's features,
+    basis and target (field remainder after the monopole and the quadrupole at
+    the super box centre, by brute force over the cell's bodies) match the
+    oracle's."""
+    n, ratio, near_k = 1 << 15, 1.25, 5.0
+    y, m = _state("plummer", n, 41)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    supers = [3, 17, 30]
+    smp = t.super_samples(near_k, supers, 1 << 16)
+    assert smp.total == len(smp.cells) > 0
+    for s in supers:
+        far, _ = mo.super_band(n, s, a["cm"], a["crit"], a["bmin"], a["bmax"], max(near_k, 5.0), ratio, shift=4)
+        got = [int(c) for ss, c in smp.cells if ss == s]
+        assert got == [int(c) for c in far] or sorted(got) == sorted(int(c) for c in far)
+    rows = np.random.default_rng(1).choice(len(smp.cells), min(200, len(smp.cells)), replace=False)
+    for k in rows:
+        s, c = int(smp.cells[k, 0]), int(smp.cells[k, 1])
+        p, _ = _super_box(a, n, s)
+        l0, l1 = _leaf_slice(n, c)
+        fx, fb, ft = mo.target_vector(p, a["cm"][l0:l1], a["mass"][l0:l1], a["bmin"][c], a["bmax"][c], order=2)
+        assert np.allclose(smp.x[k], fx, rtol=1e-5, atol=1e-6 * np.abs(fx).max()), (k, smp.x[k], fx)
+        assert np.allclose(smp.b[k], fb, rtol=1e-5, atol=1e-6 * np.abs(fb).max()), (k, smp.b[k], fb)
+        assert np.allclose(smp.t[k], ft, rtol=1e-3, atol=1e-4 * np.abs(ft).max()), (k, smp.t[k], ft)
+    t.close()
+
+
+def test_learned_two_level_walk_matches_oracle(ctx):
+    """nbh_bh_fcompute_group_local2_mlp minus nbh_bh_fcompute_group_local2 is
+    the network's correction: for every body of a super-group, the sum over the
+    oracle's far cells of that super-group of R + J (x - p), with R and J the
+    network's predictions (BF16 emulation) scaled back.  An all-zero network
+    changes nothing, bit for bit."""
+    n, ratio, near_k = 1 << 15, 1.25, 5.0
+    y, m = _state("plummer", n, 42)
+    t = Tree(ctx, y, m, ratio)
+    a = t.arrays()
+    body = _body_order(ctx, t)
+    plain = t.local2(near_k)
+    zero = Net(ctx)
+    zero.set(np.zeros(mo.N_PARAMS, F32))
+    assert np.array_equal(t.local2(near_k, zero), plain)
+    net = Net(ctx, 9)
+    p = _random_params(13, 0.1)
+    net.set(p)
+    got = t.local2(near_k, net)
+    diff = (got - plain)[3 * n:].reshape(3, n).T
+    checked = 0
+    for s in (2, 11, 29):
+        far, _ = mo.super_band(n, s, a["cm"], a["crit"], a["bmin"], a["bmax"], max(near_k, 5.0), ratio, shift=4)
+        if not far:
+            continue
+        c0, sg = _super_box(a, n, s)
+        X, B, W0, W1 = [], [], [], []
+        for c in far:
+            l0, l1 = _leaf_slice(n, c)
+            Q = mo.second_moments(a["cm"][l0:l1], a["mass"][l0:l1])[1]
+            f, b, scale, d = mo.cell_features(c0, a["cm"][c], a["mass"][c], a["bmin"][c], a["bmax"][c], Q)
+            X.append(f); B.append(b); W0.append(scale * f[0]); W1.append(scale * f[0] / (mo.K_JAC * d))
+        pred = mo.predict(p, np.array(X, np.float32), np.array(B, np.float32), "bf16").astype(np.float64)
+        R = (pred[:, :3] * np.array(W0)[:, None]).sum(0)
+        Jv = (pred[:, 3:] * np.array(W1)[:, None]).sum(0)
+        J = np.array([[Jv[0], Jv[3], Jv[4]], [Jv[3], Jv[1], Jv[5]], [Jv[4], Jv[5], Jv[2]]])
+        leaves = body[n + 1024 * s:n + 1024 * s + 1024]
+        X0 = a["cm"][n + 1024 * s:n + 1024 * s + 1024]
+        want = R[None, :] + (X0 - c0) @ J.T
+        err = np.linalg.norm(diff[leaves] - want, axis=1) / np.linalg.norm(want, axis=1)
+        assert np.percentile(err, 99) < 2e-3 and err.max() < 1e-2, (s, np.percentile(err, 99), err.max())
+        checked += 1
+    assert checked >= 2
+    zero.close()
+    net.close()
+    t.close()
+
+
+def test_learned_two_level_gain(ctx, capsys):
+    """Train the two-level walk's correction on the super-group far cells of one
+    Plummer sphere (theta 0.8, near 5) and apply it to another: the walk's
+    error against the FP64 direct sum falls (the network removes part of the
+    remainder of the super-groups' shared far fields), and the network's
+    held-out loss is below that of predicting zero."""
+    n, ratio, near_k = 1 << 17, 1.25, 5.0
+    ytr, mtr = _state("plummer", n, 43)
+    ttr = Tree(ctx, ytr, mtr, ratio)
+    smp = ttr.super_samples(near_k, np.arange(n >> 10), 1 << 22)
+    k = min(smp.total, 1 << 22)
+    net = Net(ctx, 5)
+    el = net.train(smp.dx, smp.db, smp.dt, k, 40, 1024, 2e-3)
+    ttr.close()
+    y, m = _state("plummer", n, 44)
+    t = Tree(ctx, y, m, ratio)
+    hold = t.super_samples(near_k, np.arange(n >> 10), 1 << 22)
+    kh = min(hold.total, 1 << 22)
+    _, loss_net = net.gradient(hold.dx, hold.db, hold.dt, kh)
+    loss_zero = float(np.mean(np.sum(hold.t.astype(np.float64) ** 2, axis=1)))
+    idx = np.random.default_rng(3).choice(n, 4096, replace=False)
+    a0 = _sample_accel(ctx, t.dy, t.dm, n, idx)
+    e_plain = _rms_rel(t.local2(near_k), a0, n, idx)
+    e_net = _rms_rel(t.local2(near_k, net), a0, n, idx)
+    with capsys.disabled():
+        print(f"\n[two-level correction gain] train rows {smp.total}, loss {el[0]:.3g}->{el[-1]:.3g}; held-out "
+              f"loss {loss_net:.3g} (zero prediction {loss_zero:.3g}); rms/p99 two-level theta 0.8: "
+              f"{e_plain[0]:.4e}/{e_plain[1]:.4e} -> with the network {e_net[0]:.4e}/{e_net[1]:.4e}")
+    assert loss_net < 0.8 * loss_zero
+    assert e_net[0] < e_plain[0]
+    net.close()
+    t.close()

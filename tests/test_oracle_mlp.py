@@ -128,6 +128,45 @@ def test_residual_tends_to_quadrupole():
     assert errs[1] < 0.6 * errs[0] and errs[2] < 0.6 * errs[1]
 
 
+def test_remainder_has_octupole_order():
+    """cell_remainder = field - monopole - quadrupole: shrinking the cell about
+    its mass centre by 2 divides it by about 8 (the octupole is cubic in the
+    size); the same check on a copy that drops, or doubles, the quadrupole
+    term gives about 4, so neither mistake passes."""
+    rng = np.random.default_rng(8)
+    pos, mass = _cluster(rng)
+    cm, Q = mo.second_moments(pos, mass)
+    p = cm + 30.0 * np.array([0.48, -0.6, 0.64])
+
+    def shrunk(k):
+        return cm + (pos - cm) / k
+
+    def ratio(fn):
+        r1 = np.linalg.norm(fn(p, shrunk(1.0), mass)[0])
+        r2 = np.linalg.norm(fn(p, shrunk(2.0), mass)[0])
+        return r1 / r2
+
+    assert 7.0 < ratio(mo.cell_remainder) < 9.0
+    assert 3.5 < ratio(mo.cell_residual) < 4.5                  # quadrupole dropped
+    doubled = lambda pp, pos_, m_: (mo.cell_residual(pp, pos_, m_)[0]
+                                    - 2.0 * mo.quadrupole_field(pp, *mo.second_moments(pos_, m_)), None)
+    assert 3.5 < ratio(doubled) < 4.5
+    # the Jacobian of the remainder against finite differences of its field
+    R, J = mo.cell_remainder(p, pos, mass)
+    h = 1e-3
+    for j in range(3):
+        e = np.zeros(3); e[j] = h
+        fd = (mo.cell_remainder(p + e, pos, mass)[0] - mo.cell_remainder(p - e, pos, mass)[0]) / (2 * h)
+        assert np.allclose(fd, J[:, j], rtol=1e-4, atol=1e-6 * np.abs(J).max())
+    # order-2 targets are the remainder over M rho^3 / d^2
+    lo, hi = pos.min(0), pos.max(0)
+    f, b, t = mo.target_vector(p, pos, mass, lo, hi, order=2)
+    _, _, scale, d = mo.cell_features(p, cm, mass.sum(), lo, hi, Q)
+    assert np.allclose(t[:3] * scale * f[0], R, rtol=1e-12)
+    assert np.allclose(t[3:] * scale * f[0] / (mo.K_JAC * d), [J[0, 0], J[1, 1], J[2, 2], J[0, 1], J[0, 2], J[1, 2]],
+                       rtol=1e-12)
+
+
 def _rotation(rng):
     q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
     return q * np.sign(np.linalg.det(q))
