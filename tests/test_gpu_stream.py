@@ -115,3 +115,20 @@ def test_cli_run_and_resume(tmp_path):
     assert (Path(str(full) + ".idx")).read_text() == (Path(str(part) + ".idx")).read_text()
     assert (Path(str(full) + "0.dat")).read_bytes() == (Path(str(part) + "0.dat")).read_bytes()
     assert len(_frames(str(full))) == 7
+
+
+def test_bench_front_end_tree_ratio():
+    """nbody-bench --bench=gpu_tree_ratio (the reference's gpu_tree_ratio sweep,
+    test/bench/main.cpp:148-216) on a small state: one row per hip_bh variant,
+    one positive time per step per ratio."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "nbody_amd" / "nbody-bench"
+    r = subprocess.run([str(exe), "--bench=gpu_tree_ratio", "--stars_count=16384", "--values=2,4",
+                        "--max_time=0.03"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [line for line in r.stdout.splitlines() if not line.startswith("%")]
+    assert lines[0] == "time" and lines[1].split() == ["name", "2", "4"]
+    rows = {line.split()[0]: [float(v) for v in line.split()[1:]] for line in lines[2:]}
+    assert set(rows) == {"hip_bh+heap+nested", "hip_bh+heap_stackless", "hip_bh+group", "hip_bh+two_level"}
+    assert all(len(v) == 2 and all(t > 0 for t in v) for v in rows.values())

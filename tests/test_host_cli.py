@@ -46,3 +46,10 @@ def test_setup_failures(tmp_path):
     assert r.returncode == 255 and "can't open stream to resume" in r.stderr
     r = _run(f"--initial_state={tmp_path}/missing.txt", "--initial_type=G1")
     assert r.returncode == 255 and "can't load initial state" in r.stderr
+
+
+def test_bench_front_end_rejects_unknown_bench():
+    exe = EXE.parent / "nbody-bench"
+    assert exe.exists(), "front end not built: run python -m nbody_amd.build"
+    r = subprocess.run([str(exe), "--bench=nonexistent"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255 and "unknown bench" in r.stderr
