@@ -1457,7 +1457,8 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 // accepted node into the list.
 __global__ void __launch_bounds__(kGroup)
 k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
-           int n, float near_k, double inv_ratio, TwoLevel two, unsigned long long* __restrict__ counters)
+           int n, float near_k, double inv_ratio, TwoLevel two, unsigned long long* __restrict__ counters,
+           int stack_limit)
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 farq[kFarRing];
@@ -1520,7 +1521,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 			}
 		}
 		const bool tolist = valid && !far && !open;
-		if(sp + __popcll(__ballot(open)) > kWalkStack)
+		if(sp + __popcll(__ballot(open)) > stack_limit)
 		{
 			overflow = true;           // the groups of this super-group will walk for themselves
 			open = false;
@@ -2600,8 +2601,17 @@ static nbh_status super_walk(nbh_ctx* ctx, nbh_tree* t, float near_k, bool want_
 	two->far_cap = want_far ? kFarCap : 0;
 	const int n = static_cast<int>(t->n);
 	const float super_near = fmaxf(near_k, kSuperNearMin);
+	// NBH_TEST_SUPER_STACK=k (64 <= k <= the stack size) in the environment lets
+	// the super-group walk use only k stack entries, so that its overflow path
+	// (the groups of a super-group that ran out walk from the root) can be tested
+	int stack_limit = kWalkStack;
+	if(const char* env = getenv("NBH_TEST_SUPER_STACK"))
+	{
+		const int k = atoi(env);
+		if(k >= kGroup && k <= kWalkStack) { stack_limit = k; }
+	}
 	hipLaunchKernelGGL(k_bh_super, dim3(static_cast<unsigned>(supers)), dim3(kGroup), 0, ctx->stream, t->wrec,
-	                   t->bmin, t->bmax, n, super_near, 1.0 / t->ratio, *two, t->counters);
+	                   t->bmin, t->bmax, n, super_near, 1.0 / t->ratio, *two, t->counters, stack_limit);
 	NBH_LAUNCHED();
 	return NBH_OK;
 }

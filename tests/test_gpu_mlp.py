@@ -843,3 +843,26 @@ def test_learned_two_level_gain(ctx, capsys):
     assert e_net[0] < e_plain[0]
     net.close()
     t.close()
+
+
+def test_two_level_walk_stack_overflow_falls_back(ctx, monkeypatch):
+    """When the super-group walk runs out of stack (forced here through the
+    library's NBH_TEST_SUPER_STACK hook, 64 entries), every group of that
+    super-group ignores the partial list and expansion and walks from the root:
+    the result equals the one-level local walk bit for bit, and the fallbacks
+    are counted in walk counter out[2]."""
+    n, ratio, near_k = 1 << 16, 1.0 / 0.6, 3.0
+    y, m = _state("plummer", n, 45)
+    t = Tree(ctx, y, m, ratio)
+    df = ctx.alloc(6 * n)
+    native.check(ctx.lib.nbh_bh_fcompute_group_local(ctx.h, t.h, t.dy.ptr, df.ptr, near_k), "local")
+    one = df.read()
+    monkeypatch.setenv("NBH_TEST_SUPER_STACK", "64")
+    two = t.local2(near_k)
+    c = t.counters()
+    monkeypatch.delenv("NBH_TEST_SUPER_STACK")
+    assert c[2] == n >> 10                        # every super-group fell back
+    assert np.array_equal(two, one)
+    normal = t.local2(near_k)
+    assert t.counters()[2] == 0 and not np.array_equal(normal, one)
+    t.close()
