@@ -34,10 +34,14 @@ typedef enum nbh_precision {
 	NBH_FP64 = 0,   /* IEEE binary64, same operation order as the reference CPU engines */
 	NBH_FP32 = 1,   /* FP32 interaction math on relative coordinates, compensated sums,
 	                   FP64 state in and out */
-	NBH_FP32_CORR = 2  /* direct sum only: NBH_FP32 plus an FP64 correction pass -- for each
+	NBH_FP32_CORR = 2, /* direct sum only: NBH_FP32 plus an FP64 correction pass -- for each
 	                   target, the sources within 32 places of it in the Morton order of the
 	                   FP32 tiles (its nearest neighbours along the curve) are recomputed in
 	                   FP64 and their FP32 terms, recomputed the tiles' way, taken out */
+	NBH_FP64_BLOCK = 3 /* direct sum only: IEEE binary64 in the operation order of the
+	                   reference's `block` engine (nbody/nbody_engine_block.cpp:14-126):
+	                   a_i = -sum_j (r_i - r_j) m_j / (sqrt(r2) r2), self pair included,
+	                   no division by m_i */
 } nbh_precision;
 
 /* Tree layouts of the Barnes-Hut engine (nbody/nbody_engine_simple_bh.h:13-20). */
@@ -50,6 +54,7 @@ typedef struct nbh_ctx nbh_ctx;     /* one device + one stream */
 typedef struct nbh_tree nbh_tree;   /* device-resident heap kd-tree */
 typedef struct nbh_mlp nbh_mlp;     /* correction network (parameters + Adam state) */
 typedef struct nbh_snapshot nbh_snapshot;   /* staging for asynchronous state copies */
+typedef struct nbh_ah nbh_ah;       /* Ahmad-Cohen neighbour lists + frozen far forces */
 
 /* ---- context, errors ---------------------------------------------------- */
 const char* nbh_last_error(void);
@@ -130,6 +135,28 @@ nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const double* mass
    Barnes-Hut paths on sampled targets where a full FP64 sum is too costly. */
 nbh_status nbh_direct_sample(nbh_ctx* ctx, const double* y, const double* mass, size_t n,
                              const uint32_t* idx, size_t m, double* acc);
+
+/* ---- Ahmad-Cohen neighbour scheme (nbody/nbody_engine_ah.cpp:16-160) -------
+   State of one engine: per body the frozen far force and the neighbour list.
+   The reference keeps them in std::vector members of nbody_engine_ah
+   (nbody/nbody_engine_ah.h:13-17); here they live in HBM. */
+nbh_status nbh_ah_create(nbh_ctx* ctx, size_t n, nbh_ah** ah);
+nbh_status nbh_ah_destroy(nbh_ah* ah);
+/* Full recompute, replacing nbody_engine_ah::fcompute_full (nbody_engine_ah.cpp:44-114):
+   f = [v ; F/m] over all pairs in FP64 (nbody_data::force, body2 in index order),
+   body2 listed as a neighbour of body1 when |r1 - r2|^2 < max_dist^2 or
+   |F12|^2 > min_force^2, and the far force (sum of the other pairs' F) frozen.
+   Reads one 8-byte total back to the host to size the lists (grown as needed). */
+nbh_status nbh_ah_fcompute_full(nbh_ctx* ctx, nbh_ah* ah, const double* y, const double* mass, double* f,
+                                double max_dist, double min_force);
+/* Between full recomputes, replacing nbody_engine_ah::fcompute_sparse
+   (nbody_engine_ah.cpp:116-160): f = [v ; (far + sum over neighbours of F)/m]. */
+nbh_status nbh_ah_fcompute_sparse(nbh_ctx* ctx, nbh_ah* ah, const double* y, const double* mass, double* f);
+/* The lists of the last full recompute, copied to host memory: *total entries;
+   offs (n + 1 offsets), list (*total body numbers) and univ (far forces [3][n])
+   are written when not NULL. */
+nbh_status nbh_ah_lists(nbh_ctx* ctx, const nbh_ah* ah, size_t* total, uint64_t* offs, uint32_t* list,
+                        double* univ);
 
 /* ---- Barnes-Hut heap kd-tree --------------------------------------------
    n must be a power of two (the heap layout of nbody/nbody_space_heap.cpp:23

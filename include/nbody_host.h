@@ -108,6 +108,8 @@ int         nbs_engine_has_error(const nbs_engine* e);
 /* Device context / tree of a hip engine (NULL otherwise), for direct C-ABI use */
 void*       nbs_engine_hip_context(nbs_engine* e);
 const void* nbs_engine_hip_tree(nbs_engine* e);
+/* the nbh_ah state of a `hip_ah` engine (NULL for other engines) */
+const void* nbs_engine_hip_neighbours(nbs_engine* e);
 /* Device pointer of a hip buffer (NULL otherwise) */
 void*       nbs_memory_device_ptr(nbs_memory* m);
 size_t      nbs_memory_size(const nbs_memory* m);

@@ -340,6 +340,10 @@ class nbody_engine:
     def hip_tree(self):
         return self._lib.nbs_engine_hip_tree(self.handle)
 
+    def hip_neighbours(self):
+        """The nbh_ah state of a `hip_ah` engine (None for other engines)."""
+        return self._lib.nbs_engine_hip_neighbours(self.handle)
+
 
 def nbody_create_engine(params: dict) -> Optional[nbody_engine]:
     """Engine factory; None for an unknown engine or an invalid parameter."""

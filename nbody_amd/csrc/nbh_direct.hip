@@ -24,6 +24,7 @@
 #include <cmath>
 
 #include "nbh_common.h"
+#include "nbh_f64.h"
 
 namespace {
 
@@ -32,50 +33,16 @@ constexpr double kMinDistance = 1e-8;   // nbody/nbtype.h:79
 // ---------------------------------------------------------------------------
 // FP64, reference operation order
 // ---------------------------------------------------------------------------
-// Correctly rounded binary64 square root and quotient without the range steps
-// of the general sequences.  The compiler's sqrt is rsq + two Goldschmidt
-// refinements + two Newton corrections, wrapped in a scaling by 2^+-256 for
-// inputs below 2^-767 and a pass-through for 0 and inf; its division is rcp +
-// two Newton refinements of the reciprocal, the quotient and one fma
-// correction, wrapped in v_div_scale / v_div_fmas / v_div_fixup, which scale
-// operands whose exponents are near the ends of the range and handle 0, inf and
-// NaN.  For the operands of the pair term -- r2 in [1e-8, 1.3e13] (coordinates of
-// magnitude <= 1e6) and numerators -m1 m2 with masses of magnitude in [1e-60,
-// 1e60] -- those wrappers are identities, so the bare sequences below give the
-// same bits (checked against the CPU restatement by the tests).  Tiles holding
-// any other operand take the general sequences.
-__device__ inline double sqrt_rn(double x)
-{
-	const double y = __builtin_amdgcn_rsq(x);
-	double g = x * y;
-	double h = y * 0.5;
-	const double r = fma(-h, g, 0.5);
-	g = fma(g, r, g);
-	double d = fma(-g, g, x);
-	h = fma(h, r, h);
-	g = fma(d, h, g);
-	d = fma(-g, g, x);
-	return fma(d, h, g);
-}
+// Pair laws of the FP64 kernel.  kLawSimple: one term of nbody_data::force
+// (nbody/nbody_data.cpp:35-44) with the mass factor -m1 precomputed,
+// t += d * ((gm1 * m2) / (r2 * sqrt(r2))), divided by m1 at the end
+// (nbody_engine_simple.cpp:169-171).  kLawBlock: nbody_engine_block's term
+// (nbody/nbody_engine_block.cpp:96-112), t -= d * (m2 / (sqrt(r2) * r2)),
+// which is the acceleration itself.  d = r1 - r2 in both, r2 clamped at 1e-8.
+constexpr int kLawSimple = 0;
+constexpr int kLawBlock = 1;
 
-__device__ inline double div_rn(double a, double b)
-{
-	double r = __builtin_amdgcn_rcp(b);
-	double e = fma(-b, r, 1.0);
-	r = fma(r, e, r);
-	e = fma(-b, r, 1.0);
-	r = fma(r, e, r);
-	const double q = a * r;
-	const double rem = fma(-b, q, a);
-	return fma(rem, r, q);
-}
-
-__device__ inline bool coord_in_range(double v) { return fabs(v) <= 1e6; }
-__device__ inline bool mass_in_range(double m) { return fabs(m) >= 1e-60 && fabs(m) <= 1e60; }
-
-// One pair term of nbody_data::force (nbody/nbody_data.cpp:35-44) with the
-// mass factor -m1 precomputed: t += d * ((gm1 * m2) / (r2 * sqrt(r2))).
-template <bool kFast>
+template <int kLaw, bool kFast>
 __device__ inline void pair64(double x1, double y1, double z1, double gm1, const double4& s, double& tx, double& ty,
                               double& tz)
 {
@@ -87,7 +54,7 @@ __device__ inline void pair64(double x1, double y1, double z1, double gm1, const
 	if constexpr(kFast)
 	{
 		r2 = fmax(r2, kMinDistance);     // r2 is finite here: the same as the branch below
-		coeff = div_rn(gm1 * s.w, r2 * sqrt_rn(r2));
+		coeff = kLaw == kLawSimple ? div_rn(gm1 * s.w, r2 * sqrt_rn(r2)) : div_rn(s.w, sqrt_rn(r2) * r2);
 	}
 	else
 	{
@@ -95,21 +62,32 @@ __device__ inline void pair64(double x1, double y1, double z1, double gm1, const
 		{
 			r2 = kMinDistance;
 		}
-		coeff = (gm1 * s.w) / (r2 * sqrt(r2));
+		coeff = kLaw == kLawSimple ? (gm1 * s.w) / (r2 * sqrt(r2)) : s.w / (sqrt(r2) * r2);
 	}
-	tx += dx * coeff;
-	ty += dy * coeff;
-	tz += dz * coeff;
+	if constexpr(kLaw == kLawSimple)
+	{
+		tx += dx * coeff;
+		ty += dy * coeff;
+		tz += dz * coeff;
+	}
+	else
+	{
+		tx -= dx * coeff;
+		ty -= dy * coeff;
+		tz -= dz * coeff;
+	}
 }
 
 // kF64Tgt targets per work-item (i0 + k * blockDim.x), sources staged in LDS as
 // double4 (x, y, z, m) tiles of blockDim.x and read back as wave-wide
 // broadcasts, each reused for every target of the work-item.  The source loop
-// runs in index order.  The self pair is skipped in the reference; here, on
-// the fast path, it is summed: dx = dy = dz = +0 there, so its term is -0 and
-// adding -0 leaves every value (+0 included) unchanged.
+// runs in index order.  The simple law skips the self pair in the reference;
+// here, on the fast path, it is summed: dx = dy = dz = +0 there, so its term is
+// -0 and adding -0 leaves every value (+0 included) unchanged.  The block law
+// sums it in the reference too (its term subtracts +0).
 constexpr int kF64Tgt = 2;
 
+template <int kLaw>
 __global__ void __launch_bounds__(1024)
 k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass, double* __restrict__ f, size_t n)
 {
@@ -130,7 +108,7 @@ k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass, dou
 		gm1[k] = -1.0 * m1[k];            // (-GravityConst * mass1), G = 1
 		tx[k] = ty[k] = tz[k] = 0.0;
 		safe = safe && coord_in_range(x1[k]) && coord_in_range(y1[k]) && coord_in_range(z1[k]) &&
-		       (!active || mass_in_range(m1[k]));
+		       (!active || kLaw == kLawBlock || mass_in_range(m1[k]));
 	}
 	for(size_t base = 0; base < n; base += bs)
 	{
@@ -151,7 +129,10 @@ k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass, dou
 			for(int jj = 0; jj < cnt; ++jj)
 			{
 				const double4 sj = src[jj];
-				for(int k = 0; k < kF64Tgt; ++k) { pair64<true>(x1[k], y1[k], z1[k], gm1[k], sj, tx[k], ty[k], tz[k]); }
+				for(int k = 0; k < kF64Tgt; ++k)
+				{
+					pair64<kLaw, true>(x1[k], y1[k], z1[k], gm1[k], sj, tx[k], ty[k], tz[k]);
+				}
 			}
 		}
 		else
@@ -161,9 +142,9 @@ k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass, dou
 				const double4 sj = src[jj];
 				for(int k = 0; k < kF64Tgt; ++k)
 				{
-					if(base + jj != i0 + static_cast<size_t>(k) * bs)
+					if(kLaw == kLawBlock || base + jj != i0 + static_cast<size_t>(k) * bs)
 					{
-						pair64<false>(x1[k], y1[k], z1[k], gm1[k], sj, tx[k], ty[k], tz[k]);
+						pair64<kLaw, false>(x1[k], y1[k], z1[k], gm1[k], sj, tx[k], ty[k], tz[k]);
 					}
 				}
 			}
@@ -177,9 +158,9 @@ k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass, dou
 			f[i] = y[3 * n + i];
 			f[n + i] = y[4 * n + i];
 			f[2 * n + i] = y[5 * n + i];
-			f[3 * n + i] = tx[k] / m1[k];
-			f[4 * n + i] = ty[k] / m1[k];
-			f[5 * n + i] = tz[k] / m1[k];
+			f[3 * n + i] = kLaw == kLawSimple ? tx[k] / m1[k] : tx[k];
+			f[4 * n + i] = kLaw == kLawSimple ? ty[k] / m1[k] : ty[k];
+			f[5 * n + i] = kLaw == kLawSimple ? tz[k] / m1[k] : tz[k];
 		}
 	}
 }
@@ -579,15 +560,23 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	NBH_CHECK(y != nullptr && mass != nullptr && f != nullptr, "null buffer");
 	NBH_CHECK(y != f, "y and f must not alias");
 	NBH_CHECK(n < (static_cast<size_t>(1) << 31), "n = %zu too large", n);
-	if(precision == NBH_FP64)
+	if(precision == NBH_FP64 || precision == NBH_FP64_BLOCK)
 	{
 		int bs = block_size == 0 ? 256 : block_size;
 		NBH_CHECK(bs >= 64 && bs <= 1024 && bs % 64 == 0,
 		          "block_size %d: must be a multiple of 64 in [64, 1024]", block_size);
 		unsigned grid = static_cast<unsigned>((n + kF64Tgt * bs - 1) / (kF64Tgt * bs));
-		nbh::scoped_timer timer(ctx, "direct_fp64");
-		hipLaunchKernelGGL(k_direct_fp64, dim3(grid), dim3(bs), bs * sizeof(double4),
-		                   ctx->stream, y, mass, f, n);
+		nbh::scoped_timer timer(ctx, precision == NBH_FP64 ? "direct_fp64" : "direct_fp64_block");
+		if(precision == NBH_FP64)
+		{
+			hipLaunchKernelGGL(k_direct_fp64<kLawSimple>, dim3(grid), dim3(bs), bs * sizeof(double4), ctx->stream, y,
+			                   mass, f, n);
+		}
+		else
+		{
+			hipLaunchKernelGGL(k_direct_fp64<kLawBlock>, dim3(grid), dim3(bs), bs * sizeof(double4), ctx->stream, y,
+			                   mass, f, n);
+		}
 		NBH_LAUNCHED();
 		return NBH_OK;
 	}

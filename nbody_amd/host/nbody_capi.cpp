@@ -230,6 +230,12 @@ const void* nbs_engine_hip_tree(nbs_engine* e)
 	return h == nullptr ? nullptr : h->tree();
 }
 
+const void* nbs_engine_hip_neighbours(nbs_engine* e)
+{
+	nbody_engine_hip_ah* h = dynamic_cast<nbody_engine_hip_ah*>(E(e));
+	return h == nullptr ? nullptr : h->neighbours();
+}
+
 void* nbs_memory_device_ptr(nbs_memory* m)
 {
 	nbody_engine_hip::smemory* s = dynamic_cast<nbody_engine_hip::smemory*>(M(m));

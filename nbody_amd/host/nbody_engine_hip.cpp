@@ -626,7 +626,93 @@ void nbody_engine_hip::print_info() const
 	nbody_print("\tdevice: %d (%s, %d CUs)", m_device, name, cus);
 	nbody_print("\tblock_size: %d", m_block_size);
 	nbody_print("\tforce_precision: %s",
-	            m_precision == NBH_FP64 ? "fp64" : m_precision == NBH_FP32 ? "fp32" : "fp32_corr");
+	            m_precision == NBH_FP64 ? "fp64" : m_precision == NBH_FP32 ? "fp32" :
+	            m_precision == NBH_FP32_CORR ? "fp32_corr" : "fp64 (block operation order)");
+}
+
+// ---------------------------------------------------------------------------
+// block (nbody/nbody_engine_block.cpp:14-126)
+// ---------------------------------------------------------------------------
+nbody_engine_hip_block::nbody_engine_hip_block()
+{
+	m_precision = NBH_FP64_BLOCK;
+}
+
+const char* nbody_engine_hip_block::type_name() const
+{
+	return "nbody_engine_hip_block";
+}
+
+// ---------------------------------------------------------------------------
+// Ahmad-Cohen (nbody/nbody_engine_ah.cpp:3-160)
+// ---------------------------------------------------------------------------
+nbody_engine_hip_ah::nbody_engine_hip_ah(size_t full_recompute_rate, double max_dist, double min_force) :
+	m_rate(full_recompute_rate),
+	m_max_dist(max_dist),
+	m_min_force(min_force),
+	m_ah(nullptr),
+	m_have_lists(false)
+{
+}
+
+nbody_engine_hip_ah::~nbody_engine_hip_ah()
+{
+	nbh_ah_destroy(m_ah);
+}
+
+const char* nbody_engine_hip_ah::type_name() const
+{
+	return "nbody_engine_hip_ah";
+}
+
+bool nbody_engine_hip_ah::init(nbody_data* data)
+{
+	if(!nbody_engine_hip::init(data))
+	{
+		return false;
+	}
+	nbh_ah_destroy(m_ah);
+	m_ah = nullptr;
+	m_have_lists = false;
+	return check(nbh_ah_create(m_ctx, data->get_count(), &m_ah), "neighbour-list allocation");
+}
+
+// nbody/nbody_engine_ah.cpp:16-42.  A full recompute also runs when no lists
+// exist yet (the first call of a resumed run at a step that is not a multiple
+// of the rate), where the reference would read empty lists.
+void nbody_engine_hip_ah::fcompute(const nbcoord_t& t, const memory* _y, memory* _f)
+{
+	(void)t;
+	const smemory* y = nullptr;
+	smemory* f = nullptr;
+	if(!check_f(_y, _f, &y, &f))
+	{
+		return;
+	}
+	if(m_ah == nullptr)
+	{
+		nbody_log("hip_ah: engine not initialised");
+		return;
+	}
+	advise_compute_count();
+	if(!m_have_lists || m_data->get_step() % m_rate == 0)
+	{
+		m_have_lists = check(nbh_ah_fcompute_full(m_ctx, m_ah, y->data(), m_mass->data(), f->data(), m_max_dist,
+		                                          m_min_force),
+		                     "fcompute (full recompute)");
+	}
+	else
+	{
+		check(nbh_ah_fcompute_sparse(m_ctx, m_ah, y->data(), m_mass->data(), f->data()), "fcompute (neighbours)");
+	}
+}
+
+void nbody_engine_hip_ah::print_info() const
+{
+	nbody_engine_hip::print_info();
+	nbody_print("\tfull_recompute_rate: %zu", m_rate);
+	nbody_print("\tmax_dist: %g", m_max_dist);
+	nbody_print("\tmin_force: %g", m_min_force);
 }
 
 bool nbody_engine_hip::compute_statistics(bool with_energy, double out[12])

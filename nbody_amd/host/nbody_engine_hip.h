@@ -112,6 +112,37 @@ public:
 	bool compute_statistics(bool with_energy, double out[12]) override;
 };
 
+// The `block` engine's arithmetic on the GPU (nbody/nbody_engine_block.{h,cpp}):
+// FP64 source tiles, a = -sum m_j (r_i - r_j) / (sqrt(r2) r2) with the self pair
+// included and no division by m_i.  Everything else as nbody_engine_hip.
+class nbody_engine_hip_block : public nbody_engine_hip
+{
+public:
+	nbody_engine_hip_block();
+	const char* type_name() const override;
+};
+
+// Ahmad-Cohen neighbour scheme (nbody/nbody_engine_ah.{h,cpp}) on the GPU: a
+// full FP64 recompute every full_recompute_rate-th step builds the neighbour
+// lists (|r1 - r2| < max_dist or |F12| > min_force) and freezes the far force;
+// the other steps sum the far force and the neighbours only.
+class nbody_engine_hip_ah : public nbody_engine_hip
+{
+	size_t  m_rate;
+	double  m_max_dist;
+	double  m_min_force;
+	nbh_ah* m_ah;
+	bool    m_have_lists;
+public:
+	nbody_engine_hip_ah(size_t full_recompute_rate, double max_dist, double min_force);
+	~nbody_engine_hip_ah() override;
+	const char* type_name() const override;
+	bool init(nbody_data* data) override;
+	void fcompute(const nbcoord_t& t, const memory* y, memory* f) override;
+	void print_info() const override;
+	const nbh_ah* neighbours() const { return m_ah; }
+};
+
 class nbody_engine_hip_bh : public nbody_engine_hip
 {
 	nbcoord_t       m_ratio;

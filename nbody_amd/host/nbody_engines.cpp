@@ -44,12 +44,21 @@ bool common_hip_params(const nbody_params& param, nbody_engine_hip* engine)
 		nbody_log("force_precision=fp32_corr is a direct-sum mode (engine hip)");
 		return false;
 	}
+	const std::string type = param.value("engine", "");
+	if(prec != "fp64" && (type == "hip_block" || type == "hip_ah"))
+	{
+		nbody_log("engine %s computes in FP64 only", type.c_str());
+		return false;
+	}
 	if(engine->select_device(device) != 0)
 	{
 		return false;
 	}
 	engine->set_block_size(static_cast<int>(block_size));
-	engine->set_precision(prec == "fp64" ? NBH_FP64 : prec == "fp32" ? NBH_FP32 : NBH_FP32_CORR);
+	if(type != "hip_block")
+	{
+		engine->set_precision(prec == "fp64" ? NBH_FP64 : prec == "fp32" ? NBH_FP32 : NBH_FP32_CORR);
+	}
 	return true;
 }
 
@@ -61,6 +70,46 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 	if(type == "hip")
 	{
 		nbody_engine_hip* engine = new nbody_engine_hip();
+		if(!common_hip_params(param, engine))
+		{
+			delete engine;
+			return nullptr;
+		}
+		return engine;
+	}
+	// nbody/nbody_engines.cpp:16-19 (`block`) on the GPU
+	if(type == "hip_block")
+	{
+		nbody_engine_hip_block* engine = new nbody_engine_hip_block();
+		if(!common_hip_params(param, engine))
+		{
+			delete engine;
+			return nullptr;
+		}
+		return engine;
+	}
+	// nbody/nbody_engines.cpp:8-15 (`ah`) on the GPU; the same keys and defaults.
+	// full_recompute_rate 0 (a division by zero in the reference) returns NULL.
+	if(type == "hip_ah")
+	{
+		long long rate = 0;
+		double max_dist = 0, min_force = 0;
+		if(!param.get_long("full_recompute_rate", 1000, &rate) || rate <= 0)
+		{
+			nbody_log("Invalid full_recompute_rate: must be a positive integer");
+			return nullptr;
+		}
+		if(!param.get_double("max_dist", 10, &max_dist) || !(max_dist >= 0))
+		{
+			nbody_log("Invalid max_dist");
+			return nullptr;
+		}
+		if(!param.get_double("min_force", 1e-4, &min_force) || !(min_force >= 0))
+		{
+			nbody_log("Invalid min_force");
+			return nullptr;
+		}
+		nbody_engine_hip_ah* engine = new nbody_engine_hip_ah(static_cast<size_t>(rate), max_dist, min_force);
 		if(!common_hip_params(param, engine))
 		{
 			delete engine;

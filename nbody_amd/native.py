@@ -54,6 +54,11 @@ HIP_API = {
     "nbh_clamp": (i32, [vp, vp, sz, dbl]),
     "nbh_fcompute_direct": (i32, [vp, vp, vp, vp, sz, i32, i32]),
     "nbh_direct_sample": (i32, [vp, vp, vp, sz, vp, sz, vp]),
+    "nbh_ah_create": (i32, [vp, sz, C.POINTER(vp)]),
+    "nbh_ah_destroy": (i32, [vp]),
+    "nbh_ah_fcompute_full": (i32, [vp, vp, vp, vp, vp, dbl, dbl]),
+    "nbh_ah_fcompute_sparse": (i32, [vp, vp, vp, vp, vp]),
+    "nbh_ah_lists": (i32, [vp, vp, C.POINTER(sz), vp, vp, vp]),
     "nbh_tree_create": (i32, [vp, sz, C.POINTER(vp)]),
     "nbh_tree_destroy": (i32, [vp]),
     "nbh_tree_build": (i32, [vp, vp, vp, vp, dbl]),
@@ -142,6 +147,7 @@ HOST_API = {
     "nbs_engine_has_error": (i32, [vp]),
     "nbs_engine_hip_context": (vp, [vp]),
     "nbs_engine_hip_tree": (vp, [vp]),
+    "nbs_engine_hip_neighbours": (vp, [vp]),
     "nbs_memory_device_ptr": (vp, [vp]),
     "nbs_memory_size": (sz, [vp]),
     "nbs_memory_fake_create": (vp, [sz]),

@@ -130,6 +130,189 @@ void oracle_accel_targets(size_t n, const double* y, const double* mass,
 	}
 }
 
+/* nbody/nbody_engine_block.cpp:14-126: targets and sources in tiles of 64
+   bodies; per pair dr = r1 - r2, coeff = m2 / (sqrt(r2) * r2) on the clamped r2,
+   total -= dr * coeff; f[3n:6n] = total (no division by m1).  The self pair is
+   summed: its dr is +0, so it subtracts +0 and changes nothing.  The reference
+   reads past the end unless n % 64 == 0; here every n works and n % 64 == 0
+   gives the reference's sums (sources in index order for every target). */
+void oracle_fcompute_block(size_t n, const double* y, const double* mass, double* f, int threads)
+{
+	const double* rx = y;
+	const double* ry = y + n;
+	const double* rz = y + 2 * n;
+	long long nn = (long long)n;
+#ifdef _OPENMP
+	#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : omp_get_max_threads())
+#else
+	(void)threads;
+#endif
+	for(long long ii = 0; ii < nn; ++ii)
+	{
+		size_t i = (size_t)ii;
+		double tx = 0.0, ty = 0.0, tz = 0.0;
+		for(size_t j = 0; j < n; ++j)
+		{
+			double dx = rx[i] - rx[j];
+			double dy = ry[i] - ry[j];
+			double dz = rz[i] - rz[j];
+			double r2 = dx * dx + dy * dy + dz * dz;
+			if(r2 < ORACLE_MIN_DISTANCE)
+			{
+				r2 = ORACLE_MIN_DISTANCE;
+			}
+			double r = sqrt(r2);
+			double coeff = mass[j] / (r * r2);
+			tx -= dx * coeff;
+			ty -= dy * coeff;
+			tz -= dz * coeff;
+		}
+		f[i] = y[3 * n + i];
+		f[n + i] = y[4 * n + i];
+		f[2 * n + i] = y[5 * n + i];
+		f[3 * n + i] = tx;
+		f[4 * n + i] = ty;
+		f[5 * n + i] = tz;
+	}
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ahmad-Cohen neighbour scheme: nbody/nbody_engine_ah.cpp:16-160            */
+/* ------------------------------------------------------------------------ */
+/* Full pass (nbody_engine_ah.cpp:44-114): every pair as the simple engine;
+   body2 is a neighbour of body1 when |r1 - r2|^2 < max_dist_sqr or
+   |F|^2 > min_force_sqr (vertex3::norm is the squared length, vertex.h:151-154),
+   and the other bodies' forces are summed into the frozen far force univ.
+   Neighbour lists are stored in index order as CSR: offs[n + 1], list. */
+int oracle_ah_full(oracle_ah* h, const double* y, const double* mass, double* f)
+{
+	const size_t n = h->n;
+	const double* rx = y;
+	const double* ry = y + n;
+	const double* rz = y + 2 * n;
+	size_t total = 0;
+	for(int pass = 0; pass < 2; ++pass)
+	{
+		if(pass == 1)
+		{
+			free(h->list);
+			h->list = (unsigned*)malloc((total > 0 ? total : 1) * sizeof(unsigned));
+			if(h->list == NULL)
+			{
+				return -1;
+			}
+		}
+		total = 0;
+		for(size_t b1 = 0; b1 < n; ++b1)
+		{
+			const double v1[3] = { rx[b1], ry[b1], rz[b1] };
+			double tu[3] = { 0.0, 0.0, 0.0 };
+			double tf[3] = { 0.0, 0.0, 0.0 };
+			h->offs[b1] = total;
+			for(size_t b2 = 0; b2 != n; ++b2)
+			{
+				if(b1 == b2)
+				{
+					continue;
+				}
+				const double v2[3] = { rx[b2], ry[b2], rz[b2] };
+				double fr[3];
+				oracle_force(v1, v2, mass[b1], mass[b2], fr);
+				const double dx = v1[0] - v2[0], dy = v1[1] - v2[1], dz = v1[2] - v2[2];
+				const double d2 = dx * dx + dy * dy + dz * dz;
+				const double f2 = fr[0] * fr[0] + fr[1] * fr[1] + fr[2] * fr[2];
+				if(d2 < h->max_dist_sqr || f2 > h->min_force_sqr)
+				{
+					if(pass == 1)
+					{
+						h->list[total] = (unsigned)b2;
+					}
+					++total;
+				}
+				else
+				{
+					tu[0] += fr[0];
+					tu[1] += fr[1];
+					tu[2] += fr[2];
+				}
+				tf[0] += fr[0];
+				tf[1] += fr[1];
+				tf[2] += fr[2];
+			}
+			if(pass == 1)
+			{
+				f[b1] = y[3 * n + b1];
+				f[n + b1] = y[4 * n + b1];
+				f[2 * n + b1] = y[5 * n + b1];
+				f[3 * n + b1] = tf[0] / mass[b1];
+				f[4 * n + b1] = tf[1] / mass[b1];
+				f[5 * n + b1] = tf[2] / mass[b1];
+				h->univ[3 * b1] = tu[0];
+				h->univ[3 * b1 + 1] = tu[1];
+				h->univ[3 * b1 + 2] = tu[2];
+			}
+		}
+		h->offs[n] = total;
+	}
+	h->total = total;
+	return 0;
+}
+
+/* Sparse pass (nbody_engine_ah.cpp:116-160): the frozen far force plus the
+   neighbours' forces at the current positions, in list order. */
+void oracle_ah_sparse(const oracle_ah* h, const double* y, const double* mass, double* f)
+{
+	const size_t n = h->n;
+	const double* rx = y;
+	const double* ry = y + n;
+	const double* rz = y + 2 * n;
+	for(size_t b1 = 0; b1 < n; ++b1)
+	{
+		const double v1[3] = { rx[b1], ry[b1], rz[b1] };
+		double tf[3] = { h->univ[3 * b1], h->univ[3 * b1 + 1], h->univ[3 * b1 + 2] };
+		for(size_t k = h->offs[b1]; k < h->offs[b1 + 1]; ++k)
+		{
+			const size_t b2 = h->list[k];
+			const double v2[3] = { rx[b2], ry[b2], rz[b2] };
+			double fr[3];
+			oracle_force(v1, v2, mass[b1], mass[b2], fr);
+			tf[0] += fr[0];
+			tf[1] += fr[1];
+			tf[2] += fr[2];
+		}
+		f[b1] = y[3 * n + b1];
+		f[n + b1] = y[4 * n + b1];
+		f[2 * n + b1] = y[5 * n + b1];
+		f[3 * n + b1] = tf[0] / mass[b1];
+		f[4 * n + b1] = tf[1] / mass[b1];
+		f[5 * n + b1] = tf[2] / mass[b1];
+	}
+}
+
+int oracle_ah_alloc(oracle_ah* h, size_t n, double max_dist, double min_force)
+{
+	memset(h, 0, sizeof(*h));
+	h->n = n;
+	h->max_dist_sqr = max_dist * max_dist;     /* nbody_engine_ah.cpp:3-9 */
+	h->min_force_sqr = min_force * min_force;
+	h->offs = (size_t*)calloc(n + 1, sizeof(size_t));
+	h->univ = (double*)calloc(3 * (n > 0 ? n : 1), sizeof(double));
+	if(h->offs == NULL || h->univ == NULL)
+	{
+		oracle_ah_free(h);
+		return -1;
+	}
+	return 0;
+}
+
+void oracle_ah_free(oracle_ah* h)
+{
+	free(h->offs);
+	free(h->univ);
+	free(h->list);
+	memset(h, 0, sizeof(*h));
+}
+
 /* ------------------------------------------------------------------------ */
 /* vector ops                                                                */
 /* ------------------------------------------------------------------------ */

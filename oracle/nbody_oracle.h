@@ -56,6 +56,28 @@ void oracle_fcompute_openmp(size_t n, const double* y, const double* mass, doubl
 void oracle_accel_targets(size_t n, const double* y, const double* mass,
                           const size_t* idx, size_t ntgt, double* acc, int threads);
 
+/* nbody/nbody_engine_block.cpp:14-126: a_i = -sum_j (r_i - r_j) m_j / (r r2),
+   self pair included (it adds 0), no division by m_i. */
+void oracle_fcompute_block(size_t n, const double* y, const double* mass, double* f, int threads);
+
+/* ---- Ahmad-Cohen neighbour scheme (nbody/nbody_engine_ah.cpp:16-160) ---- */
+typedef struct oracle_ah {
+	size_t    n;
+	double    max_dist_sqr;
+	double    min_force_sqr;
+	size_t*   offs;     /* [n + 1] CSR offsets of the neighbour lists */
+	unsigned* list;     /* [total] neighbours of each body in index order */
+	double*   univ;     /* [3n] frozen far force (F, not F/m) */
+	size_t    total;
+} oracle_ah;
+
+int  oracle_ah_alloc(oracle_ah* h, size_t n, double max_dist, double min_force);
+void oracle_ah_free(oracle_ah* h);
+/* full recompute (fcompute_full, :44-114): f, neighbour lists and far forces */
+int  oracle_ah_full(oracle_ah* h, const double* y, const double* mass, double* f);
+/* between full recomputes (fcompute_sparse, :116-160) */
+void oracle_ah_sparse(const oracle_ah* h, const double* y, const double* mass, double* f);
+
 /* ---- engine vector ops over `count` doubles ---- */
 void   oracle_fill(double* a, double v, size_t count);
 void   oracle_fmadd_inplace(double* a, const double* b, double c, size_t count);        /* a += b*c */

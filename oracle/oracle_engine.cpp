@@ -1,5 +1,6 @@
 // oracle_engine.cpp -- TEST INFRASTRUCTURE.  The CPU restatement of the
-// reference's `simple` and `simple_bh` engines (nbody_oracle.c) behind the
+// reference's `simple`, `openmp`, `block`, `ah` and `simple_bh` engines
+// (nbody_oracle.c) behind the
 // product's nbody_engine interface, so the product's solvers can be run on the
 // oracle and compared with the HIP engines.  Loaded only by tests/ (and by the
 // cpu_baseline leg of bench.py); nothing in the product links it.
@@ -44,13 +45,19 @@ private:
 	oracle_heap m_heap;
 	bool        m_built;
 	int         m_threads;
+	int         m_direct;     // direct sums: 0 simple/openmp, 1 block, 2 ah
+	oracle_ah   m_ah;
+	double      m_ah_dist, m_ah_force;
 
 public:
-	oracle_engine(bool bh, double ratio, int layout, int traverse, size_t rate, int threads) :
+	oracle_engine(bool bh, double ratio, int layout, int traverse, size_t rate, int threads, int direct = 0,
+	              double ah_dist = 0, double ah_force = 0) :
 		m_data(nullptr), m_mass(nullptr), m_y(nullptr), m_bh(bh), m_ratio(ratio), m_layout(layout),
-		m_traverse(traverse), m_rate(rate), m_built(false), m_threads(threads)
+		m_traverse(traverse), m_rate(rate), m_built(false), m_threads(threads), m_direct(direct),
+		m_ah_dist(ah_dist), m_ah_force(ah_force)
 	{
 		std::memset(&m_heap, 0, sizeof(m_heap));
+		std::memset(&m_ah, 0, sizeof(m_ah));
 	}
 
 	~oracle_engine() override
@@ -58,13 +65,17 @@ public:
 		delete m_mass;
 		delete m_y;
 		oracle_heap_free(&m_heap);
+		oracle_ah_free(&m_ah);
 	}
+
+	const oracle_ah* ah() const { return &m_ah; }
 
 	const oracle_heap* heap() const { return &m_heap; }
 
 	const char* type_name() const override
 	{
-		return m_bh ? "oracle_engine_simple_bh" : "oracle_engine_simple";
+		if(m_bh) { return "oracle_engine_simple_bh"; }
+		return m_direct == 1 ? "oracle_engine_block" : m_direct == 2 ? "oracle_engine_ah" : "oracle_engine_simple";
 	}
 
 	bool init(nbody_data* data) override
@@ -85,6 +96,15 @@ public:
 			y[4 * n + i] = data->get_velosites()[i].y;
 			y[5 * n + i] = data->get_velosites()[i].z;
 			m_mass->data()[i] = data->get_mass()[i];
+		}
+		if(m_direct == 2)
+		{
+			oracle_ah_free(&m_ah);
+			if(oracle_ah_alloc(&m_ah, n, m_ah_dist, m_ah_force) != 0)
+			{
+				nbody_log("oracle: out of memory");
+				return false;
+			}
 		}
 		if(m_bh)
 		{
@@ -126,6 +146,27 @@ public:
 		if(f == nullptr) { nbody_log("f is not smemory"); return; }
 		advise_compute_count();
 		size_t n = m_data->get_count();
+		if(m_direct == 1)
+		{
+			oracle_fcompute_block(n, y->data(), m_mass->data(), f->data(), m_threads);
+			return;
+		}
+		if(m_direct == 2)
+		{
+			// nbody/nbody_engine_ah.cpp:16-42: full recompute every rate-th step
+			if(m_data->get_step() % m_rate == 0)
+			{
+				if(oracle_ah_full(&m_ah, y->data(), m_mass->data(), f->data()) != 0)
+				{
+					nbody_log("oracle: out of memory");
+				}
+			}
+			else
+			{
+				oracle_ah_sparse(&m_ah, y->data(), m_mass->data(), f->data());
+			}
+			return;
+		}
 		if(!m_bh)
 		{
 			if(m_threads == 1)
@@ -251,6 +292,39 @@ nbs_engine* oracle_engine_create_simple_bh(double ratio, int layout, int travers
 {
 	return reinterpret_cast<nbs_engine*>(static_cast<nbody_engine*>(
 		new oracle_engine(true, ratio, layout, traverse, rate, threads)));
+}
+
+// nbody/nbody_engines.cpp:16-19 (`block`, OpenMP over target tiles)
+nbs_engine* oracle_engine_create_block(int threads)
+{
+	return reinterpret_cast<nbs_engine*>(static_cast<nbody_engine*>(
+		new oracle_engine(false, 0, 0, 0, 0, threads, 1)));
+}
+
+// nbody/nbody_engines.cpp:8-15 (`ah`: full_recompute_rate, max_dist, min_force)
+nbs_engine* oracle_engine_create_ah(size_t rate, double max_dist, double min_force)
+{
+	if(rate == 0)
+	{
+		return nullptr;
+	}
+	return reinterpret_cast<nbs_engine*>(static_cast<nbody_engine*>(
+		new oracle_engine(false, 0, 0, 0, rate, 1, 2, max_dist, min_force)));
+}
+
+// Neighbour lists of the last full recompute of an `ah` oracle engine: the
+// total count, and with non-null arrays the CSR offsets [n + 1] and the list.
+size_t oracle_engine_ah_lists(nbs_engine* e, size_t* offs, unsigned* list)
+{
+	const oracle_engine* o = dynamic_cast<const oracle_engine*>(reinterpret_cast<nbody_engine*>(e));
+	if(o == nullptr || o->ah()->offs == nullptr)
+	{
+		return 0;
+	}
+	const oracle_ah* h = o->ah();
+	if(offs != nullptr) { std::memcpy(offs, h->offs, (h->n + 1) * sizeof(size_t)); }
+	if(list != nullptr && h->list != nullptr) { std::memcpy(list, h->list, h->total * sizeof(unsigned)); }
+	return h->total;
 }
 
 // Copy the last built heap (tree size 2n, xyz interleaved); body_n as int64 (-1 internal).

@@ -23,6 +23,11 @@ _lib = None
 _elib = None
 
 
+class ah_t(C.Structure):
+    _fields_ = [("n", sz), ("max_dist_sqr", C.c_double), ("min_force_sqr", C.c_double),
+                ("offs", C.POINTER(sz)), ("list", C.POINTER(C.c_uint)), ("univ", PD), ("total", sz)]
+
+
 class heap_t(C.Structure):
     _fields_ = [("count", sz), ("size", sz), ("cm", PD), ("mass", PD), ("crit_r2", PD),
                 ("bmin", PD), ("bmax", PD), ("body_n", C.POINTER(sz)), ("ratio_sqr", C.c_double)]
@@ -40,6 +45,11 @@ def lib():
             "oracle_fcompute_simple": (None, [sz, PD, PD, PD]),
             "oracle_fcompute_openmp": (None, [sz, PD, PD, PD, C.c_int]),
             "oracle_accel_targets": (None, [sz, PD, PD, C.POINTER(sz), sz, PD, C.c_int]),
+            "oracle_fcompute_block": (None, [sz, PD, PD, PD, C.c_int]),
+            "oracle_ah_alloc": (C.c_int, [C.POINTER(ah_t), sz, C.c_double, C.c_double]),
+            "oracle_ah_free": (None, [C.POINTER(ah_t)]),
+            "oracle_ah_full": (C.c_int, [C.POINTER(ah_t), PD, PD, PD]),
+            "oracle_ah_sparse": (None, [C.POINTER(ah_t), PD, PD, PD]),
             "oracle_fill": (None, [PD, C.c_double, sz]),
             "oracle_fmadd_inplace": (None, [PD, PD, C.c_double, sz]),
             "oracle_fmadd": (None, [PD, PD, PD, C.c_double, sz]),
@@ -85,6 +95,12 @@ def elib():
         L.oracle_engine_create_simple.argtypes = [C.c_int]
         L.oracle_engine_create_simple_bh.restype = C.c_void_p
         L.oracle_engine_create_simple_bh.argtypes = [C.c_double, C.c_int, C.c_int, sz, C.c_int]
+        L.oracle_engine_create_block.restype = C.c_void_p
+        L.oracle_engine_create_block.argtypes = [C.c_int]
+        L.oracle_engine_create_ah.restype = C.c_void_p
+        L.oracle_engine_create_ah.argtypes = [sz, C.c_double, C.c_double]
+        L.oracle_engine_ah_lists.restype = sz
+        L.oracle_engine_ah_lists.argtypes = [C.c_void_p, C.POINTER(sz), C.POINTER(C.c_uint)]
         L.oracle_engine_heap.restype = C.c_int
         L.oracle_engine_heap.argtypes = [C.c_void_p, C.POINTER(C.c_int64), PD, PD, PD, PD, PD]
         _elib = L
@@ -99,6 +115,25 @@ def p(a: np.ndarray):
 def simple_engine(threads: int = 1) -> nbody_engine:
     """The reference's `simple` engine (threads=1) or `openmp` engine (threads>1)."""
     return nbody_engine(elib().oracle_engine_create_simple(threads))
+
+
+def block_engine(threads: int = 1) -> nbody_engine:
+    """The reference's `block` engine (nbody/nbody_engine_block.cpp:14-126)."""
+    return nbody_engine(elib().oracle_engine_create_block(threads))
+
+
+def ah_engine(rate: int = 1000, max_dist: float = 10.0, min_force: float = 1e-4) -> nbody_engine:
+    """The reference's `ah` engine (nbody/nbody_engine_ah.cpp:3-160), same defaults."""
+    return nbody_engine(elib().oracle_engine_create_ah(rate, max_dist, min_force))
+
+
+def ah_engine_lists(e: nbody_engine, n: int):
+    """(offsets[n + 1], list) of the last full recompute of an oracle `ah` engine."""
+    total = elib().oracle_engine_ah_lists(e.handle, None, None)
+    offs = np.empty(n + 1, dtype=np.uint64)
+    lst = np.empty(max(total, 1), dtype=np.uint32)
+    elib().oracle_engine_ah_lists(e.handle, offs.ctypes.data_as(C.POINTER(sz)), lst.ctypes.data_as(C.POINTER(C.c_uint)))
+    return offs, lst[:total]
 
 
 def simple_bh_engine(ratio: float, layout: str = "heap", traverse: str = "nested_tree",
@@ -128,6 +163,42 @@ def fcompute_simple(y: np.ndarray, mass: np.ndarray, threads: int = 1) -> np.nda
     else:
         lib().oracle_fcompute_openmp(n, p(y), p(mass), p(f), threads)
     return f
+
+
+def fcompute_block(y: np.ndarray, mass: np.ndarray, threads: int = 1) -> np.ndarray:
+    n = len(mass)
+    f = np.empty(6 * n)
+    lib().oracle_fcompute_block(n, p(y), p(mass), p(f), threads)
+    return f
+
+
+class AH:
+    """oracle_ah owner: the Ahmad-Cohen full and sparse passes."""
+
+    def __init__(self, n: int, max_dist: float, min_force: float):
+        self.h = ah_t()
+        assert lib().oracle_ah_alloc(C.byref(self.h), n, max_dist, min_force) == 0
+        self.n = n
+
+    def __del__(self):
+        lib().oracle_ah_free(C.byref(self.h))
+
+    def full(self, y: np.ndarray, mass: np.ndarray) -> np.ndarray:
+        f = np.empty(6 * self.n)
+        assert lib().oracle_ah_full(C.byref(self.h), p(y), p(np.ascontiguousarray(mass)), p(f)) == 0
+        return f
+
+    def sparse(self, y: np.ndarray, mass: np.ndarray) -> np.ndarray:
+        f = np.empty(6 * self.n)
+        lib().oracle_ah_sparse(C.byref(self.h), p(y), p(np.ascontiguousarray(mass)), p(f))
+        return f
+
+    def lists(self):
+        n, total = self.n, self.h.total
+        offs = np.ctypeslib.as_array(self.h.offs, shape=(n + 1,)).astype(np.uint64)
+        lst = np.ctypeslib.as_array(self.h.list, shape=(total,)).copy() if total else np.empty(0, np.uint32)
+        univ = np.ctypeslib.as_array(self.h.univ, shape=(3 * n,)).reshape(n, 3).copy()
+        return offs, lst, univ
 
 
 def accel_targets(y: np.ndarray, mass: np.ndarray, idx: np.ndarray, threads: int = 0) -> np.ndarray:

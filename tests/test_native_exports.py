@@ -51,6 +51,11 @@ def test_factory_validates_before_device():
     assert nbody_create_engine({"engine": "hip_bh", "tree_build_rate": "-1"}) is None
     assert nbody_create_engine({"engine": "hip_bh", "distance_to_node_radius_ratio": "0"}) is None
     assert nbody_create_engine({"engine": "hip_bh", "force_precision": "fp32_corr"}) is None   # direct only
+    assert nbody_create_engine({"engine": "hip_block", "force_precision": "fp32"}) is None     # FP64 only
+    assert nbody_create_engine({"engine": "hip_ah", "force_precision": "fp32"}) is None
+    assert nbody_create_engine({"engine": "hip_ah", "full_recompute_rate": 0}) is None         # reference: x % 0
+    assert nbody_create_engine({"engine": "hip_ah", "max_dist": "-1"}) is None
+    assert nbody_create_engine({"engine": "hip_ah", "min_force": "abc"}) is None
     grp = {"engine": "hip_bh", "force_precision": "fp32", "bh_walk": "group"}
     assert nbody_create_engine({**grp, "bh_far_field": "tree"}) is None
     assert nbody_create_engine({**grp, "bh_far_field": "local"}) is None                     # needs quadrupole
