@@ -961,20 +961,31 @@ __device__ inline void sweep32(const float4* __restrict__ buf, int count, float 
 	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
 }
 
-// Monopole + quadrupole field of the cells in qb at the lane's body.  Entries
-// of 3 float4: (mass-centre offset from the group centre, M), 2.5 x (Q'xx Q'yy
-// Q'zz Q'xy), 2.5 x (Q'xz Q'yz - -), with Q' = 3 Q - tr(Q) I the traceless second
-// moment about the mass centre; r = cm - x, d = |r|, u = 1/d:
+// Cells waiting for the monopole + quadrupole sweep, 40 bytes each in LDS:
+// e0 = (mass-centre offset from the group centre, M), e1 = 2.5 x (Q'xx Q'yy
+// Q'zz Q'xy), e2 = 2.5 x (Q'xz Q'yz), with Q' = 3 Q - tr(Q) I the traceless
+// second moment about the mass centre (all scaled as the sweeps are).
+template <int kN>
+struct quad_lds {
+	float4 e0[kN];
+	float4 e1[kN];
+	float2 e2[kN];
+};
+
+// Monopole + quadrupole field of the cells in qb at the lane's body; with
+// r = cm - x, d = |r|, u = 1/d:
 //   a = M r / d^3 + (2.5 (r.Q'.r) / d^2 r - Q' r) / d^5
 //     = (M + s u^4) u^3 r - 0.4 u^5 (2.5 Q' r),   s = r.(2.5 Q').r,
 // all in the scaled units of sweep32 (Q' times kSweepQuad).  Kahan per sweep.
-__device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float tx, float ty, float tz,
-                                float& ax, float& ay, float& az, float& cx, float& cy, float& cz)
+template <int kN>
+__device__ inline void sweep_mq(const quad_lds<kN>& qb, int count, float tx, float ty, float tz, float& ax, float& ay,
+                                float& az, float& cx, float& cy, float& cz)
 {
 	float px = 0.0f, py = 0.0f, pz = 0.0f;
 	for(int j = 0; j < count; ++j)
 	{
-		const float4 e0 = qb[3 * j], e1 = qb[3 * j + 1], e2 = qb[3 * j + 2];
+		const float4 e0 = qb.e0[j], e1 = qb.e1[j];
+		const float2 e2 = qb.e2[j];
 		const float rx = e0.x - tx, ry = e0.y - ty, rz = e0.z - tz;
 		const float u = rsq_clamped(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
 		const float u2 = u * u;
@@ -995,18 +1006,21 @@ __device__ inline void sweep_mq(const float4* __restrict__ qb, int count, float 
 	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
 }
 
-constexpr int kQuadBuf = 96;     // cells waiting for the monopole + quadrupole sweep
+// Cells waiting for the monopole + quadrupole sweep; a sweep runs before a
+// step whenever more than kQuadBuf - 64 wait.
+constexpr int kQuadBuf = 88;
 
 // Entry of a node (scaled mass-centre offset and mass in ent, second moments Q)
 // at slot `slot` of qb, with the 2.5 x Q' that sweep_mq expects.
-__device__ inline void put_quad(float4* qb, int slot, const float4& ent, const float* __restrict__ Q)
+template <int kN>
+__device__ inline void put_quad(quad_lds<kN>& qb, int slot, const float4& ent, const float* __restrict__ Q)
 {
 	const float q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5];
 	const float tr = q0 + q1 + q2;
 	const float k = static_cast<float>(2.5 * kSweepQuad);
-	qb[3 * slot] = ent;
-	qb[3 * slot + 1] = make_float4((3.0f * q0 - tr) * k, (3.0f * q1 - tr) * k, (3.0f * q2 - tr) * k, 3.0f * q3 * k);
-	qb[3 * slot + 2] = make_float4(3.0f * q4 * k, 3.0f * q5 * k, 0.0f, 0.0f);
+	qb.e0[slot] = ent;
+	qb.e1[slot] = make_float4((3.0f * q0 - tr) * k, (3.0f * q1 - tr) * k, (3.0f * q2 - tr) * k, 3.0f * q3 * k);
+	qb.e2[slot] = make_float2(3.0f * q4 * k, 3.0f * q5 * k);
 }
 
 // Quadrupole field at the group centre of a cell whose mass centre sits at r
@@ -1143,24 +1157,28 @@ __device__ inline void local_terms(const float* T, float dx, float dy, float dz,
 	oz += s5 * dz - 7.5f * dd * adz + 17.5f * wz;
 }
 
-// Far cells wait in an LDS ring of kFarRing entries -- the unscaled offset of the
-// mass centre from the group centre and the mass as a float4, and the node --
-// and are expanded 64 at a time, one per lane, so the expansion runs on full
-// waves (the second moments are read from the tree there).  A walk step adds at
-// most 64 entries and a batch is taken as soon as 64 wait, so at most 127 are
-// ever pending.
+// Far cells wait in an LDS ring of kFarRing node numbers and are expanded 64 at
+// a time, one per lane, so the expansion runs on full waves; the expansion
+// reads the node's mass centre, mass and second moments from its record again
+// (one cache line the walk step has just read).  A walk step adds at most 64
+// entries and a batch is taken as soon as 64 wait, so at most 127 are ever
+// pending.
 constexpr int kFarRing = 128;
 
-// Expand `count` (<= 64) ring entries from `head`, one per lane.
+// Expand `count` (<= 64) ring entries from `head`, one per lane, about the
+// centre (ccx, ccy, ccz): offsets of the mass centres from it, unscaled.
 template <bool kLocal>
-__device__ inline void far_batch(const float4* fq, const int* fn, const nbh_walk_rec* __restrict__ wr, int head,
-                                 int count, int lane, float* acc)
+__device__ inline void far_batch(const int* fn, const nbh_walk_rec* __restrict__ wr, int head, int count, int lane,
+                                 double ccx, double ccy, double ccz, float* acc)
 {
 	if(lane < count)
 	{
 		const int s = (head + lane) & (kFarRing - 1);
-		const float4 e = fq[s];
-		const float* Q = wr[fn[s]].q;
+		const nbh_walk_rec* r = wr + fn[s];
+		const double4 c = r->c;
+		const float4 e = make_float4(static_cast<float>(c.x - ccx), static_cast<float>(c.y - ccy),
+		                             static_cast<float>(c.z - ccz), static_cast<float>(r->m));
+		const float* Q = r->q;
 		const float q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], q4 = Q[4], q5 = Q[5];
 		const float tr = q0 + q1 + q2;
 		quad_expansion(e.x, e.y, e.z, 3.0f * q0 - tr, 3.0f * q1 - tr, 3.0f * q2 - tr, 3.0f * q3, 3.0f * q4, 3.0f * q5,
@@ -1233,8 +1251,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
-	__shared__ float4 qbuf[kQuad ? 3 * kQuadBuf : 1];
-	__shared__ float4 farq[kQuad ? kFarRing : 1];
+	__shared__ quad_lds<kQuad ? kQuadBuf : 1> qbuf;
 	__shared__ int farn[kQuad ? kFarRing : 1];
 	const int lane = threadIdx.x;
 	const int gi = xcd_group(blockIdx.x, gridDim.x);    // group ordinal
@@ -1303,7 +1320,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			}
 			if(ftail - fhead >= kGroup)
 			{
-				far_batch<kLocal>(farq, farn, wr, fhead, kGroup, lane, acc);
+				far_batch<kLocal>(farn, wr, fhead, kGroup, lane, ccx, ccy, ccz, acc);
 				fhead += kGroup;
 			}
 		}
@@ -1352,10 +1369,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			else if(accept && !(kLocal && far)) { buf[nbuf + __popcll(bmono & lt)] = ent; }
 			if(far)
 			{
-				const int slot = (ftail + __popcll(bfar & lt)) & (kFarRing - 1);
-				farq[slot] = make_float4(static_cast<float>(rec.x - ccx), static_cast<float>(rec.y - ccy),
-				                         static_cast<float>(rec.z - ccz), static_cast<float>(m));
-				farn[slot] = node;
+				farn[(ftail + __popcll(bfar & lt)) & (kFarRing - 1)] = node;
 			}
 			ftail += __popcll(bfar);
 			nq += __popcll(bnear);
@@ -1384,7 +1398,8 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 		if(nq > 0) { sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz); }
 		for(; fhead < ftail; fhead += kGroup)
 		{
-			far_batch<kLocal>(farq, farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, acc);
+			far_batch<kLocal>(farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, ccx, ccy, ccz,
+			                  acc);
 		}
 		for(int sh = 1; sh < 64; sh <<= 1)
 		{
@@ -1461,7 +1476,6 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
            int stack_limit)
 {
 	__shared__ int stack[kWalkStack];
-	__shared__ float4 farq[kFarRing];
 	__shared__ int farn[kFarRing];
 	const int lane = threadIdx.x;
 	const int si = xcd_group(blockIdx.x, gridDim.x);
@@ -1491,7 +1505,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 		walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
 		if(ftail - fhead >= kGroup)
 		{
-			far_batch<true>(farq, farn, wr, fhead, kGroup, lane, acc);
+			far_batch<true>(farn, wr, fhead, kGroup, lane, ccx, ccy, ccz, acc);
 			fhead += kGroup;
 		}
 		__syncthreads();
@@ -1531,9 +1545,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 		const uint64_t blist = __ballot(tolist);
 		if(far)
 		{
-			const int slot = (ftail + __popcll(bfar & lt)) & (kFarRing - 1);
-			farq[slot] = make_float4(ux, uy, uz, static_cast<float>(m));
-			farn[slot] = node;
+			farn[(ftail + __popcll(bfar & lt)) & (kFarRing - 1)] = node;
 			if(two.far_list != nullptr)
 			{
 				const int fslot = nfar_listed + __popcll(bfar & lt);
@@ -1556,7 +1568,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 	}
 	for(; fhead < ftail; fhead += kGroup)
 	{
-		far_batch<true>(farq, farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, acc);
+		far_batch<true>(farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, ccx, ccy, ccz, acc);
 	}
 	for(int sh = 1; sh < 64; sh <<= 1)
 	{
@@ -1911,7 +1923,7 @@ k_bh_group32_mlp(const nbh_walk_rec* __restrict__ wr, const double* __restrict__
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
-	__shared__ float4 qbuf[3 * kQuadBuf];
+	__shared__ quad_lds<kQuadBuf> qbuf;
 	__shared__ __align__(16) uint16_t bandf[kBand * nbh_net::kIn];
 	__shared__ float bandw[2 * kBand];      // per far row: scale, scale / (kJac d)
 	__shared__ float bandb[kBand * nbh_net::kBasis];
