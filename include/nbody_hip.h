@@ -129,6 +129,19 @@ nbh_status nbh_clamp(nbh_ctx* ctx, double* y, size_t nbodies, double b);
 nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const double* mass, double* f,
                                size_t n, nbh_precision precision, int block_size);
 
+/* nbh_fcompute_direct without any device allocation: no counterpart in the
+   reference.  For callers that cannot allocate (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+   the same result bits as nbh_fcompute_direct; the FP32 paths keep their
+   device scratch (sort keys, Morton-ordered records, split partial sums) in
+   one fixed 64 MiB array per device instead of the context's grown buffer, so
+   n is limited to 131072 bodies (NBH_ERR_INVALID beyond, or when the scratch
+   the device's CU count asks for exceeds the array).  The array is shared by
+   every context on a device and keeps the last call's contents: calls on one
+   device whose executions overlap (different streams or threads) race. */
+nbh_status nbh_fcompute_direct_noalloc(nbh_ctx* ctx, const double* y, const double* mass, double* f,
+                                       size_t n, nbh_precision precision, int block_size);
+
 /* FP64 accelerations of the m bodies listed in idx (device array of indices < n):
    acc[3t..3t+2] for body idx[t], same operations and source order as
    nbody_engine_simple::fcompute.  Used to measure the error of the FP32 and

@@ -22,6 +22,7 @@
 // order by a second kernel, which also copies the velocities into f[0:3n].
 
 #include <cmath>
+#include <mutex>
 
 #include "nbh_common.h"
 #include "nbh_f64.h"
@@ -537,6 +538,85 @@ k_morton_keys(const double* __restrict__ y, size_t n, const double* __restrict__
 	}
 }
 
+// Launch shape and scratch layout of the FP32 path for n bodies: the sources
+// are split so that the grid holds ~8 work-groups per CU; the scratch holds
+// the split partial sums, the Morton-ordered records, the sort keys / values
+// (two halves each), sort counts, scan chunk sums, bounding-box partials and
+// the box, each at a 256-byte boundary.
+struct fp32_layout {
+	size_t gx, nsplit, jchunk;
+	unsigned gbox;
+	size_t off_sorted, off_keys, off_vals, off_counts, off_scan, off_bpart, off_box, bytes;
+};
+
+fp32_layout fp32_plan(const nbh_ctx* ctx, size_t n)
+{
+	fp32_layout l;
+	l.gx = (n + kF32TgtPerGroup - 1) / kF32TgtPerGroup;
+	const size_t want = static_cast<size_t>(ctx->cu_count > 0 ? ctx->cu_count : 256) * 8;
+	size_t nsplit = (want + l.gx - 1) / l.gx;
+	const size_t max_split = (n + kF32Tile - 1) / kF32Tile;
+	if(nsplit > max_split) { nsplit = max_split; }
+	if(nsplit > 64) { nsplit = 64; }
+	if(nsplit < 1) { nsplit = 1; }
+	size_t jchunk = (n + nsplit - 1) / nsplit;
+	l.jchunk = (jchunk + kF32Tile - 1) / kF32Tile * kF32Tile;
+	l.nsplit = (n + l.jchunk - 1) / l.jchunk;
+	l.gbox = nbh::grid_for(ctx, n, 256);
+	auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+	l.off_sorted = align(l.nsplit * 3 * n * sizeof(double));
+	l.off_keys = l.off_sorted + align(n * sizeof(double4));
+	l.off_vals = l.off_keys + align(2 * n * sizeof(uint64_t));
+	l.off_counts = l.off_vals + align(2 * n * sizeof(uint32_t));
+	l.off_scan = l.off_counts + align(nbh::sort_counts_size(n) * sizeof(uint32_t));
+	l.off_bpart = l.off_scan + align(nbh::sort_scan_size() * sizeof(uint32_t));
+	l.off_box = l.off_bpart + align(6 * static_cast<size_t>(l.gbox) * sizeof(double));
+	l.bytes = l.off_box + align(6 * sizeof(double));
+	return l;
+}
+
+// The FP32 path with its scratch at `base` (l.bytes bytes).  Every scratch
+// word it reads was written earlier in the same call, so the scratch needs no
+// clearing.  timed: bracket the tile kernel with the context's timer.
+nbh_status fp32_run(nbh_ctx* ctx, const double* y, const double* mass, double* f, size_t n, nbh_precision precision,
+                    const fp32_layout& l, char* base, bool timed)
+{
+	double* part = reinterpret_cast<double*>(base);
+	double4* sorted = reinterpret_cast<double4*>(base + l.off_sorted);
+	uint64_t* keys = reinterpret_cast<uint64_t*>(base + l.off_keys);
+	uint32_t* vals = reinterpret_cast<uint32_t*>(base + l.off_vals);
+	uint32_t* counts = reinterpret_cast<uint32_t*>(base + l.off_counts);
+	uint32_t* scanp = reinterpret_cast<uint32_t*>(base + l.off_scan);
+	double* bpart = reinterpret_cast<double*>(base + l.off_bpart);
+	double* box = reinterpret_cast<double*>(base + l.off_box);
+	hipLaunchKernelGGL(k_bbox_partial, dim3(l.gbox), dim3(256), 0, ctx->stream, y, n, bpart);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, ctx->stream, bpart, static_cast<int>(l.gbox), box);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_morton_keys, dim3(l.gbox), dim3(256), 0, ctx->stream, y, n, box, keys, vals);
+	NBH_LAUNCHED();
+	uint32_t* perm = nullptr;
+	NBH_TRY(nbh::sort_pairs(ctx, keys, vals, keys + n, vals + n, n, 30, counts, scanp, &perm));
+	hipLaunchKernelGGL(k_sorted_copy, dim3(l.gbox), dim3(256), 0, ctx->stream, y, mass, perm, sorted, n);
+	NBH_LAUNCHED();
+	{
+		nbh::scoped_timer timer(timed ? ctx : nullptr, "direct_fp32");
+		hipLaunchKernelGGL(k_direct_fp32, dim3(static_cast<unsigned>(l.gx), static_cast<unsigned>(l.nsplit)),
+		                   dim3(kF32Threads), 0, ctx->stream, sorted, part, n, l.jchunk);
+		NBH_LAUNCHED();
+	}
+	hipLaunchKernelGGL(k_direct_fp32_finish, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0,
+	                   ctx->stream, y, perm, part, f, n, static_cast<int>(l.nsplit));
+	NBH_LAUNCHED();
+	if(precision == NBH_FP32_CORR)
+	{
+		hipLaunchKernelGGL(k_direct_window, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream,
+		                   sorted, perm, f, n, kCorrWindow);
+		NBH_LAUNCHED();
+	}
+	return NBH_OK;
+}
+
 }  // namespace
 
 extern "C" nbh_status nbh_direct_sample(nbh_ctx* ctx, const double* y, const double* mass, size_t n,
@@ -582,63 +662,73 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 	}
 	NBH_CHECK(precision == NBH_FP32 || precision == NBH_FP32_CORR, "unknown precision %d",
 	          static_cast<int>(precision));
-	// Split the sources so that the grid holds ~8 work-groups per CU.
-	size_t gx = (n + kF32TgtPerGroup - 1) / kF32TgtPerGroup;
-	size_t want = static_cast<size_t>(ctx->cu_count > 0 ? ctx->cu_count : 256) * 8;
-	size_t nsplit = (want + gx - 1) / gx;
-	size_t max_split = (n + kF32Tile - 1) / kF32Tile;
-	if(nsplit > max_split) { nsplit = max_split; }
-	if(nsplit > 64) { nsplit = 64; }
-	if(nsplit < 1) { nsplit = 1; }
-	size_t jchunk = (n + nsplit - 1) / nsplit;
-	jchunk = (jchunk + kF32Tile - 1) / kF32Tile * kF32Tile;
-	nsplit = (n + jchunk - 1) / jchunk;
-	// scratch: partial sums, sort keys / values (two halves each), sort counts,
-	// scan chunk sums, bounding-box partials and the box
-	const unsigned gbox = nbh::grid_for(ctx, n, 256);
-	auto align = [](size_t b) { return (b + 255) / 256 * 256; };
-	const size_t off_sorted = align(nsplit * 3 * n * sizeof(double));
-	const size_t off_keys = off_sorted + align(n * sizeof(double4));
-	const size_t off_vals = off_keys + align(2 * n * sizeof(uint64_t));
-	const size_t off_counts = off_vals + align(2 * n * sizeof(uint32_t));
-	const size_t off_scan = off_counts + align(nbh::sort_counts_size(n) * sizeof(uint32_t));
-	const size_t off_bpart = off_scan + align(nbh::sort_scan_size() * sizeof(uint32_t));
-	const size_t off_box = off_bpart + align(6 * static_cast<size_t>(gbox) * sizeof(double));
+	const fp32_layout lay = fp32_plan(ctx, n);
 	void* sp = nullptr;
-	NBH_TRY(nbh::scratch(ctx, off_box + align(6 * sizeof(double)), &sp));
-	char* base = static_cast<char*>(sp);
-	double* part = reinterpret_cast<double*>(base);
-	double4* sorted = reinterpret_cast<double4*>(base + off_sorted);
-	uint64_t* keys = reinterpret_cast<uint64_t*>(base + off_keys);
-	uint32_t* vals = reinterpret_cast<uint32_t*>(base + off_vals);
-	uint32_t* counts = reinterpret_cast<uint32_t*>(base + off_counts);
-	uint32_t* scanp = reinterpret_cast<uint32_t*>(base + off_scan);
-	double* bpart = reinterpret_cast<double*>(base + off_bpart);
-	double* box = reinterpret_cast<double*>(base + off_box);
-	hipLaunchKernelGGL(k_bbox_partial, dim3(gbox), dim3(256), 0, ctx->stream, y, n, bpart);
-	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, ctx->stream, bpart, static_cast<int>(gbox), box);
-	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_morton_keys, dim3(gbox), dim3(256), 0, ctx->stream, y, n, box, keys, vals);
-	NBH_LAUNCHED();
-	uint32_t* perm = nullptr;
-	NBH_TRY(nbh::sort_pairs(ctx, keys, vals, keys + n, vals + n, n, 30, counts, scanp, &perm));
-	hipLaunchKernelGGL(k_sorted_copy, dim3(gbox), dim3(256), 0, ctx->stream, y, mass, perm, sorted, n);
-	NBH_LAUNCHED();
+	NBH_TRY(nbh::scratch(ctx, lay.bytes, &sp));
+	return fp32_run(ctx, y, mass, f, n, precision, lay, static_cast<char*>(sp), true);
+}
+
+// ---------------------------------------------------------------------------
+// nbh_fcompute_direct_noalloc: the same computation with its device scratch in
+// a fixed array of this library, for callers that cannot allocate.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocBytes = static_cast<size_t>(64) << 20;
+constexpr size_t kNoAllocMaxBodies = 131072;
+constexpr int kNoAllocDevices = 64;
+
+}  // namespace
+
+// One copy per device, held from the loading of this library until the process
+// ends.  The name is a host-side handle: kernels get the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kNoAllocDevices] = {};
+
+// Device address of `scratch` on ctx's device (current after nbh::enter),
+// looked up on the first call for that device.
+nbh_status noalloc_scratch(nbh_ctx* ctx, char** out)
+{
+	NBH_CHECK(ctx->device >= 0 && ctx->device < kNoAllocDevices, "device %d out of range", ctx->device);
+	std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+	if(g_noalloc_ptr[ctx->device] == nullptr)
 	{
-		nbh::scoped_timer timer(ctx, "direct_fp32");
-		hipLaunchKernelGGL(k_direct_fp32, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(nsplit)),
-		                   dim3(kF32Threads), 0, ctx->stream, sorted, part, n, jchunk);
-		NBH_LAUNCHED();
+		void* p = nullptr;
+		NBH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		NBH_CHECK(p != nullptr, "no device address for the scratch array");
+		g_noalloc_ptr[ctx->device] = p;
 	}
-	hipLaunchKernelGGL(k_direct_fp32_finish, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0,
-	                   ctx->stream, y, perm, part, f, n, static_cast<int>(nsplit));
-	NBH_LAUNCHED();
-	if(precision == NBH_FP32_CORR)
-	{
-		hipLaunchKernelGGL(k_direct_window, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream,
-		                   sorted, perm, f, n, kCorrWindow);
-		NBH_LAUNCHED();
-	}
+	*out = static_cast<char*>(g_noalloc_ptr[ctx->device]);
 	return NBH_OK;
+}
+
+}  // namespace
+
+extern "C" nbh_status nbh_fcompute_direct_noalloc(nbh_ctx* ctx, const double* y, const double* mass, double* f,
+                                                  size_t n, nbh_precision precision, int block_size)
+{
+	NBH_TRY(nbh::enter(ctx));
+	if(n == 0) { return NBH_OK; }
+	NBH_CHECK(y != nullptr && mass != nullptr && f != nullptr, "null buffer");
+	NBH_CHECK(y != f, "y and f must not alias");
+	NBH_CHECK(n <= kNoAllocMaxBodies, "n = %zu exceeds the %zu bodies of the allocation-free direct sum", n,
+	          kNoAllocMaxBodies);
+	if(precision == NBH_FP64 || precision == NBH_FP64_BLOCK)
+	{
+		// the FP64 kernels use no scratch: the allocating entry point is allocation-free here
+		return nbh_fcompute_direct(ctx, y, mass, f, n, precision, block_size);
+	}
+	NBH_CHECK(precision == NBH_FP32 || precision == NBH_FP32_CORR, "unknown precision %d",
+	          static_cast<int>(precision));
+	const fp32_layout lay = fp32_plan(ctx, n);
+	NBH_CHECK(lay.bytes <= kNoAllocBytes, "n = %zu needs %zu scratch bytes, more than the %zu of the fixed array", n,
+	          lay.bytes, kNoAllocBytes);
+	char* base = nullptr;
+	NBH_TRY(noalloc_scratch(ctx, &base));
+	return fp32_run(ctx, y, mass, f, n, precision, lay, base, false);
 }

@@ -53,6 +53,7 @@ HIP_API = {
     "nbh_snapshot_wait": (i32, [vp, vp, i32, C.POINTER(vp)]),
     "nbh_clamp": (i32, [vp, vp, sz, dbl]),
     "nbh_fcompute_direct": (i32, [vp, vp, vp, vp, sz, i32, i32]),
+    "nbh_fcompute_direct_noalloc": (i32, [vp, vp, vp, vp, sz, i32, i32]),
     "nbh_direct_sample": (i32, [vp, vp, vp, sz, vp, sz, vp]),
     "nbh_ah_create": (i32, [vp, sz, C.POINTER(vp)]),
     "nbh_ah_destroy": (i32, [vp]),
