@@ -234,6 +234,8 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
         tb, cb, tw, cw = C.c_double(), C.c_uint64(), C.c_double(), C.c_uint64()
         native.check(hip.nbh_timing_read(ctx, b"tree_build", C.byref(tb), C.byref(cb)))
         native.check(hip.nbh_timing_read(ctx, b"bh_traverse", C.byref(tw), C.byref(cw)))
+        tg, cg = C.c_double(), C.c_uint64()
+        native.check(hip.nbh_timing_read(ctx, b"bh_group_walk", C.byref(tg), C.byref(cg)))
         native.check(hip.nbh_timing_enable(ctx, 0))
         cnt = (C.c_uint64 * 5)()
         native.check(hip.nbh_bh_walk_counters(ctx, e.hip_tree(), cnt))
@@ -274,7 +276,8 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             "far_field": params.get("bh_far_field", "per_body") if "bh_multipole" in params else None,
             "steps_per_s": steps / wall,
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
-            "walk_ms": walk_ms, "interactions_per_body": walk_counts[0] / n,
+            "walk_ms": walk_ms, "group_walk_ms": (tg.value / cg.value) if cg.value else None,
+            "interactions_per_body": walk_counts[0] / n,
             "node_visits_per_group": walk_counts[1] / max(1, n // 64),
             # node records read by the group walks: one 64-byte record per visit
             "node_GB_per_s": 64.0 * walk_counts[1] / (walk_ms * 1e-3) / 1e9,
@@ -311,6 +314,24 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
         base = [r for r in out["runs"] if r["variant"] == "plain theta=0.5"]
         if base:
             out["best_at_rms_1e-3"]["speedup_vs_plain_theta_0.5"] = best["steps_per_s"] / base[0]["steps_per_s"]
+    # roofline of the dominant kernel of the fastest variant within RMS 1e-3: the group
+    # walks of the two-level walk, their live HIP-event duration against the VALU
+    # instructions and HBM bytes per launch that the counter runs in profiles/ measured
+    # for this variant and state
+    rl = bh_roofline_from_profile(n)
+    if rl is not None:
+        r = [x for x in out["runs"] if x["variant"] == rl["variant"] and x.get("group_walk_ms")]
+        if r:
+            ms = r[0]["group_walk_ms"]
+            rate = rl["valu"] / 1024 / (ms * 1e-3) / (CLOCK_GHZ * 1e9)
+            out["roofline"] = {
+                "kernel": "k_bh_group32<true, true, true> (group walks of the two-level walk)",
+                "variant": rl["variant"], "bound": "valu",
+                "achieved": rate, "peak": 0.5, "unit": "VALU wave instructions per SIMD per clock",
+                "frac": rate / 0.5, "traffic": rl["traffic"],
+                "hbm_GB_per_s": rl["traffic"] / (ms * 1e-3) / 1e9,
+                "hbm_frac": rl["traffic"] / (ms * 1e-3) / 8e12, "avg_kernel_ms": ms,
+                "source": rl["source"]}
     # equal-error comparison: for each plain monopole run, the fastest run of another
     # kind whose RMS and 99th-percentile errors are both no larger on every seed
     # (the RMS over the samples alone can swing with one body near a force
@@ -339,6 +360,10 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
                                 if same else None),
             "rms_by_seed": {k: v["rms"] for k, v in lr_["force_rel_err_by_seed"].items()},
         }
+        if same:
+            # what the network itself does: the same walk with and without it
+            out["learned_gate"]["network_lowers_error"] = bool(
+                lr_["rms_max_over_seeds"] < same[0]["rms_max_over_seeds"])
     eq = []
     for p in out["runs"]:
         if not plain(p):
@@ -516,6 +541,32 @@ def traffic_from_profile(n: int):
     for line in path.read_text().splitlines():
         if line.strip().startswith(key):
             return float(line.split(":", 1)[1].strip())
+    return None
+
+
+CLOCK_GHZ = 2.4
+BH_COUNTERS = "bh_two_level_walk_counters_plummer_{n}.md"
+BH_COUNTER_VARIANT = {"theta=0.62 near=3": "quadrupole theta=0.62, local far field, near 3"}
+
+
+def bh_roofline_from_profile(n: int):
+    """VALU wave instructions and HBM bytes per launch of the two-level group walk
+    at this N, from the counter summary in profiles/ (lines
+    'valu_wave_instr_per_launch <key>: <count>' and 'traffic_bytes_per_launch
+    <key>: <bytes>'), with the bench variant they were measured on; or None."""
+    path = ROOT / "profiles" / BH_COUNTERS.format(n=n)
+    if not path.exists():
+        return None
+    got = {}
+    for line in path.read_text().splitlines():
+        for tag, field in (("valu_wave_instr_per_launch ", "valu"), ("traffic_bytes_per_launch ", "traffic")):
+            if line.startswith(tag) and ":" in line:
+                key, val = line[len(tag):].split(":", 1)
+                got.setdefault(key.strip(), {})[field] = float(val)
+    for key, vals in got.items():
+        if key in BH_COUNTER_VARIANT and "valu" in vals and "traffic" in vals:
+            return {"variant": BH_COUNTER_VARIANT[key], "valu": vals["valu"], "traffic": vals["traffic"],
+                    "source": f"profiles/{path.name}"}
     return None
 
 

@@ -353,8 +353,10 @@ nbh_status nbh_bh_fcompute_group_mlp(nbh_ctx* ctx, const nbh_tree* tree, const n
    When enabled, the main kernel of each entry point is bracketed by HIP events
    on the context's stream.  nbh_timing_read synchronises the stream and returns
    the summed device time (ms) and launch count of one kernel class since the
-   last reset.  Classes: "direct_fp64", "direct_fp32", "tree_build",
-   "tree_refit", "bh_traverse", "fmaddn", "stats".  No counterpart in the
+   last reset.  Classes: "direct_fp64", "direct_fp32", "direct_fp64_block",
+   "tree_build", "tree_refit", "bh_traverse" (a whole walk), "bh_group_walk"
+   (the group walks of the two-level walk alone), "ah_full", "ah_sparse",
+   "fmaddn", "stats".  No counterpart in the
    reference (its timing is host wall clock, test/bench/bench.h:105-115). */
 nbh_status nbh_timing_enable(nbh_ctx* ctx, int enable);
 nbh_status nbh_timing_reset(nbh_ctx* ctx);

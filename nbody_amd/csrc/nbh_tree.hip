@@ -2657,9 +2657,13 @@ static nbh_status two_level(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const
 		                   t->mass, t->bmin, t->bmax, t->quad, n, two, nbh::mlp_frags(net), nbh::mlp_params(net));
 		NBH_LAUNCHED();
 	}
-	hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec,
-	                   t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two);
-	NBH_LAUNCHED();
+	{
+		// the group walks alone, inside bh_traverse
+		nbh::scoped_timer group_timer(ctx, "bh_group_walk");
+		hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec,
+		                   t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two);
+		NBH_LAUNCHED();
+	}
 	return NBH_OK;
 }
 
