@@ -1266,7 +1266,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	const float tz = static_cast<float>((me.z - ccz) * kSweepLen);
 	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	unsigned long long vis = 0, taken = 0, spill = 0, nfar = 0, nnear = 0;
+	unsigned vis = 0, taken = 0, spill = 0, nfar = 0, nnear = 0;
 	constexpr int kAcc = kLocal ? 34 : 9;
 	float acc[kAcc];
 	for(int k = 0; k < kAcc; ++k) { acc[k] = 0.0f; }
@@ -1459,11 +1459,11 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	}
 	if(lane == 0)
 	{
-		atomicAdd(&counters[0], taken * kGroup);
-		atomicAdd(&counters[1], vis);
-		if(spill) { atomicAdd(&counters[2], spill); }
-		if(nfar) { atomicAdd(&counters[3], nfar); }
-		if(nnear) { atomicAdd(&counters[4], nnear); }
+		atomicAdd(&counters[0], static_cast<unsigned long long>(taken) * kGroup);
+		atomicAdd(&counters[1], static_cast<unsigned long long>(vis));
+		if(spill) { atomicAdd(&counters[2], static_cast<unsigned long long>(spill)); }
+		if(nfar) { atomicAdd(&counters[3], static_cast<unsigned long long>(nfar)); }
+		if(nnear) { atomicAdd(&counters[4], static_cast<unsigned long long>(nnear)); }
 	}
 }
 
