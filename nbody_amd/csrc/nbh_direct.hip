@@ -85,10 +85,10 @@ __device__ inline void pair64(double x1, double y1, double z1, double gm1, const
 // runs in index order.  The simple law skips the self pair in the reference;
 // here, on the fast path, it is summed: dx = dy = dz = +0 there, so its term is
 // -0 and adding -0 leaves every value (+0 included) unchanged.  The block law
-// sums it in the reference too (its term subtracts +0).
-constexpr int kF64Tgt = 2;
-
-template <int kLaw>
+// sums it in the reference too (its term subtracts +0).  Two targets per
+// work-item when that still leaves two waves per SIMD (n >= 2^18 on 256 CUs),
+// else one, so that smaller n still spreads over every SIMD.
+template <int kLaw, int kF64Tgt>
 __global__ void __launch_bounds__(1024)
 k_direct_fp64(const double* __restrict__ y, const double* __restrict__ mass, double* __restrict__ f, size_t n)
 {
@@ -645,17 +645,33 @@ extern "C" nbh_status nbh_fcompute_direct(nbh_ctx* ctx, const double* y, const d
 		int bs = block_size == 0 ? 256 : block_size;
 		NBH_CHECK(bs >= 64 && bs <= 1024 && bs % 64 == 0,
 		          "block_size %d: must be a multiple of 64 in [64, 1024]", block_size);
-		unsigned grid = static_cast<unsigned>((n + kF64Tgt * bs - 1) / (kF64Tgt * bs));
+		// two targets per work-item when the grid still holds two waves per SIMD
+		const size_t simds = static_cast<size_t>(ctx->cu_count > 0 ? ctx->cu_count : 256) * 4;
+		const int tgt = n >= 2 * 64 * 2 * simds ? 2 : 1;
+		unsigned grid = static_cast<unsigned>((n + tgt * bs - 1) / (tgt * bs));
 		nbh::scoped_timer timer(ctx, precision == NBH_FP64 ? "direct_fp64" : "direct_fp64_block");
+		const size_t lds = bs * sizeof(double4);
 		if(precision == NBH_FP64)
 		{
-			hipLaunchKernelGGL(k_direct_fp64<kLawSimple>, dim3(grid), dim3(bs), bs * sizeof(double4), ctx->stream, y,
-			                   mass, f, n);
+			if(tgt == 2)
+			{
+				hipLaunchKernelGGL((k_direct_fp64<kLawSimple, 2>), dim3(grid), dim3(bs), lds, ctx->stream, y, mass, f, n);
+			}
+			else
+			{
+				hipLaunchKernelGGL((k_direct_fp64<kLawSimple, 1>), dim3(grid), dim3(bs), lds, ctx->stream, y, mass, f, n);
+			}
 		}
 		else
 		{
-			hipLaunchKernelGGL(k_direct_fp64<kLawBlock>, dim3(grid), dim3(bs), bs * sizeof(double4), ctx->stream, y,
-			                   mass, f, n);
+			if(tgt == 2)
+			{
+				hipLaunchKernelGGL((k_direct_fp64<kLawBlock, 2>), dim3(grid), dim3(bs), lds, ctx->stream, y, mass, f, n);
+			}
+			else
+			{
+				hipLaunchKernelGGL((k_direct_fp64<kLawBlock, 1>), dim3(grid), dim3(bs), lds, ctx->stream, y, mass, f, n);
+			}
 		}
 		NBH_LAUNCHED();
 		return NBH_OK;
