@@ -172,8 +172,8 @@ nbh_status nbh_ah_lists(nbh_ctx* ctx, const nbh_ah* ah, size_t* total, uint64_t*
                         double* univ);
 
 /* ---- Barnes-Hut heap kd-tree --------------------------------------------
-   n must be a power of two (the heap layout of nbody/nbody_space_heap.cpp:23
-   puts the leaves at [n, 2n)).  The tree is built on the GPU with the same
+   nbh_tree_build needs n to be a power of two (the heap layout of
+   nbody/nbody_space_heap.cpp:23 puts the leaves at [n, 2n)).  The tree is built on the GPU with the same
    median splits as nbody_space_heap::build (nbody_space_heap.cpp:13-37, 136-191). */
 nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** tree);
 nbh_status nbh_tree_destroy(nbh_tree* tree);
@@ -181,8 +181,29 @@ nbh_status nbh_tree_destroy(nbh_tree* tree);
 nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* tree, const double* y, const double* mass,
                           double ratio);
 /* Refit with the existing body order (nbody_space_heap::rebuild, :39-62;
-   kupdate_leaf_bh / kupdate_node_bh, nbody_engine_cuda_impl.cu:477-641). */
+   kupdate_leaf_bh / kupdate_node_bh, nbody_engine_cuda_impl.cu:477-641); a
+   Morton tree keeps its structure and recomputes its moments. */
 nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* tree, const double* y, double ratio);
+/* Morton radix tree, no counterpart in the reference: the bodies sorted by
+   63-bit Morton codes over their bounding cube (21 bits per axis, ties by body
+   number), internal nodes by the binary radix tree construction of Karras
+   (2012), moments, boxes and opening radii as nbh_tree_build computes them.
+   Ids: root 1, internal nodes 1..n-1, the body at sorted position j is leaf
+   n + j.  Its cells are compact where the heap's count-balanced nodes of a
+   centrally concentrated state span from the core to the outskirts, so a walk
+   takes far fewer of them at the same ratio.  Only the group walks
+   (nbh_bh_fcompute_group, _quad, _local, _local2) run on it; their groups and
+   super-groups are runs of 64 and 1024 bodies in Morton order.  A walk step
+   whose children would not fit on the walk stack takes the nodes as monopoles
+   (counted in nbh_bh_walk_counters out[2]).  Any n >= 2 builds; the walks need
+   n to be a multiple of 64, and the two-level walk walks in one level unless n
+   is a multiple of 1024. */
+nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* tree, const double* y, const double* mass,
+                                double ratio);
+/* Children of internal ids (child[2k], child[2k + 1] for id k = 1..n-1; entry 0
+   unused) and parents (parent[id] for ids 2..2n-1) of the last Morton tree;
+   either pointer may be NULL. */
+nbh_status nbh_tree_get_children(nbh_ctx* ctx, const nbh_tree* tree, int32_t* child, int32_t* parent);
 /* Traversal: f[6n] = [v ; F/m] for every body, targets taken in heap leaf
    order.  FP64 reproduces nbody_space_heap::traverse (layout HEAP) and
    nbody_space_heap_stackless::traverse (layout HEAP_STACKLESS) with the

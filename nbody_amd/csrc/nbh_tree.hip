@@ -44,6 +44,8 @@ constexpr int kScanOneThreads = 1024;
 constexpr int kScanOneItems = 16;                    // scans of up to 16384 counts in one work-group
 constexpr int kLocalSeg = 4096;                      // segments split in LDS, one work-group each
 constexpr int kLocalThreads = 1024;
+constexpr int kBoxParts = 256;                       // partial boxes of the Morton tree's bounding box
+constexpr int kRadixLevels = 128;                    // prefix lengths of index-extended 63-bit keys: 0..127
 
 }  // namespace
 
@@ -86,6 +88,23 @@ struct nbh_tree {
 	// the super-groups' far cells for the learned correction (allocated on first use)
 	int*      far_list = nullptr;
 	int*      far_count = nullptr;
+	// Morton radix tree (nbh_tree_build_radix; allocated on its first build)
+	int2*     child = nullptr;     // [n] children of internal ids 1..n-1
+	int*      parent = nullptr;    // [2n] parent id of every non-root node
+	int*      level = nullptr;     // [n] level of internal ids: the common prefix length of their keys
+	int*      level_list = nullptr;   // [n] internal ids level by level
+	int*      level_info = nullptr;   // [3 * kRadixLevels + 2] level counts, offsets, cursors; groups, super-groups
+	int2*     gmark = nullptr;     // [n] group marks at their first sorted leaf (build scratch)
+	int2*     smark = nullptr;     // [n] super-group marks
+	uint32_t* flags = nullptr;     // [2n] group and super-group flags, scanned in place
+	int4*     grp = nullptr;       // [n] walk groups (node, first leaf, leaves, super-group)
+	int2*     sup = nullptr;       // [n / 512 + 1] super-groups (node, first leaf)
+	double*   box = nullptr;       // [6 * (kBoxParts + 1)] partial boxes, then the bounding box
+	int       ngroups = 0;         // walk groups and super-groups of the last Morton build
+	int       nsupers = 0;
+	int       level_count[kRadixLevels] = {};   // host copy of the level counts (refit)
+	int       two_slots = 0;       // super-groups the two-level buffers hold
+	bool      radix = false;       // the last build made the Morton tree
 	double    ratio = 0.0;         // opening ratio of the last build / refit
 	bool      built = false;
 };
@@ -555,10 +574,11 @@ k_leaves(const uint32_t* __restrict__ order, const double* __restrict__ y, const
 __device__ inline double min_ref(double a, double b) { return (b < a) ? b : a; }   // std::min
 __device__ inline double max_ref(double a, double b) { return (a < b) ? b : a; }   // std::max
 
-__device__ inline void update_node(size_t idx, double4* __restrict__ xyzr, double* __restrict__ nmass,
-                                   double* __restrict__ bmin, double* __restrict__ bmax, double ratio_sqr)
+// Node idx from its children l and r (2 idx and 2 idx + 1 in the heap).
+__device__ inline void update_node(size_t idx, size_t l, size_t r, double4* __restrict__ xyzr,
+                                   double* __restrict__ nmass, double* __restrict__ bmin, double* __restrict__ bmax,
+                                   double ratio_sqr)
 {
-	size_t l = 2 * idx, r = 2 * idx + 1;
 	double ml = nmass[l], mr = nmass[r];
 	double m = ml + mr;
 	double4 cl = xyzr[l], cr = xyzr[r];
@@ -585,14 +605,15 @@ __device__ inline void update_node(size_t idx, double4* __restrict__ xyzr, doubl
 // Second central moments Q = sum m (x - cm)(x - cm)^T of node idx from its
 // children (parallel-axis theorem); components xx yy zz xy xz yz.  Not part of
 // the reference tree: input of the learned correction.
-__device__ inline void update_quad(size_t idx, const double4* __restrict__ xyzr, const double* __restrict__ nmass,
-                                   double* __restrict__ quad)
+__device__ inline void update_quad(size_t idx, size_t l, size_t r, const double4* __restrict__ xyzr,
+                                   const double* __restrict__ nmass, double* __restrict__ quad)
 {
 	const double4 c = xyzr[idx];
 	double q[6];
 	for(int k = 0; k < 6; ++k) { q[k] = 0.0; }
-	for(size_t ch = 2 * idx; ch <= 2 * idx + 1; ++ch)
+	for(int side = 0; side < 2; ++side)
 	{
+		const size_t ch = side == 0 ? l : r;
 		const double4 cc = xyzr[ch];
 		const double m = nmass[ch];
 		const double dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
@@ -614,8 +635,8 @@ k_level(size_t lo, size_t hi, double4* __restrict__ xyzr, double* __restrict__ n
 	size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
 	for(size_t idx = lo + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < hi; idx += stride)
 	{
-		update_node(idx, xyzr, nmass, bmin, bmax, ratio_sqr);
-		update_quad(idx, xyzr, nmass, quad);
+		update_node(idx, 2 * idx, 2 * idx + 1, xyzr, nmass, bmin, bmax, ratio_sqr);
+		update_quad(idx, 2 * idx, 2 * idx + 1, xyzr, nmass, quad);
 	}
 }
 
@@ -628,8 +649,359 @@ k_levels_top(size_t top, double4* __restrict__ xyzr, double* __restrict__ nmass,
 	{
 		for(size_t idx = lo + threadIdx.x; idx < 2 * lo; idx += blockDim.x)
 		{
-			update_node(idx, xyzr, nmass, bmin, bmax, ratio_sqr);
-			update_quad(idx, xyzr, nmass, quad);
+			update_node(idx, 2 * idx, 2 * idx + 1, xyzr, nmass, bmin, bmax, ratio_sqr);
+			update_quad(idx, 2 * idx, 2 * idx + 1, xyzr, nmass, quad);
+		}
+		__syncthreads();
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Morton radix tree (the fast path of the FP32 group walks; no counterpart in
+// the reference, whose heap kd-tree stays the tree of the parity walks)
+// ---------------------------------------------------------------------------
+// Bodies sorted by 63-bit Morton codes (21 bits per axis over the bounding
+// cube), ties broken by body index; internal nodes by the binary radix tree
+// construction of Karras (2012, "Maximizing parallelism in the construction of
+// BVHs, octrees, and k-d trees"): internal node i covers a key range with i at
+// one end and splits it at the highest bit where its keys differ.  Node ids
+// follow the heap's conventions where the walks rely on them: the root is id 1
+// (internal node i is id i + 1), the body at sorted position j is leaf id n + j,
+// and ids >= n are leaves.  Cells are bounded by the boxes of their bodies,
+// with the heap's opening radius, so a cell of the Morton tree is compact where
+// a count-balanced heap node of a centrally concentrated state spans from the
+// core to the outskirts.
+
+// Spread the low 21 bits of v to every third bit.
+__device__ inline uint64_t morton_spread(uint64_t v)
+{
+	v &= 0x1fffffull;
+	v = (v | (v << 32)) & 0x1f00000000ffffull;
+	v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+	v = (v | (v << 8)) & 0x100f00f00f00f00full;
+	v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+	v = (v | (v << 2)) & 0x1249249249249249ull;
+	return v;
+}
+
+// Cell coordinate of x on a grid of 2^21 cells over [lo, lo + 2^21 / scale).
+__device__ inline uint64_t morton_cell(double x, double lo, double scale)
+{
+	double c = floor((x - lo) * scale);
+	c = c < 0.0 ? 0.0 : c;
+	c = c > 2097151.0 ? 2097151.0 : c;
+	return static_cast<uint64_t>(c);
+}
+
+// box[6] = bounding box lo.xyz hi.xyz of the bodies: scale = 2^21 / longest side
+__global__ void __launch_bounds__(256)
+k_morton_codes(const double* __restrict__ y, size_t n, const double* __restrict__ box, uint64_t* __restrict__ keys,
+               uint32_t* __restrict__ vals)
+{
+	const double side = fmax(fmax(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
+	const double scale = side > 0.0 ? 2097152.0 / side : 0.0;
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		const uint64_t cx = morton_cell(y[i], box[0], scale);
+		const uint64_t cy = morton_cell(y[n + i], box[1], scale);
+		const uint64_t cz = morton_cell(y[2 * n + i], box[2], scale);
+		keys[i] = (morton_spread(cx) << 2) | (morton_spread(cy) << 1) | morton_spread(cz);
+		vals[i] = static_cast<uint32_t>(i);
+	}
+}
+
+// Bounding box partials of the positions, then the box (lo.xyz hi.xyz).
+__global__ void __launch_bounds__(256)
+k_box_partial(const double* __restrict__ y, size_t n, double* __restrict__ part)
+{
+	__shared__ double sh[6][256];
+	double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+	{
+		for(int d = 0; d < 3; ++d)
+		{
+			const double v = y[static_cast<size_t>(d) * n + i];
+			lo[d] = fmin(lo[d], v);
+			hi[d] = fmax(hi[d], v);
+		}
+	}
+	for(int d = 0; d < 3; ++d) { sh[d][threadIdx.x] = lo[d]; sh[3 + d][threadIdx.x] = hi[d]; }
+	__syncthreads();
+	for(int w = 128; w > 0; w >>= 1)
+	{
+		if(static_cast<int>(threadIdx.x) < w)
+		{
+			for(int d = 0; d < 3; ++d)
+			{
+				sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + w]);
+				sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + w]);
+			}
+		}
+		__syncthreads();
+	}
+	if(threadIdx.x < 6) { part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0]; }
+}
+
+__global__ void __launch_bounds__(64)
+k_box_final(const double* __restrict__ part, int count, double* __restrict__ box)
+{
+	if(threadIdx.x < 6)
+	{
+		const int d = threadIdx.x;
+		double v = part[d];
+		for(int k = 1; k < count; ++k) { v = d < 3 ? fmin(v, part[6 * k + d]) : fmax(v, part[6 * k + d]); }
+		box[d] = v;
+	}
+}
+
+// Length of the common prefix of sorted keys i and j (with the index as a
+// tie-breaking suffix), -1 outside [0, n).
+__device__ inline int radix_delta(const uint64_t* __restrict__ k, long long n, long long i, long long j)
+{
+	if(j < 0 || j >= n) { return -1; }
+	const uint64_t a = k[i], b = k[j];
+	if(a == b)
+	{
+		return 64 + __clzll(static_cast<unsigned long long>(i ^ j));
+	}
+	return __clzll(a ^ b);
+}
+
+// Walk groups and super-groups of the Morton tree: the maximal subtrees of at
+// most kRadixGroup leaves, and those of at most kRadixSuper leaves that hold at
+// least half as many (smaller ones leave their groups to walk in one level).
+// Their leaves are consecutive in Morton order and their boxes are node boxes,
+// so they never straddle the core of the state the way a fixed run of the
+// Morton order can.
+constexpr int kRadixGroup = 64;
+constexpr int kRadixSuper = 1024;
+
+// Internal node i of [0, n - 1): its key range, split and children (ids), the
+// parent links of the children, its level (the common prefix length of its
+// keys, which grows strictly from a node to its children, so nodes of one level
+// are independent), the count of each level, and the marks of walk groups and
+// super-groups at the first sorted position of their leaves: (node id, leaves).
+__global__ void __launch_bounds__(256)
+k_radix_nodes(const uint64_t* __restrict__ k, long long n, int2* __restrict__ child, int* __restrict__ parent,
+              int* __restrict__ level, int* __restrict__ level_count, int2* __restrict__ gmark,
+              int2* __restrict__ smark)
+{
+	__shared__ int hist[kRadixLevels];
+	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x) { hist[b] = 0; }
+	__syncthreads();
+	const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+	for(long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n - 1; i += stride)
+	{
+		const int dir = radix_delta(k, n, i, i + 1) - radix_delta(k, n, i, i - 1) > 0 ? 1 : -1;
+		const int dmin = radix_delta(k, n, i, i - dir);
+		long long lmax = 2;
+		while(radix_delta(k, n, i, i + lmax * dir) > dmin) { lmax <<= 1; }
+		long long l = 0;
+		for(long long t = lmax >> 1; t >= 1; t >>= 1)
+		{
+			if(radix_delta(k, n, i, i + (l + t) * dir) > dmin) { l += t; }
+		}
+		const long long j = i + l * dir;
+		const int dnode = radix_delta(k, n, i, j);
+		long long s = 0;
+		for(long long t = (l + 1) >> 1;; t = (t + 1) >> 1)
+		{
+			if(s + t < l + 1 && radix_delta(k, n, i, i + (s + t) * dir) > dnode) { s += t; }
+			if(t <= 1) { break; }
+		}
+		const long long gamma = i + s * dir + (dir < 0 ? -1 : 0);
+		const long long lo = i < j ? i : j, hi = i < j ? j : i;
+		const int left = lo == gamma ? static_cast<int>(n + gamma) : static_cast<int>(gamma + 1);
+		const int right = hi == gamma + 1 ? static_cast<int>(n + gamma + 1) : static_cast<int>(gamma + 2);
+		const int id = static_cast<int>(i + 1);
+		child[id] = make_int2(left, right);
+		parent[left] = id;
+		parent[right] = id;
+		level[id] = dnode;
+		atomicAdd(&hist[dnode], 1);
+		const long long cnt = hi - lo + 1;
+		const long long cl = gamma - lo + 1, cr = hi - gamma;
+		if(id == kRoot && cnt <= kRadixGroup) { gmark[lo] = make_int2(id, static_cast<int>(cnt)); }
+		if(id == kRoot && cnt <= kRadixSuper && 2 * cnt >= kRadixSuper) { smark[lo] = make_int2(id, static_cast<int>(cnt)); }
+		if(cnt > kRadixGroup)
+		{
+			if(cl <= kRadixGroup) { gmark[lo] = make_int2(left, static_cast<int>(cl)); }
+			if(cr <= kRadixGroup) { gmark[gamma + 1] = make_int2(right, static_cast<int>(cr)); }
+		}
+		if(cnt > kRadixSuper)
+		{
+			if(cl <= kRadixSuper && 2 * cl >= kRadixSuper) { smark[lo] = make_int2(left, static_cast<int>(cl)); }
+			if(cr <= kRadixSuper && 2 * cr >= kRadixSuper) { smark[gamma + 1] = make_int2(right, static_cast<int>(cr)); }
+		}
+	}
+	__syncthreads();
+	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x)
+	{
+		if(hist[b] != 0) { atomicAdd(&level_count[b], hist[b]); }
+	}
+}
+
+__global__ void __launch_bounds__(256)
+k_fill_int(int* __restrict__ a, size_t count, int v)
+{
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) { a[i] = v; }
+}
+
+// 1 where a mark starts a group / super-group, for the scans that number them.
+__global__ void __launch_bounds__(256)
+k_radix_flags(const int2* __restrict__ gmark, const int2* __restrict__ smark, size_t n, uint32_t* __restrict__ gflag,
+              uint32_t* __restrict__ sflag)
+{
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
+	{
+		gflag[p] = gmark[p].x != 0 ? 1u : 0u;
+		sflag[p] = smark[p].x != 0 ? 1u : 0u;
+	}
+}
+
+// Groups grp[g] = (node id, first sorted leaf, leaves, super-group or -1) and
+// super-groups sup[s] = (node id, first sorted leaf), both in Morton order
+// (gidx / sidx: exclusive scans of the flags); info[0] = groups, info[1] =
+// super-groups.
+__global__ void __launch_bounds__(256)
+k_radix_compact(const int2* __restrict__ gmark, const int2* __restrict__ smark, const uint32_t* __restrict__ gidx,
+                const uint32_t* __restrict__ sidx, size_t n, int4* __restrict__ grp, int2* __restrict__ sup,
+                int* __restrict__ info)
+{
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
+	{
+		const int2 g = gmark[p];
+		if(g.x != 0) { grp[gidx[p]] = make_int4(g.x, static_cast<int>(p), g.y, -1); }
+		const int2 sm = smark[p];
+		if(sm.x != 0) { sup[sidx[p]] = make_int2(sm.x, static_cast<int>(p)); }
+		if(p == n - 1)
+		{
+			info[0] = static_cast<int>(gidx[p] + (g.x != 0 ? 1u : 0u));
+			info[1] = static_cast<int>(sidx[p] + (sm.x != 0 ? 1u : 0u));
+		}
+	}
+}
+
+// The super-group of each group: the last super-group starting at or before the
+// group's first leaf, if its node's leaves reach the group (a group lies inside
+// at most one super-group: both are subtrees).
+__global__ void __launch_bounds__(256)
+k_radix_group_super(int4* __restrict__ grp, const int2* __restrict__ sup, const int* __restrict__ info,
+                    const int2* __restrict__ child, size_t n)
+{
+	const int ng = info[0], ns = info[1];
+	for(int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x)
+	{
+		const int first = grp[g].y;
+		int lo = 0, hi = ns - 1, s = -1;
+		while(lo <= hi)
+		{
+			const int mid = (lo + hi) >> 1;
+			if(sup[mid].y <= first) { s = mid; lo = mid + 1; }
+			else { hi = mid - 1; }
+		}
+		if(s >= 0)
+		{
+			// leaves of the super-group's node: its first leaf and its size, found
+			// by walking its rightmost branch down to a leaf
+			int node = sup[s].x;
+			while(node < static_cast<int>(n)) { node = child[node].y; }
+			const int last = node - static_cast<int>(n);
+			if(first > last) { s = -1; }
+		}
+		grp[g].w = s;
+	}
+}
+
+// Offsets of the levels (exclusive scan of kRadixLevels counts, one work-group).
+__global__ void __launch_bounds__(kRadixLevels)
+k_radix_level_offsets(const int* __restrict__ count, int* __restrict__ offset, int* __restrict__ cursor)
+{
+	__shared__ int sh[kRadixLevels];
+	const int t = threadIdx.x;
+	sh[t] = count[t];
+	__syncthreads();
+	if(t == 0)
+	{
+		int run = 0;
+		for(int b = 0; b < kRadixLevels; ++b)
+		{
+			const int c = sh[b];
+			sh[b] = run;
+			run += c;
+		}
+	}
+	__syncthreads();
+	offset[t] = sh[t];
+	cursor[t] = 0;
+}
+
+// Internal ids listed level by level, one id per work-item: ranks within the
+// work-group from LDS counters, one global reservation per level present.
+__global__ void __launch_bounds__(256)
+k_radix_level_lists(const int* __restrict__ level, size_t n, const int* __restrict__ offset, int* __restrict__ cursor,
+                    int* __restrict__ list)
+{
+	__shared__ int cnt[kRadixLevels];
+	__shared__ int base[kRadixLevels];
+	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x) { cnt[b] = 0; }
+	__syncthreads();
+	const size_t id = 1 + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+	int lv = -1, rank = 0;
+	if(id < n)
+	{
+		lv = level[id];
+		rank = atomicAdd(&cnt[lv], 1);
+	}
+	__syncthreads();
+	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x)
+	{
+		base[b] = cnt[b] != 0 ? offset[b] + atomicAdd(&cursor[b], cnt[b]) : 0;
+	}
+	__syncthreads();
+	if(lv >= 0) { list[base[lv] + rank] = static_cast<int>(id); }
+}
+
+// Moments of the internal ids list[0, count) of one level, from their children
+// (the heap's update_node and update_quad); the levels run from the longest
+// prefix to the root, one launch each.
+__global__ void __launch_bounds__(256)
+k_radix_level(const int* __restrict__ list, int count, const int2* __restrict__ child, double4* __restrict__ xyzr,
+              double* __restrict__ nmass, double* __restrict__ bmin, double* __restrict__ bmax,
+              double* __restrict__ quad, double ratio_sqr)
+{
+	const int stride = gridDim.x * blockDim.x;
+	for(int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride)
+	{
+		const int id = list[k];
+		const int2 c = child[id];
+		update_node(id, c.x, c.y, xyzr, nmass, bmin, bmax, ratio_sqr);
+		update_quad(id, c.x, c.y, xyzr, nmass, quad);
+	}
+}
+
+// Levels lv_hi down to lv_lo, each too small to fill the device, in one
+// work-group (info: the level counts, then their offsets in list).
+__global__ void __launch_bounds__(1024)
+k_radix_levels_small(const int* __restrict__ list, const int* __restrict__ info, int lv_hi, int lv_lo,
+                     const int2* __restrict__ child, double4* __restrict__ xyzr, double* __restrict__ nmass,
+                     double* __restrict__ bmin, double* __restrict__ bmax, double* __restrict__ quad,
+                     double ratio_sqr)
+{
+	for(int lv = lv_hi; lv >= lv_lo; --lv)
+	{
+		const int count = info[lv], off = info[kRadixLevels + lv];
+		for(int k = threadIdx.x; k < count; k += blockDim.x)
+		{
+			const int id = list[off + k];
+			const int2 c = child[id];
+			update_node(id, c.x, c.y, xyzr, nmass, bmin, bmax, ratio_sqr);
+			update_quad(id, c.x, c.y, xyzr, nmass, quad);
 		}
 		__syncthreads();
 	}
@@ -882,8 +1254,10 @@ __device__ inline double half_diagonal(double x0, double y0, double z0, double x
 // one node per lane (lane 2i takes the left node of entry i, lane 2i + 1 the
 // right one); every node the step opens pushes one entry, its two children.
 // Popped in this order, the stack never holds more than (levels + 2) * 32
-// entries.
-__device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid)
+// entries.  In the Morton tree (kRadix) an entry is the id of the opened node,
+// whose children come from `child`, and 0 stands for the root alone.
+template <bool kRadix = false>
+__device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid, const int2* __restrict__ child = nullptr)
 {
 	const int cnt = sp < kGroup / 2 ? sp : kGroup / 2;
 	const int i = lane >> 1;
@@ -892,12 +1266,34 @@ __device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid)
 	if(i < cnt)
 	{
 		const int e = stack[sp - cnt + i];
-		node = e == kRoot ? kRoot : e + (lane & 1);
-		valid = e != kRoot || (lane & 1) == 0;
+		if constexpr(kRadix)
+		{
+			if(e == 0)
+			{
+				valid = (lane & 1) == 0;
+			}
+			else
+			{
+				const int2 c = child[e];
+				node = (lane & 1) ? c.y : c.x;
+				valid = true;
+			}
+		}
+		else
+		{
+			node = e == kRoot ? kRoot : e + (lane & 1);
+			valid = e != kRoot || (lane & 1) == 0;
+		}
 	}
 	sp -= cnt;
 	return node;
 }
+
+// Stack entry of an opened node, and the entry the walk starts from.
+template <bool kRadix>
+__device__ inline int walk_entry(int node) { return kRadix ? node : 2 * node; }
+template <bool kRadix>
+__device__ inline int walk_start() { return kRadix ? 0 : kRoot; }
 
 // The group walks sweep in scaled units: offsets x' = 1e4 x, masses m' = 1e8 m
 // and second moments Q' = 1e16 Q (mass x length^2), each formed in FP64 and
@@ -924,11 +1320,12 @@ __device__ inline float4 sweep_entry(const double4& rec, double m, double ccx, d
 
 // Pop the next chunk of the walk stack (walk_pop) and request its node records;
 // returns the number of stack entries taken.
+template <bool kRadix = false>
 __device__ inline int walk_fetch(const int* stack, int& sp, int lane, const nbh_walk_rec* __restrict__ wr, int& node,
-                                 bool& valid, double4& rec, double& m)
+                                 bool& valid, double4& rec, double& m, const int2* __restrict__ child = nullptr)
 {
 	const int cnt = sp < kGroup / 2 ? sp : kGroup / 2;
-	node = walk_pop(stack, sp, lane, valid);
+	node = walk_pop<kRadix>(stack, sp, lane, valid, child);
 	rec = make_double4(0.0, 0.0, 0.0, 0.0);
 	m = 0.0;
 	if(valid)
@@ -1208,6 +1605,18 @@ __device__ inline void far_batch(const int* fn, const nbh_walk_rec* __restrict__
 constexpr int kSuperShift = 4;
 constexpr float kSuperNearMin = 5.0f;
 
+// The tree a group walk runs on.  Heap (child == nullptr): children of node k
+// are 2k and 2k + 1, and the walk groups and super-groups are the heap nodes
+// of 64 and 1024 bodies.  Morton tree: the children of internal id k are
+// child[k]; group g is grp[g] = (node id, first sorted leaf, leaves (<= 64),
+// super-group or -1) and super-group s is sup[s] = (node id, first leaf).  In
+// both, a group's or super-group's box is its node's box.
+struct WalkTree {
+	const int2* child = nullptr;
+	const int4* grp = nullptr;
+	const int2* sup = nullptr;
+};
+
 struct TwoLevel {
 	int*   lists = nullptr;   // [super][cap] listed nodes
 	int*   count = nullptr;   // [super] list length (> cap: overflowed)
@@ -1243,11 +1652,12 @@ __device__ inline void list_fetch(const int* __restrict__ list, int pos, int cou
 // kLocal (with kQuad): a far cell's monopole field is expanded too, to third
 // order (local_expansion), instead of being swept per body.  kTwo (with kLocal):
 // the group takes its super-group's list instead of walking (TwoLevel).
-template <bool kQuad, bool kLocal = false, bool kTwo = false>
+template <bool kQuad, bool kLocal = false, bool kTwo = false, bool kRadix = false>
 __global__ void __launch_bounds__(kGroup)
 k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
              const int32_t* __restrict__ body_n, const double* __restrict__ y, double* __restrict__ f, int n,
-             float near_k, unsigned long long* __restrict__ counters, TwoLevel two = TwoLevel())
+             float near_k, unsigned long long* __restrict__ counters, TwoLevel two = TwoLevel(),
+             WalkTree tree = WalkTree())
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ float4 buf[kWalkBuf];
@@ -1255,11 +1665,25 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	__shared__ int farn[kQuad ? kFarRing : 1];
 	const int lane = threadIdx.x;
 	const int gi = xcd_group(blockIdx.x, gridDim.x);    // group ordinal
-	const int g = n / kGroup + gi;                      // heap node of this group
-	const double gx0 = bmin[3 * g], gy0 = bmin[3 * g + 1], gz0 = bmin[3 * g + 2];
-	const double gx1 = bmax[3 * g], gy1 = bmax[3 * g + 1], gz1 = bmax[3 * g + 2];
+	// the group's node, first leaf, size and super-group
+	int gnode = n / kGroup + gi, gfirst = gi * kGroup, gcount = kGroup, gsuper = gi >> kSuperShift;
+	if constexpr(kRadix)
+	{
+		const int4 gr = tree.grp[gi];
+		gnode = gr.x;
+		gfirst = gr.y;
+		gcount = gr.z;
+		gsuper = gr.w;
+	}
+	const double* glo = bmin + 3 * static_cast<size_t>(gnode);
+	const double* ghi = bmax + 3 * static_cast<size_t>(gnode);
+	const double gx0 = glo[0], gy0 = glo[1], gz0 = glo[2];
+	const double gx1 = ghi[0], gy1 = ghi[1], gz1 = ghi[2];
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
-	const int tn = n + gi * kGroup + lane;             // this lane's leaf
+	// this lane's leaf; lanes beyond a smaller group's size repeat its first body
+	// and write nothing
+	const bool body_lane = lane < gcount;
+	const int tn = n + gfirst + (body_lane ? lane : 0);
 	const double4 me = wr[tn].c;
 	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
 	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
@@ -1277,8 +1701,8 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	bool use_list = false;
 	if constexpr(kTwo)
 	{
-		si = gi >> kSuperShift;
-		use_list = two.count[si] <= two.cap;
+		si = gsuper;
+		use_list = si >= 0 && two.count[si] <= two.cap;
 		lcnt = use_list ? two.count[si] : 0;
 	}
 	const int* list = kTwo ? two.lists + static_cast<size_t>(si) * two.cap : nullptr;
@@ -1286,7 +1710,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	int nbuf = 0;
 	int nq = 0;
 	int fhead = 0, ftail = 0;     // far-cell ring: entries [fhead, ftail) pending
-	if(lane == 0) { stack[0] = kRoot; }
+	if(lane == 0) { stack[0] = walk_start<kRadix>(); }
 	__syncthreads();
 	while(sp > 0 || lpos < lcnt)
 	{
@@ -1304,7 +1728,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 		}
 		else
 		{
-			walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
+			walk_fetch<kRadix>(stack, sp, lane, wr, node, valid, rec, m, tree.child);
 		}
 		if(nbuf > kWalkBuf - kGroup)
 		{
@@ -1382,7 +1806,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			if(accept) { buf[nbuf + __popcll(bacc & lt)] = ent; }
 			nbuf += __popcll(bacc);
 		}
-		if(open) { stack[sp + __popcll(bopen2 & lt)] = 2 * node; }
+		if(open) { stack[sp + __popcll(bopen2 & lt)] = walk_entry<kRadix>(node); }
 		sp += __popcll(bopen2);
 		vis += valid ? 1 : 0;
 		taken += accept ? 1 : 0;
@@ -1424,10 +1848,13 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			if(use_list)
 			{
 				// the super-group's expansion about its own box centre
-				const int sg = (n >> (6 + kSuperShift)) + si;
-				const double scx = 0.5 * (bmin[3 * sg] + bmax[3 * sg]);
-				const double scy = 0.5 * (bmin[3 * sg + 1] + bmax[3 * sg + 1]);
-				const double scz = 0.5 * (bmin[3 * sg + 2] + bmax[3 * sg + 2]);
+				const size_t sg = kRadix ? static_cast<size_t>(tree.sup[si].x)
+				                         : (static_cast<size_t>(n) >> (6 + kSuperShift)) + si;
+				const double* slo = bmin + 3 * sg;
+				const double* shi = bmax + 3 * sg;
+				const double scx = 0.5 * (slo[0] + shi[0]);
+				const double scy = 0.5 * (slo[1] + shi[1]);
+				const double scz = 0.5 * (slo[2] + shi[2]);
 				const float* c = two.coef + 34 * static_cast<size_t>(si);
 				float e[34];
 				for(int k = 0; k < 34; ++k) { e[k] = c[k]; }
@@ -1441,14 +1868,18 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			}
 		}
 	}
-	const int b = body_n[tn];
-	f[b] = y[3 * n + b];
-	f[n + b] = y[4 * n + b];
-	f[2 * n + b] = y[5 * n + b];
-	f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx) + static_cast<double>(kx);
-	f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy) + static_cast<double>(ky);
-	f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz) + static_cast<double>(kz);
-	// counters: [0] pair interactions (entries x 64 targets), [1] node visits, [2] spills
+	if(body_lane)
+	{
+		const int b = body_n[tn];
+		f[b] = y[3 * n + b];
+		f[n + b] = y[4 * n + b];
+		f[2 * n + b] = y[5 * n + b];
+		f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx) + static_cast<double>(kx);
+		f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy) + static_cast<double>(ky);
+		f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz) + static_cast<double>(kz);
+	}
+	// counters: [0] pair interactions (entries x the group's bodies), [1] node
+	// visits, [2] spills
 	for(int off = 32; off > 0; off >>= 1)
 	{
 		vis += __shfl_down(vis, off, 64);
@@ -1459,7 +1890,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	}
 	if(lane == 0)
 	{
-		atomicAdd(&counters[0], static_cast<unsigned long long>(taken) * kGroup);
+		atomicAdd(&counters[0], static_cast<unsigned long long>(taken) * gcount);
 		atomicAdd(&counters[1], static_cast<unsigned long long>(vis));
 		if(spill) { atomicAdd(&counters[2], static_cast<unsigned long long>(spill)); }
 		if(nfar) { atomicAdd(&counters[3], static_cast<unsigned long long>(nfar)); }
@@ -1470,18 +1901,22 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 // Super-group walk of the two-level local walk (TwoLevel): the group walk's
 // loop with the super box, far cells into the super expansion, every other
 // accepted node into the list.
+template <bool kRadix = false>
 __global__ void __launch_bounds__(kGroup)
 k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
            int n, float near_k, double inv_ratio, TwoLevel two, unsigned long long* __restrict__ counters,
-           int stack_limit)
+           int stack_limit, WalkTree tree = WalkTree())
 {
 	__shared__ int stack[kWalkStack];
 	__shared__ int farn[kFarRing];
 	const int lane = threadIdx.x;
 	const int si = xcd_group(blockIdx.x, gridDim.x);
-	const int sg = (n >> (6 + kSuperShift)) + si;        // heap node of the super-group
-	const double gx0 = bmin[3 * sg], gy0 = bmin[3 * sg + 1], gz0 = bmin[3 * sg + 2];
-	const double gx1 = bmax[3 * sg], gy1 = bmax[3 * sg + 1], gz1 = bmax[3 * sg + 2];
+	// node of the super-group
+	const size_t sg = kRadix ? static_cast<size_t>(tree.sup[si].x) : (static_cast<size_t>(n) >> (6 + kSuperShift)) + si;
+	const double* glo = bmin + 3 * sg;
+	const double* ghi = bmax + 3 * sg;
+	const double gx0 = glo[0], gy0 = glo[1], gz0 = glo[2];
+	const double gx1 = ghi[0], gy1 = ghi[1], gz1 = ghi[2];
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
 	const float near_d = static_cast<float>(static_cast<double>(near_k) * half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -1494,7 +1929,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 	bool overflow = false;
 	int sp = 1;
 	int fhead = 0, ftail = 0;
-	if(lane == 0) { stack[0] = kRoot; }
+	if(lane == 0) { stack[0] = walk_start<kRadix>(); }
 	__syncthreads();
 	while(sp > 0)
 	{
@@ -1502,7 +1937,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 		int node;
 		double4 rec;
 		double m;
-		walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
+		walk_fetch<kRadix>(stack, sp, lane, wr, node, valid, rec, m, tree.child);
 		if(ftail - fhead >= kGroup)
 		{
 			far_batch<true>(farn, wr, fhead, kGroup, lane, ccx, ccy, ccz, acc);
@@ -1558,7 +1993,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 			const int slot = lcount + __popcll(blist & lt);
 			if(slot < two.cap) { list[slot] = node; }
 		}
-		if(open) { stack[sp + __popcll(bopen & lt)] = 2 * node; }
+		if(open) { stack[sp + __popcll(bopen & lt)] = walk_entry<kRadix>(node); }
 		ftail += __popcll(bfar);
 		lcount += __popcll(blist);
 		sp += __popcll(bopen);
@@ -2286,7 +2721,7 @@ nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(out != nullptr, "null output pointer");
 	*out = nullptr;
-	NBH_CHECK(n >= 1 && (n & (n - 1)) == 0, "body count %zu is not a power of two", n);
+	NBH_CHECK(n >= 1, "empty tree");
 	NBH_CHECK(n <= (static_cast<size_t>(1) << 28), "body count %zu too large", n);
 	nbh_tree* t = new nbh_tree();
 	t->ctx = ctx;
@@ -2316,6 +2751,188 @@ nbh_status nbh_tree_create(nbh_ctx* ctx, size_t n, nbh_tree** out)
 	return NBH_OK;
 }
 
+// The Morton tree's buffers, all together or none.
+static nbh_status radix_alloc(nbh_tree* t)
+{
+	if(t->child != nullptr)
+	{
+		return NBH_OK;
+	}
+	const size_t n = t->n;
+	constexpr int kBufs = 11;
+	void* p[kBufs] = {};
+	const size_t sizes[kBufs] = { n * sizeof(int2), 2 * n * sizeof(int), n * sizeof(int), n * sizeof(int),
+	                              (3 * kRadixLevels + 2) * sizeof(int), n * sizeof(int2), n * sizeof(int2),
+	                              2 * n * sizeof(uint32_t), n * sizeof(int4),
+	                              (n / (kRadixSuper / 2) + 1) * sizeof(int2), 6 * (kBoxParts + 1) * sizeof(double) };
+	hipError_t e = hipSuccess;
+	for(int k = 0; k < kBufs && e == hipSuccess; ++k)
+	{
+		e = hipMalloc(&p[k], sizes[k]);
+	}
+	if(e != hipSuccess)
+	{
+		for(void* q : p)
+		{
+			if(q != nullptr) { (void)hipFree(q); }
+		}
+		return nbh::hip_fail(e, "Morton tree allocation", __FILE__, __LINE__);
+	}
+	t->child = static_cast<int2*>(p[0]);
+	t->parent = static_cast<int*>(p[1]);
+	t->level = static_cast<int*>(p[2]);
+	t->level_list = static_cast<int*>(p[3]);
+	t->level_info = static_cast<int*>(p[4]);
+	t->gmark = static_cast<int2*>(p[5]);
+	t->smark = static_cast<int2*>(p[6]);
+	t->flags = static_cast<uint32_t*>(p[7]);
+	t->grp = static_cast<int4*>(p[8]);
+	t->sup = static_cast<int2*>(p[9]);
+	t->box = static_cast<double*>(p[10]);
+	return NBH_OK;
+}
+
+// Levels with fewer internal nodes than this run together in one work-group.
+constexpr int kRadixSmallLevel = 4096;
+
+// Moments of the Morton tree (its structure and level lists already built),
+// level by level from the longest prefix up to the root, then the packed walk
+// records.  build: leaves from `order` (sorted body numbers), else from the body
+// numbers of the last build.
+static nbh_status radix_moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mass, const uint32_t* order,
+                                double ratio, bool build)
+{
+	const size_t n = t->n;
+	t->ratio = ratio;
+	const double ratio_sqr = ratio * ratio;
+	hipLaunchKernelGGL(k_leaves, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, order, y, mass, n,
+	                   t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, build ? 1 : 0);
+	NBH_LAUNCHED();
+	int offset[kRadixLevels];
+	int run = 0;
+	for(int lv = 0; lv < kRadixLevels; ++lv)
+	{
+		offset[lv] = run;
+		run += t->level_count[lv];
+	}
+	int lv = kRadixLevels - 1;
+	while(lv >= 0)
+	{
+		if(t->level_count[lv] == 0)
+		{
+			--lv;
+			continue;
+		}
+		if(t->level_count[lv] >= kRadixSmallLevel)
+		{
+			hipLaunchKernelGGL(k_radix_level, dim3(nbh::grid_for(ctx, t->level_count[lv], 256)), dim3(256), 0,
+			                   ctx->stream, t->level_list + offset[lv], t->level_count[lv], t->child, t->xyzr, t->mass,
+			                   t->bmin, t->bmax, t->quad, ratio_sqr);
+			NBH_LAUNCHED();
+			--lv;
+			continue;
+		}
+		int lo = lv;
+		while(lo > 0 && t->level_count[lo - 1] < kRadixSmallLevel) { --lo; }
+		hipLaunchKernelGGL(k_radix_levels_small, dim3(1), dim3(1024), 0, ctx->stream, t->level_list, t->level_info, lv,
+		                   lo, t->child, t->xyzr, t->mass, t->bmin, t->bmax, t->quad, ratio_sqr);
+		NBH_LAUNCHED();
+		lv = lo - 1;
+	}
+	hipLaunchKernelGGL(k_pack_records, dim3(nbh::grid_for(ctx, 2 * n, 256)), dim3(256), 0, ctx->stream, t->xyzr,
+	                   t->mass, t->quad, t->wrec, 2 * n);
+	NBH_LAUNCHED();
+	return NBH_OK;
+}
+
+nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mass, double ratio)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(y != nullptr && mass != nullptr, "null buffer");
+	NBH_CHECK(t->n >= 2, "the Morton tree needs at least 2 bodies");
+	NBH_TRY(radix_alloc(t));
+	nbh::scoped_timer timer(ctx, "tree_build");
+	const size_t n = t->n;
+	t->built = false;
+	const unsigned parts = std::min<unsigned>(kBoxParts, nbh::grid_for(ctx, n, 256));
+	hipLaunchKernelGGL(k_box_partial, dim3(parts), dim3(256), 0, ctx->stream, y, n, t->box + 6);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, ctx->stream, t->box + 6, static_cast<int>(parts), t->box);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_morton_codes, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, y, n, t->box,
+	                   t->keys, t->vals);
+	NBH_LAUNCHED();
+	uint32_t* order = nullptr;
+	// 63-bit keys: 8 passes, so the sorted keys end where they started
+	NBH_TRY(nbh::sort_pairs(ctx, t->keys, t->vals, t->keys + n, t->vals + n, n, 63, t->counts, t->scan_part, &order));
+	const unsigned gn = nbh::grid_for(ctx, 2 * n, 256);
+	hipLaunchKernelGGL(k_fill_int, dim3(gn), dim3(256), 0, ctx->stream, t->parent, 2 * n, 0);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_fill_int, dim3(gn), dim3(256), 0, ctx->stream, reinterpret_cast<int*>(t->gmark), 2 * n, 0);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_fill_int, dim3(gn), dim3(256), 0, ctx->stream, reinterpret_cast<int*>(t->smark), 2 * n, 0);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, ctx->stream, t->level_info,
+	                   static_cast<size_t>(3 * kRadixLevels + 2), 0);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_radix_nodes, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->keys,
+	                   static_cast<long long>(n), t->child, t->parent, t->level, t->level_info, t->gmark, t->smark);
+	NBH_LAUNCHED();
+	// groups and super-groups numbered in Morton order
+	hipLaunchKernelGGL(k_radix_flags, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->gmark,
+	                   t->smark, n, t->flags, t->flags + n);
+	NBH_LAUNCHED();
+	NBH_TRY(scan_exclusive(ctx, t->flags, n, t->scan_part));
+	NBH_TRY(scan_exclusive(ctx, t->flags + n, n, t->scan_part));
+	int* info = t->level_info + 3 * kRadixLevels;
+	hipLaunchKernelGGL(k_radix_compact, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->gmark,
+	                   t->smark, t->flags, t->flags + n, n, t->grp, t->sup, info);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_radix_group_super, dim3(nbh::grid_for(ctx, n / 16 + 1, 256)), dim3(256), 0, ctx->stream,
+	                   t->grp, t->sup, info, t->child, n);
+	NBH_LAUNCHED();
+	// internal ids listed by level
+	hipLaunchKernelGGL(k_radix_level_offsets, dim3(1), dim3(kRadixLevels), 0, ctx->stream, t->level_info,
+	                   t->level_info + kRadixLevels, t->level_info + 2 * kRadixLevels);
+	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_radix_level_lists, dim3(static_cast<unsigned>((n - 1 + 255) / 256)), dim3(256), 0,
+	                   ctx->stream, t->level, n, t->level_info + kRadixLevels, t->level_info + 2 * kRadixLevels,
+	                   t->level_list);
+	NBH_LAUNCHED();
+	// the level counts and the group counts drive the launches that follow
+	int host_info[3 * kRadixLevels + 2];
+	NBH_HIP(hipMemcpyAsync(host_info, t->level_info, sizeof(host_info), hipMemcpyDeviceToHost, ctx->stream));
+	NBH_HIP(hipStreamSynchronize(ctx->stream));
+	for(int lv = 0; lv < kRadixLevels; ++lv) { t->level_count[lv] = host_info[lv]; }
+	t->ngroups = host_info[3 * kRadixLevels];
+	t->nsupers = host_info[3 * kRadixLevels + 1];
+	NBH_CHECK(t->ngroups >= 1 && static_cast<size_t>(t->ngroups) <= n, "bad group count %d", t->ngroups);
+	NBH_CHECK(t->nsupers >= 0 && static_cast<size_t>(t->nsupers) <= n / (kRadixSuper / 2) + 1,
+	          "bad super-group count %d", t->nsupers);
+	NBH_TRY(radix_moments(ctx, t, y, mass, order, ratio, true));
+	t->radix = true;
+	t->built = true;
+	return NBH_OK;
+}
+
+nbh_status nbh_tree_get_children(nbh_ctx* ctx, const nbh_tree* t, int32_t* child, int32_t* parent)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	NBH_CHECK(t->built && t->radix, "no Morton tree has been built");
+	if(child != nullptr)
+	{
+		NBH_HIP(hipMemcpyAsync(child, t->child, t->n * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
+	}
+	if(parent != nullptr)
+	{
+		NBH_HIP(hipMemcpyAsync(parent, t->parent, 2 * t->n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	}
+	NBH_HIP(hipStreamSynchronize(ctx->stream));
+	return NBH_OK;
+}
+
 nbh_status nbh_tree_destroy(nbh_tree* t)
 {
 	if(t == nullptr)
@@ -2329,7 +2946,9 @@ nbh_status nbh_tree_destroy(nbh_tree* t)
 	}
 	void* ptrs[] = { t->list[0], t->list[1], t->list[2], t->tmp, t->side, t->keys, t->vals, t->counts, t->scan_part,
 	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->wrec, t->body_n, t->counters,
-	                 t->two_lists, t->two_count, t->two_coef, t->far_list, t->far_count };
+	                 t->two_lists, t->two_count, t->two_coef, t->far_list, t->far_count,
+	                 t->child, t->parent, t->level, t->level_list, t->level_info, t->gmark, t->smark, t->flags,
+	                 t->grp, t->sup, t->box };
 	for(void* p : ptrs)
 	{
 		if(p != nullptr) { (void)hipFree(p); }
@@ -2343,8 +2962,10 @@ nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* t, const double* y, const doub
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(y != nullptr && mass != nullptr, "null buffer");
+	NBH_CHECK((t->n & (t->n - 1)) == 0, "body count %zu is not a power of two (the heap tree needs 2^k leaves)", t->n);
 	nbh::scoped_timer timer(ctx, "tree_build");
 	const size_t n = t->n;
+	t->radix = false;
 	for(int a = 0; a < 3; ++a)
 	{
 		NBH_TRY(radix_sort(ctx, t, y + static_cast<size_t>(a) * n, t->list[a]));
@@ -2397,6 +3018,10 @@ nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* t, const double* y, double rat
 	NBH_CHECK(t->built, "tree has not been built");
 	NBH_CHECK(y != nullptr, "null buffer");
 	nbh::scoped_timer timer(ctx, "tree_refit");
+	if(t->radix)
+	{
+		return radix_moments(ctx, t, y, nullptr, nullptr, ratio, false);
+	}
 	return moments(ctx, t, y, nullptr, ratio, false);
 }
 
@@ -2406,6 +3031,7 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* t, const double* y, con
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(!t->radix, "this walk runs on the heap tree only (the last build made the Morton tree)");
 	NBH_CHECK(y != nullptr && mass != nullptr && f != nullptr && y != f, "bad buffers");
 	NBH_CHECK(layout == NBH_LAYOUT_HEAP || layout == NBH_LAYOUT_HEAP_STACKLESS, "unknown layout %d",
 	          static_cast<int>(layout));
@@ -2436,6 +3062,18 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* t, const double* y, con
 
 enum GroupMode { kGroupMonopole, kGroupQuad, kGroupLocal };
 
+static WalkTree walk_tree(const nbh_tree* t)
+{
+	WalkTree wt;
+	if(t->radix)
+	{
+		wt.child = t->child;
+		wt.grp = t->grp;
+		wt.sup = t->sup;
+	}
+	return wt;
+}
+
 static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, double* f, GroupMode mode,
                              float near_k)
 {
@@ -2443,12 +3081,38 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(t->built, "tree has not been built");
 	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
-	NBH_CHECK(t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
-	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
+	NBH_CHECK(t->radix || t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
+	NBH_CHECK(t->radix || (t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
 	const int n = static_cast<int>(t->n);
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
 	NBH_LAUNCHED();
 	nbh::scoped_timer timer(ctx, "bh_traverse");
+	if(t->radix)
+	{
+		// the Morton tree's depth is not bounded by log2(n): a step whose children
+		// would not fit on the stack takes the nodes as monopoles (counters[2])
+		const WalkTree wt = walk_tree(t);
+		if(mode == kGroupQuad)
+		{
+			hipLaunchKernelGGL((k_bh_group32<true, false, false, true>), dim3(static_cast<unsigned>(t->ngroups)), dim3(kGroup), 0,
+			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters,
+			                   TwoLevel(), wt);
+		}
+		else if(mode == kGroupLocal)
+		{
+			hipLaunchKernelGGL((k_bh_group32<true, true, false, true>), dim3(static_cast<unsigned>(t->ngroups)), dim3(kGroup), 0,
+			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters,
+			                   TwoLevel(), wt);
+		}
+		else
+		{
+			hipLaunchKernelGGL((k_bh_group32<false, false, false, true>), dim3(static_cast<unsigned>(t->ngroups)), dim3(kGroup), 0,
+			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, 0.0f, t->counters,
+			                   TwoLevel(), wt);
+		}
+		NBH_LAUNCHED();
+		return NBH_OK;
+	}
 	if(mode == kGroupQuad)
 	{
 		hipLaunchKernelGGL(k_bh_group32<true>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin,
@@ -2490,10 +3154,16 @@ constexpr int kTwoCap = 32768;
 
 // Bytes of the two-level walk's buffers: per super-group a list of `cap` node
 // indices, its length and the 34 coefficients of its expansion.
-static size_t two_bytes(size_t n, int cap)
+static size_t two_bytes(size_t supers, int cap)
 {
-	const size_t supers = n >> (6 + kSuperShift);
 	return supers * (static_cast<size_t>(cap) * sizeof(int) + sizeof(int) + 34 * sizeof(float));
+}
+
+// Super-groups of the tree's last build: 1024-body heap nodes, or the Morton
+// tree's subtrees of 512 to 1024 bodies.
+static size_t super_count(const nbh_tree* t)
+{
+	return t->radix ? static_cast<size_t>(t->nsupers) : t->n >> (6 + kSuperShift);
 }
 
 // Frees the two-level walk's buffers.  Every pointer is null afterwards, also
@@ -2509,6 +3179,7 @@ static nbh_status two_free(nbh_ctx* ctx, nbh_tree* t)
 	t->two_lists = nullptr;
 	t->two_count = nullptr;
 	t->two_coef = nullptr;
+	t->two_slots = 0;
 	for(void* p : ptrs)
 	{
 		if(p != nullptr)
@@ -2529,10 +3200,9 @@ static nbh_status two_free(nbh_ctx* ctx, nbh_tree* t)
 // so a later call allocates afresh instead of launching with a missing buffer.
 // NBH_TEST_FAIL_ALLOC=k (k = 1, 2, 3) in the environment makes the k-th of the
 // three allocations fail as out of memory (a test hook for this path).
-static nbh_status two_alloc(nbh_ctx* ctx, nbh_tree* t, int cap)
+static nbh_status two_alloc(nbh_ctx* ctx, nbh_tree* t, int cap, size_t supers)
 {
 	NBH_TRY(two_free(ctx, t));
-	const size_t supers = t->n >> (6 + kSuperShift);
 	const size_t sizes[3] = { supers * static_cast<size_t>(cap) * sizeof(int), supers * sizeof(int),
 	                          supers * 34 * sizeof(float) };
 	int fail_at = 0;
@@ -2557,6 +3227,7 @@ static nbh_status two_alloc(nbh_ctx* ctx, nbh_tree* t, int cap)
 	t->two_lists = static_cast<int*>(p[0]);
 	t->two_count = static_cast<int*>(p[1]);
 	t->two_coef = static_cast<float*>(p[2]);
+	t->two_slots = static_cast<int>(supers);
 	return NBH_OK;
 }
 
@@ -2594,11 +3265,12 @@ static nbh_status far_alloc(nbh_ctx* ctx, nbh_tree* t)
 // want_far.  Allocates the lists when they are missing.
 static nbh_status super_walk(nbh_ctx* ctx, nbh_tree* t, float near_k, bool want_far, TwoLevel* two)
 {
-	const size_t supers = t->n >> (6 + kSuperShift);
+	const size_t supers = super_count(t);
 	const int cap = t->two_cap > 0 ? t->two_cap : kTwoCap;
-	if(t->two_lists == nullptr || t->two_count == nullptr || t->two_coef == nullptr)
+	if(t->two_lists == nullptr || t->two_count == nullptr || t->two_coef == nullptr ||
+	   static_cast<size_t>(t->two_slots) < supers)
 	{
-		NBH_TRY(two_alloc(ctx, t, cap));
+		NBH_TRY(two_alloc(ctx, t, cap, supers));
 	}
 	if(want_far)
 	{
@@ -2622,8 +3294,18 @@ static nbh_status super_walk(nbh_ctx* ctx, nbh_tree* t, float near_k, bool want_
 		const int k = atoi(env);
 		if(k >= kGroup && k <= kWalkStack) { stack_limit = k; }
 	}
-	hipLaunchKernelGGL(k_bh_super, dim3(static_cast<unsigned>(supers)), dim3(kGroup), 0, ctx->stream, t->wrec,
-	                   t->bmin, t->bmax, n, super_near, 1.0 / t->ratio, *two, t->counters, stack_limit);
+	if(t->radix)
+	{
+		hipLaunchKernelGGL(k_bh_super<true>, dim3(static_cast<unsigned>(supers)), dim3(kGroup), 0, ctx->stream,
+		                   t->wrec, t->bmin, t->bmax, n, super_near, 1.0 / t->ratio, *two, t->counters, stack_limit,
+		                   walk_tree(t));
+	}
+	else
+	{
+		hipLaunchKernelGGL(k_bh_super<false>, dim3(static_cast<unsigned>(supers)), dim3(kGroup), 0, ctx->stream,
+		                   t->wrec, t->bmin, t->bmax, n, super_near, 1.0 / t->ratio, *two, t->counters, stack_limit,
+		                   WalkTree());
+	}
 	NBH_LAUNCHED();
 	return NBH_OK;
 }
@@ -2636,7 +3318,7 @@ static nbh_status two_level(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(net == nullptr || nbh::mlp_ctx(net) == ctx, "network does not belong to this context");
-	const size_t supers = t->n >> (6 + kSuperShift);
+	const size_t supers = super_count(t);
 	if(supers < 1)
 	{
 		NBH_CHECK(net == nullptr, "the learned two-level walk needs at least %d bodies", kGroup << kSuperShift);
@@ -2644,7 +3326,8 @@ static nbh_status two_level(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const
 	}
 	NBH_CHECK(t->built, "tree has not been built");
 	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
-	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
+	NBH_CHECK(t->radix || (t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
+	NBH_CHECK(net == nullptr || !t->radix, "the learned two-level walk runs on the heap tree only");
 	const int n = static_cast<int>(t->n);
 	hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, ctx->stream, t->counters);
 	NBH_LAUNCHED();
@@ -2660,8 +3343,18 @@ static nbh_status two_level(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const
 	{
 		// the group walks alone, inside bh_traverse
 		nbh::scoped_timer group_timer(ctx, "bh_group_walk");
-		hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec,
-		                   t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two);
+		if(t->radix)
+		{
+			hipLaunchKernelGGL((k_bh_group32<true, true, true, true>), dim3(static_cast<unsigned>(t->ngroups)),
+			                   dim3(kGroup), 0,
+			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two,
+			                   walk_tree(t));
+		}
+		else
+		{
+			hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream,
+			                   t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two);
+		}
 		NBH_LAUNCHED();
 	}
 	return NBH_OK;
@@ -2686,6 +3379,7 @@ nbh_status nbh_bh_super_samples(nbh_ctx* ctx, nbh_tree* t, float near_k, const u
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(!t->radix, "this walk runs on the heap tree only (the last build made the Morton tree)");
 	NBH_CHECK(near_k >= 1.0f, "near_k must be >= 1");
 	NBH_CHECK(supers != nullptr && count != nullptr, "null argument");
 	const size_t nsup = t->n >> (6 + kSuperShift);
@@ -2754,11 +3448,12 @@ nbh_status nbh_tree_set_list_cap(nbh_ctx* ctx, nbh_tree* t, int cap)
 	NBH_CHECK(cap >= 64 && cap <= (1 << 24), "list capacity %d: must be in [64, 2^24]", cap);
 	NBH_TRY(two_free(ctx, t));
 	t->two_cap = cap;
-	if((t->n >> (6 + kSuperShift)) < 1)
+	const size_t supers = t->built && t->radix ? super_count(t) : t->n >> (6 + kSuperShift);
+	if(supers < 1)
 	{
 		return NBH_OK;   // no super-group: the walk runs in one level and needs no lists
 	}
-	return two_alloc(ctx, t, cap);
+	return two_alloc(ctx, t, cap, supers);
 }
 
 nbh_status nbh_tree_bytes(nbh_ctx* ctx, const nbh_tree* t, size_t* bytes)
@@ -2766,7 +3461,8 @@ nbh_status nbh_tree_bytes(nbh_ctx* ctx, const nbh_tree* t, size_t* bytes)
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(bytes != nullptr, "null output");
-	*bytes = tree_base_bytes(t->n) + (t->two_lists != nullptr ? two_bytes(t->n, t->two_cap > 0 ? t->two_cap : kTwoCap) : 0) +
+	*bytes = tree_base_bytes(t->n) +
+	         (t->two_lists != nullptr ? two_bytes(t->two_slots, t->two_cap > 0 ? t->two_cap : kTwoCap) : 0) +
 	         (t->far_list != nullptr ? (t->n >> (6 + kSuperShift)) * (static_cast<size_t>(kFarCap) + 1) * sizeof(int) : 0);
 	return NBH_OK;
 }
@@ -2778,6 +3474,7 @@ nbh_status nbh_bh_band_samples(nbh_ctx* ctx, const nbh_tree* t, const uint32_t* 
 	NBH_TRY(nbh::enter(ctx));
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(!t->radix, "this walk runs on the heap tree only (the last build made the Morton tree)");
 	NBH_CHECK(t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
 	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");
 	NBH_CHECK(groups != nullptr && count != nullptr, "null argument");
@@ -2835,6 +3532,7 @@ nbh_status nbh_bh_fcompute_group_mlp(nbh_ctx* ctx, const nbh_tree* t, const nbh_
 	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
 	NBH_CHECK(net != nullptr && nbh::mlp_ctx(net) == ctx, "network does not belong to this context");
 	NBH_CHECK(t->built, "tree has not been built");
+	NBH_CHECK(!t->radix, "this walk runs on the heap tree only (the last build made the Morton tree)");
 	NBH_CHECK(y != nullptr && f != nullptr && y != f, "bad buffers");
 	NBH_CHECK(t->n >= static_cast<size_t>(kGroup), "group walk needs at least %d bodies", kGroup);
 	NBH_CHECK((t->levels + 2) * (kGroup / 2) <= kWalkStack, "tree too deep for the walk stack");

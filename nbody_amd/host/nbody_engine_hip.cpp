@@ -762,6 +762,7 @@ nbody_engine_hip_bh::nbody_engine_hip_bh(nbcoord_t ratio, e_traverse_type tt, si
 	m_quadrupole(false),
 	m_quad_near(3.0f),
 	m_local_far(false),
+	m_morton(false),
 	m_net_rho(0.0f),
 	m_net_near(0.0f),
 	m_net_two_level(false),
@@ -848,9 +849,16 @@ const char* nbody_engine_hip_bh::type_name() const
 bool nbody_engine_hip_bh::init(nbody_data* data)
 {
 	size_t n = data->get_count();
-	if(n == 0 || (n & (n - 1)) != 0)
+	if(m_morton ? n < 2 : (n == 0 || (n & (n - 1)) != 0))
 	{
-		nbody_log("hip_bh: body count %zu is not a power of two (heap layout needs 2^k leaves)", n);
+		if(m_morton)
+		{
+			nbody_log("hip_bh: body count %zu: the Morton tree needs at least 2 bodies", n);
+		}
+		else
+		{
+			nbody_log("hip_bh: body count %zu is not a power of two (heap layout needs 2^k leaves)", n);
+		}
 		return false;
 	}
 	if(!nbody_engine_hip::init(data))
@@ -907,7 +915,9 @@ void nbody_engine_hip_bh::fcompute(const nbcoord_t& t, const memory* _y, memory*
 	bool build = !m_tree_built || m_tree_build_rate == 0 || (m_data->get_step() % m_tree_build_rate) == 0;
 	if(build)
 	{
-		if(!check(nbh_tree_build(m_ctx, m_tree, y->data(), m_mass->data(), m_ratio), "tree build"))
+		const nbh_status st = m_morton ? nbh_tree_build_radix(m_ctx, m_tree, y->data(), m_mass->data(), m_ratio)
+		                               : nbh_tree_build(m_ctx, m_tree, y->data(), m_mass->data(), m_ratio);
+		if(!check(st, "tree build"))
 		{
 			return;
 		}
@@ -959,6 +969,7 @@ void nbody_engine_hip_bh::print_info() const
 	nbody_print("\ttree_layout: %s", tree_layout_name(m_tree_layout));
 	nbody_print("\ttree_build_rate: %zu", m_tree_build_rate);
 	nbody_print("\tbh_walk: %s", m_group_walk ? "group" : "body");
+	nbody_print("\tbh_tree: %s", m_morton ? "morton" : "heap");
 	nbody_print("\tbh_multipole: %s", m_quadrupole ? "quadrupole" : "monopole");
 	if(m_quadrupole)
 	{

@@ -155,6 +155,7 @@ class nbody_engine_hip_bh : public nbody_engine_hip
 	bool            m_quadrupole;
 	float           m_quad_near;
 	bool            m_local_far;
+	bool            m_morton;          // group walks on the Morton radix tree
 	std::vector<float> m_net_params;   // learned correction (empty: none)
 	float           m_net_rho;
 	float           m_net_near;
@@ -174,6 +175,9 @@ public:
 	//! Expand the far cells' monopole field about the group centre too (second
 	//! order) instead of sweeping it per body.
 	void set_local_far_field(bool l) { m_local_far = l; }
+	//! Build the Morton radix tree (nbh_tree_build_radix) instead of the heap
+	//! kd-tree; only the group walks run on it.
+	void set_morton_tree(bool m) { m_morton = m; }
 	//! Read a correction network file ("NBHMLP01", int32 parameter count,
 	//! float32 rho_min, float32 near_k, float32 parameters); rho >= 0 and
 	//! near_k >= 0 override the thresholds stored in it.  False if the file is

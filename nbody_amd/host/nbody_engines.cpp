@@ -210,11 +210,25 @@ nbody_engine* nbody_create_engine(const nbody_params& param)
 			nbody_log("learned_correction needs bh_walk=group");
 			return nullptr;
 		}
+		// the tree the group walks run on (no reference counterpart): the heap
+		// kd-tree of the reference, or a Morton radix tree
+		const std::string tree = param.value("bh_tree", "heap");
+		if(tree != "heap" && tree != "morton")
+		{
+			nbody_log("Invalid bh_tree. Allowed values are 'heap' or 'morton'");
+			return nullptr;
+		}
+		if(tree == "morton" && (walk != "group" || net != "none"))
+		{
+			nbody_log("bh_tree=morton needs bh_walk=group and no learned_correction");
+			return nullptr;
+		}
 		nbody_engine_hip_bh* engine = new nbody_engine_hip_bh(ratio, tt, static_cast<size_t>(rate), tl);
 		engine->set_group_walk(walk == "group");
 		engine->set_quadrupole(multipole == "quadrupole");
 		engine->set_quad_near(static_cast<float>(quad_near));
 		engine->set_local_far_field(far_field == "local");
+		engine->set_morton_tree(tree == "morton");
 		if(net != "none" && !engine->load_correction(net, rho, near_k))
 		{
 			delete engine;

@@ -961,3 +961,198 @@ void oracle_stats(size_t n, const double* pos, const double* vel, const double* 
 		out[11] = tm;
 	}
 }
+
+/* ------------------------------------------------------------------------
+ * Morton radix tree (nbh_tree_build_radix's definition; no reference
+ * counterpart).  Karras, "Maximizing parallelism in the construction of BVHs,
+ * octrees, and k-d trees" (HPG 2012), restated sequentially.
+ * ------------------------------------------------------------------------ */
+static unsigned long long radix_spread(unsigned long long v)
+{
+	v &= 0x1fffffull;
+	v = (v | (v << 32)) & 0x1f00000000ffffull;
+	v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+	v = (v | (v << 8)) & 0x100f00f00f00f00full;
+	v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+	v = (v | (v << 2)) & 0x1249249249249249ull;
+	return v;
+}
+
+static unsigned long long radix_cell(double x, double lo, double scale)
+{
+	double c = floor((x - lo) * scale);
+	if(c < 0.0) { c = 0.0; }
+	if(c > 2097151.0) { c = 2097151.0; }
+	return (unsigned long long)c;
+}
+
+typedef struct radix_key {
+	unsigned long long code;
+	unsigned body;
+} radix_key;
+
+static int radix_key_cmp(const void* a, const void* b)
+{
+	const radix_key* x = (const radix_key*)a;
+	const radix_key* y = (const radix_key*)b;
+	if(x->code != y->code) { return x->code < y->code ? -1 : 1; }
+	return x->body < y->body ? -1 : (x->body > y->body ? 1 : 0);
+}
+
+/* common prefix length of sorted keys i and j, the index as tie-breaking suffix */
+static int radix_delta(const radix_key* k, long long n, long long i, long long j)
+{
+	if(j < 0 || j >= n) { return -1; }
+	if(k[i].code == k[j].code)
+	{
+		return 64 + __builtin_clzll((unsigned long long)(i ^ j));
+	}
+	return __builtin_clzll(k[i].code ^ k[j].code);
+}
+
+static double radix_min(double a, double b) { return (b < a) ? b : a; }
+static double radix_max(double a, double b) { return (a < b) ? b : a; }
+
+/* internal id `id` from its children (the heap tree's update of a node) */
+static void radix_node(size_t id, size_t l, size_t r, double ratio_sqr, double* cm, double* nmass, double* crit,
+                       double* bmin, double* bmax, double* quad)
+{
+	const double ml = nmass[l], mr = nmass[r];
+	const double m = ml + mr;
+	double c[3], lo[3], hi[3];
+	for(int d = 0; d < 3; ++d)
+	{
+		c[d] = (cm[3 * l + d] * ml + cm[3 * r + d] * mr) / m;
+		lo[d] = radix_min(bmin[3 * l + d], bmin[3 * r + d]);
+		hi[d] = radix_max(bmax[3 * l + d], bmax[3 * r + d]);
+		bmin[3 * id + d] = lo[d];
+		bmax[3 * id + d] = hi[d];
+	}
+	const double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+	const double gx = (hi[0] + lo[0]) / 2.0 - c[0];
+	const double gy = (hi[1] + lo[1]) / 2.0 - c[1];
+	const double gz = (hi[2] + lo[2]) / 2.0 - c[2];
+	const double rad = sqrt(ex * ex + ey * ey + ez * ez) * 0.5 + sqrt(gx * gx + gy * gy + gz * gz);
+	nmass[id] = m;
+	for(int d = 0; d < 3; ++d) { cm[3 * id + d] = c[d]; }
+	crit[id] = (rad * rad) * ratio_sqr;
+	double q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	for(int side = 0; side < 2; ++side)
+	{
+		const size_t ch = side == 0 ? l : r;
+		const double mc = nmass[ch];
+		const double dx = cm[3 * ch] - c[0], dy = cm[3 * ch + 1] - c[1], dz = cm[3 * ch + 2] - c[2];
+		q[0] += quad[6 * ch + 0] + mc * dx * dx;
+		q[1] += quad[6 * ch + 1] + mc * dy * dy;
+		q[2] += quad[6 * ch + 2] + mc * dz * dz;
+		q[3] += quad[6 * ch + 3] + mc * dx * dy;
+		q[4] += quad[6 * ch + 4] + mc * dx * dz;
+		q[5] += quad[6 * ch + 5] + mc * dy * dz;
+	}
+	for(int k = 0; k < 6; ++k) { quad[6 * id + k] = q[k]; }
+}
+
+int oracle_radix_build(size_t n, const double* rx, const double* ry, const double* rz, const double* mass,
+                       double ratio, unsigned* order, int* child, int* parent, double* cm, double* nmass,
+                       double* crit, double* bmin, double* bmax, double* quad)
+{
+	if(n < 2) { return -1; }
+	radix_key* k = (radix_key*)malloc(n * sizeof(radix_key));
+	size_t* stack = (size_t*)malloc(2 * n * sizeof(size_t));
+	unsigned char* seen = (unsigned char*)calloc(2 * n, 1);
+	if(k == NULL || stack == NULL || seen == NULL)
+	{
+		free(k); free(stack); free(seen);
+		return -1;
+	}
+	double lo[3] = {rx[0], ry[0], rz[0]}, hi[3] = {rx[0], ry[0], rz[0]};
+	for(size_t i = 1; i < n; ++i)
+	{
+		const double v[3] = {rx[i], ry[i], rz[i]};
+		for(int d = 0; d < 3; ++d)
+		{
+			lo[d] = fmin(lo[d], v[d]);
+			hi[d] = fmax(hi[d], v[d]);
+		}
+	}
+	const double side = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+	const double scale = side > 0.0 ? 2097152.0 / side : 0.0;
+	for(size_t i = 0; i < n; ++i)
+	{
+		k[i].code = (radix_spread(radix_cell(rx[i], lo[0], scale)) << 2) |
+		            (radix_spread(radix_cell(ry[i], lo[1], scale)) << 1) | radix_spread(radix_cell(rz[i], lo[2], scale));
+		k[i].body = (unsigned)i;
+	}
+	qsort(k, n, sizeof(radix_key), radix_key_cmp);
+	memset(child, 0, 2 * n * sizeof(int));
+	memset(parent, 0, 2 * n * sizeof(int));
+	const long long nn = (long long)n;
+	for(long long i = 0; i < nn - 1; ++i)
+	{
+		const int dir = radix_delta(k, nn, i, i + 1) - radix_delta(k, nn, i, i - 1) > 0 ? 1 : -1;
+		const int dmin = radix_delta(k, nn, i, i - dir);
+		long long lmax = 2;
+		while(radix_delta(k, nn, i, i + lmax * dir) > dmin) { lmax *= 2; }
+		long long l = 0;
+		for(long long t = lmax / 2; t >= 1; t /= 2)
+		{
+			if(radix_delta(k, nn, i, i + (l + t) * dir) > dmin) { l += t; }
+		}
+		const long long j = i + l * dir;
+		const int dnode = radix_delta(k, nn, i, j);
+		/* binary search of the split: the last position whose key shares more
+		   than dnode bits with key i */
+		long long s = 0;
+		for(long long t = (l + 1) / 2;; t = (t + 1) / 2)
+		{
+			if(s + t <= l && radix_delta(k, nn, i, i + (s + t) * dir) > dnode) { s += t; }
+			if(t <= 1) { break; }
+		}
+		const long long gamma = i + s * dir + (dir < 0 ? -1 : 0);
+		const long long first = i < j ? i : j, last = i < j ? j : i;
+		const int left = first == gamma ? (int)(nn + gamma) : (int)(gamma + 1);
+		const int right = last == gamma + 1 ? (int)(nn + gamma + 1) : (int)(gamma + 2);
+		child[2 * (i + 1)] = left;
+		child[2 * (i + 1) + 1] = right;
+		parent[left] = (int)(i + 1);
+		parent[right] = (int)(i + 1);
+	}
+	memset(quad, 0, 12 * n * sizeof(double));
+	for(size_t j = 0; j < n; ++j)
+	{
+		const size_t id = n + j;
+		const unsigned b = k[j].body;
+		order[j] = b;
+		const double v[3] = {rx[b], ry[b], rz[b]};
+		for(int d = 0; d < 3; ++d)
+		{
+			cm[3 * id + d] = v[d];
+			bmin[3 * id + d] = v[d];
+			bmax[3 * id + d] = v[d];
+		}
+		nmass[id] = mass[b];
+		crit[id] = 0.0;
+	}
+	/* post-order from the root: a node is computed once both children are */
+	const double ratio_sqr = ratio * ratio;
+	size_t sp = 0;
+	stack[sp++] = 1;
+	while(sp > 0)
+	{
+		const size_t id = stack[sp - 1];
+		const size_t l = (size_t)child[2 * id], r = (size_t)child[2 * id + 1];
+		if(!seen[id])
+		{
+			seen[id] = 1;
+			if(r < n) { stack[sp++] = r; }
+			if(l < n) { stack[sp++] = l; }
+			continue;
+		}
+		--sp;
+		radix_node(id, l, r, ratio_sqr, cm, nmass, crit, bmin, bmax, quad);
+	}
+	free(k);
+	free(stack);
+	free(seen);
+	return 0;
+}

@@ -144,6 +144,25 @@ void oracle_bh_fcompute(const oracle_heap* h, int layout, int traverse, size_t n
 void oracle_bh_accel_targets(const oracle_heap* h, int layout, size_t n, const double* y, const double* mass,
                              const size_t* idx, size_t m, double* out, int threads);
 
+/* ---- Morton radix tree (no reference counterpart; the fast tree of the FP32
+        group walks, nbh_tree_build_radix) ----
+   Restated from its definition, sequentially: 63-bit Morton codes of the
+   positions on a 2^21-cell grid per axis over the bounding cube (cell =
+   floor((x - lo) * 2^21 / side), clamped to [0, 2^21 - 1]; side = the longest
+   extent of the bounding box), bodies sorted by (code, body number), internal
+   nodes by Karras' binary radix tree construction (node i covers a key range
+   with i at one end, split at the highest differing bit), then moments bottom-up
+   in post-order with the heap tree's formulas (children left then right).
+   Ids as nbh_tree_build_radix: root 1, internal i -> id i + 1, sorted leaf j ->
+   id n + j.  Arrays sized for ids 0..2n-1 (id 0 unused): order[n] sorted body
+   numbers, child[2n] (child[2 id], child[2 id + 1] for internal ids),
+   parent[2n] (0 for the root and id 0), cm[6n], mass[2n], crit[2n], bmin[6n],
+   bmax[6n], quad[12n] (second central moments xx yy zz xy xz yz).  Returns 0,
+   or -1 when n < 2 or memory runs out. */
+int oracle_radix_build(size_t n, const double* rx, const double* ry, const double* rz, const double* mass,
+                       double ratio, unsigned* order, int* child, int* parent, double* cm, double* nmass,
+                       double* crit, double* bmin, double* bmax, double* quad);
+
 /* ---- conservation checks (nbody/nbody_data.cpp:57-145, summation.h:19-45,
         summation_proxy.h:6-87) ----
    Bodies given in AoS form exactly as nbody_data holds them (pos[3n], vel[3n]).

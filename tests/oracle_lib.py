@@ -75,6 +75,8 @@ def lib():
             "oracle_heap_traverse_stackless": (None, [C.POINTER(heap_t), PD, C.c_double, PD]),
             "oracle_bh_fcompute": (None, [C.POINTER(heap_t), C.c_int, C.c_int, sz, PD, PD, PD, C.c_int]),
             "oracle_stats": (None, [sz, PD, PD, PD, C.c_int, PD]),
+            "oracle_radix_build": (C.c_int, [sz, PD, PD, PD, PD, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             PD, PD, PD, PD, PD, PD]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -267,6 +269,26 @@ class Heap:
         lib().oracle_bh_fcompute(C.byref(self.h), layout, traverse, self.n, p(y), p(np.ascontiguousarray(mass)),
                                  p(f), threads)
         return f
+
+
+def radix_build(y: np.ndarray, mass: np.ndarray, ratio: float) -> dict:
+    """The oracle's Morton radix tree of the state y[6n] (SoA): arrays by node id
+    (root 1, internal 1..n-1, sorted leaf j -> n + j)."""
+    n = len(mass)
+    rx, ry, rz = (np.ascontiguousarray(y[k * n:(k + 1) * n]) for k in range(3))
+    out = dict(order=np.empty(n, np.uint32), child=np.empty(2 * n, np.int32), parent=np.empty(2 * n, np.int32),
+               cm=np.zeros(6 * n), mass=np.zeros(2 * n), crit=np.zeros(2 * n), bmin=np.zeros(6 * n),
+               bmax=np.zeros(6 * n), quad=np.zeros(12 * n))
+    rc = lib().oracle_radix_build(n, p(rx), p(ry), p(rz), p(np.ascontiguousarray(mass)), ratio,
+                                  out["order"].ctypes.data_as(C.c_void_p), out["child"].ctypes.data_as(C.c_void_p),
+                                  out["parent"].ctypes.data_as(C.c_void_p), p(out["cm"]), p(out["mass"]),
+                                  p(out["crit"]), p(out["bmin"]), p(out["bmax"]), p(out["quad"]))
+    assert rc == 0
+    for k in ("cm", "bmin", "bmax"):
+        out[k] = out[k].reshape(-1, 3)
+    out["quad"] = out["quad"].reshape(-1, 6)
+    out["child"] = out["child"].reshape(-1, 2)
+    return out
 
 
 def stats(pos: np.ndarray, vel: np.ndarray, mass: np.ndarray, energy: bool = True) -> np.ndarray:
