@@ -161,7 +161,8 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
     mass = states[seed][2]
     desc = {"plummer": "Plummer sphere (a=1, M=1)",
             "two_plummer": "two Plummer spheres of n/2 bodies (a=1, M=1 each), centres 10 apart, approaching at 0.5"}
-    out = {"workload": f"Barnes-Hut heap kd-tree, FP32 forces / FP64 state, {desc.get(kind, kind)} N={n}, "
+    out = {"workload": f"Barnes-Hut (heap kd-tree, or Morton radix tree where a variant says so), FP32 forces / "
+                       f"FP64 state, {desc.get(kind, kind)} N={n}, "
                        f"seed {seed}, euler dt=1e-4 (and rk4 steps/s), tree rebuilt every force evaluation; "
                        f"force error on {samples} sampled bodies of the initial states of seeds {error_seeds}",
            "n": n, "error_seeds": error_seeds, "runs": []}
@@ -271,6 +272,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
         walk_ms = tw.value / max(1, cw.value)
         run = {
             "variant": label, "ratio": ratio, "theta": 1.0 / ratio,
+            "tree": params.get("bh_tree", "heap"),
             "multipole": params.get("bh_multipole", "monopole"),
             "quad_near": params.get("bh_quad_near", 3.0) if "bh_multipole" in params else None,
             "far_field": params.get("bh_far_field", "per_body") if "bh_multipole" in params else None,
@@ -278,7 +280,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             "ms_per_step": wall / steps * 1e3, "tree_build_ms": tb.value / max(1, cb.value),
             "walk_ms": walk_ms, "group_walk_ms": (tg.value / cg.value) if cg.value else None,
             "interactions_per_body": walk_counts[0] / n,
-            "node_visits_per_group": walk_counts[1] / max(1, n // 64),
+            "node_visits_per_body": walk_counts[1] / n,
             # node records read by the group walks: one 64-byte record per visit
             "node_GB_per_s": 64.0 * walk_counts[1] / (walk_ms * 1e-3) / 1e9,
             # nodes taken as monopoles because a walk stack was full, plus the
@@ -292,8 +294,9 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
             "p99_max_over_seeds": max(v["p99"] for v in errs.values()),
         }
         if train_info is not None or "bh_multipole" in params:
-            run["far_cells_per_group"] = walk_counts[3] / (n // 64)
-            run["near_cells_per_group"] = walk_counts[4] / (n // 64)
+            # per 64 bodies (one heap group; Morton groups hold up to 64)
+            run["far_cells_per_64_bodies"] = walk_counts[3] / (n / 64)
+            run["near_cells_per_64_bodies"] = walk_counts[4] / (n / 64)
         if train_info is not None:
             run["training"] = train_info
         out["runs"].append(run)
@@ -307,7 +310,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
     ok = [r for r in out["runs"] if r["rms_max_over_seeds"] <= 1e-3]
     if ok:
         best = max(ok, key=lambda r: r["steps_per_s"])
-        out["best_at_rms_1e-3"] = {"variant": best["variant"], "steps_per_s": best["steps_per_s"],
+        out["best_at_rms_1e-3"] = {"variant": best["variant"], "tree": best["tree"], "steps_per_s": best["steps_per_s"],
                                    "rms": best["force_rel_err"]["rms"],
                                    "rms_max_over_seeds": best["rms_max_over_seeds"],
                                    "walk_ms": best["walk_ms"], "tree_build_ms": best["tree_build_ms"]}
@@ -318,14 +321,17 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
     # walks of the two-level walk, their live HIP-event duration against the VALU
     # instructions and HBM bytes per launch that the counter runs in profiles/ measured
     # for this variant and state
-    rl = bh_roofline_from_profile(n)
+    best_variant = out.get("best_at_rms_1e-3", {}).get("variant")
+    rl = bh_roofline_from_profile(n, best_variant)
     if rl is not None:
         r = [x for x in out["runs"] if x["variant"] == rl["variant"] and x.get("group_walk_ms")]
         if r:
             ms = r[0]["group_walk_ms"]
             rate = rl["valu"] / 1024 / (ms * 1e-3) / (CLOCK_GHZ * 1e9)
+            kern = "k_bh_group32<true, true, true, true>" if r[0]["tree"] == "morton" else \
+                "k_bh_group32<true, true, true>"
             out["roofline"] = {
-                "kernel": "k_bh_group32<true, true, true> (group walks of the two-level walk)",
+                "kernel": f"{kern} (group walks of the two-level walk)",
                 "variant": rl["variant"], "bound": "valu",
                 "achieved": rate, "peak": 0.5, "unit": "VALU wave instructions per SIMD per clock",
                 "frac": rate / 0.5, "traffic": rl["traffic"],
@@ -337,7 +343,7 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
     # (the RMS over the samples alone can swing with one body near a force
     # cancellation), and its speed relative to the plain run
     def plain(r):
-        return r["multipole"] == "monopole" and not r["variant"].startswith("learned")
+        return r["multipole"] == "monopole" and r["tree"] == "heap" and not r["variant"].startswith("learned")
 
     learned_runs = [r for r in out["runs"] if r["variant"].startswith("learned")]
     base_runs = [r for r in out["runs"] if r["variant"] == "plain theta=0.5"]
@@ -385,28 +391,65 @@ def bh_bench(n: int, steps: int, warmup: int, variants, samples: int = 8192, kin
     return out
 
 
-def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dict:
+def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs"), precisions=("fp64", "fp32", "fp32_corr"),
+                        stream_base: str = "") -> dict:
     """Cold cube [-1,1]^3, v = 0, masses lognormal(0, 0.5) normalised to 1 (seed 5):
     `steps` steps of max_step 1e-3 (rkfeagin10 with one level of 8 adaptive
     substeps, bs with max_level 8) with FP32 force tiles, alone and with the FP64
     correction pass of the nearest sources (+ the compensated FP64 Butcher
     combination for rkfeagin10), against all-FP64; energy drift and the
     absolute total momentum / angular momentum of both runs (P0 = L0 = 0, so the
-    reference's percentage errors are undefined for this state)."""
+    reference's percentage errors are undefined for this state).
+
+    stream_base: run in chunks through frame streams (nbody_data_stream):
+    <stream_base>_<solver>_<precision> holds the initial state as frame 0 and the
+    state at the end of every chunk; a call continues from the last frame and
+    appends its own, and the drift is taken against frame 0."""
     import numpy as np
 
     from nbody_amd import nbody_create_engine, nbody_create_solver, nbody_data
+    from nbody_amd import nbody_data_stream, nbody_data_stream_reader
     out = {"workload": f"cold cube N={n}, lognormal(0.5) masses, {steps} steps of 1e-3", "runs": []}
+
+    def engine_for(d, prec):
+        e = nbody_create_engine({"engine": "hip", "force_precision": prec})
+        if e is None or not e.init(d):
+            raise RuntimeError("hip engine init failed")
+        return e
+
+    def fresh():
+        d = nbody_data()
+        d.set_quiet(True)
+        d.generate("cube", n, 5)
+        d.set_check_list("PLEV")
+        return d
+
     for solver in solvers:
         res = {"solver": solver}
-        for prec in ("fp64", "fp32", "fp32_corr"):
-            d = nbody_data()
-            d.set_quiet(True)
-            d.generate("cube", n, 5)
-            d.set_check_list("PLEV")
-            e = nbody_create_engine({"engine": "hip", "force_precision": prec})
-            if e is None or not e.init(d):
-                raise RuntimeError("hip engine init failed")
+        for prec in precisions:
+            d = fresh()
+            reader = None
+            base = f"{stream_base}_{solver}_{prec}" if stream_base else ""
+            if base and os.path.exists(base + ".idx"):
+                reader = nbody_data_stream_reader()
+                if reader.load(base) != 0 or reader.seek(0) != 0 or reader.read(d) != 0:
+                    raise RuntimeError(f"cannot read frame 0 of {base}")
+            e0_engine = engine_for(d, prec)
+            st0 = d.statistics(e0_engine)
+            e0_engine.destroy()
+            stream = None
+            if base:
+                if reader is not None:
+                    last = reader.get_frame_count() - 1
+                    if reader.seek(last) != 0 or reader.read(d) != 0:
+                        raise RuntimeError(f"cannot read frame {last} of {base}")
+                stream = nbody_data_stream()
+                if stream.open(base, 1 << 30, reader) != 0:
+                    raise RuntimeError(f"cannot open stream {base}")
+                if reader is None and stream.write(d) != 0:
+                    raise RuntimeError(f"cannot write frame 0 of {base}")
+            t_start, step_start = d.get_time(), d.get_step()
+            e = engine_for(d, prec)
             params = {"solver": solver, "max_step": 1e-3}
             if solver == "bs":
                 params["max_level"] = 8
@@ -418,7 +461,6 @@ def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dic
                 params["correction"] = True
             s = nbody_create_solver(params)
             s.set_engine(e)
-            st0 = d.statistics(e)
             t0 = time.perf_counter()
             for i in range(steps):
                 s.advise(1e-3)
@@ -427,10 +469,17 @@ def split_precision_run(n: int, steps: int, solvers=("rkfeagin10", "bs")) -> dic
                         f"{e.get_compute_count()} force evaluations, {time.perf_counter() - t0:.1f} s")
             e.get_data(d)
             st1 = d.statistics(e)
+            if stream is not None:
+                if stream.write(d) != 0:
+                    raise RuntimeError(f"cannot append to {base}")
+                stream.close()
+            if reader is not None:
+                reader.close()
             e0, e1 = st0[6] + st0[7], st1[6] + st1[7]
             res[prec] = {"energy_drift_rel": (e1 - e0) / abs(e0), "abs_P": float(np.linalg.norm(st1[:3])),
                          "abs_L": float(np.linalg.norm(st1[3:6])), "seconds": time.perf_counter() - t0,
-                         "force_evaluations": e.get_compute_count()}
+                         "force_evaluations": e.get_compute_count(), "t_start": t_start, "t_end": d.get_time(),
+                         "steps_start": step_start, "steps_end": d.get_step()}
             s.destroy()
             e.destroy()
         out["runs"].append(res)
@@ -547,13 +596,16 @@ def traffic_from_profile(n: int):
 CLOCK_GHZ = 2.4
 BH_COUNTERS = "bh_two_level_walk_counters_plummer_{n}.md"
 BH_COUNTER_VARIANT = {"theta=0.62 near=3": "quadrupole theta=0.62, local far field, near 3"}
+for _th in ("0.5", "0.55", "0.58", "0.62", "0.7"):
+    BH_COUNTER_VARIANT[f"morton theta={_th} near=3"] = f"morton tree: quadrupole theta={_th}, local far field, near 3"
 
 
-def bh_roofline_from_profile(n: int):
+def bh_roofline_from_profile(n: int, prefer=None):
     """VALU wave instructions and HBM bytes per launch of the two-level group walk
     at this N, from the counter summary in profiles/ (lines
     'valu_wave_instr_per_launch <key>: <count>' and 'traffic_bytes_per_launch
-    <key>: <bytes>'), with the bench variant they were measured on; or None."""
+    <key>: <bytes>'), with the bench variant they were measured on: the variant
+    `prefer` where the summary holds it, else the first one it holds; or None."""
     path = ROOT / "profiles" / BH_COUNTERS.format(n=n)
     if not path.exists():
         return None
@@ -563,11 +615,13 @@ def bh_roofline_from_profile(n: int):
             if line.startswith(tag) and ":" in line:
                 key, val = line[len(tag):].split(":", 1)
                 got.setdefault(key.strip(), {})[field] = float(val)
-    for key, vals in got.items():
-        if key in BH_COUNTER_VARIANT and "valu" in vals and "traffic" in vals:
-            return {"variant": BH_COUNTER_VARIANT[key], "valu": vals["valu"], "traffic": vals["traffic"],
-                    "source": f"profiles/{path.name}"}
-    return None
+    found = [{"variant": BH_COUNTER_VARIANT[key], "valu": vals["valu"], "traffic": vals["traffic"],
+              "source": f"profiles/{path.name}"}
+             for key, vals in got.items() if key in BH_COUNTER_VARIANT and "valu" in vals and "traffic" in vals]
+    for f in found:
+        if f["variant"] == prefer:
+            return f
+    return found[0] if found else None
 
 
 def bh_variants(learned: bool = True):
@@ -591,6 +645,11 @@ def bh_variants(learned: bool = True):
           {**local, "bh_quad_near": 5.0}),
          ("quadrupole theta=0.9, local far field, near 4", 1.0 / 0.9, "quadrupole",
           {**local, "bh_quad_near": 4.0})]
+    # the same two-level local walk on the Morton radix tree, whose cells are
+    # compact where the heap's count-balanced cells span from the core outwards
+    for th in (0.5, 0.55, 0.58, 0.62, 0.7):
+        v.append((f"morton tree: quadrupole theta={th:g}, local far field, near 3", 1.0 / th, "quadrupole",
+                  {**local, "bh_quad_near": 3.0, "bh_tree": "morton"}))
     if learned:
         v.append(("learned theta=0.8, two-level local walk, near 5", 1.25, "two_level", {"bh_quad_near": 5.0}))
     return v
@@ -677,6 +736,12 @@ def main(argv=None) -> int:
                     help="skip the colliding two-Plummer Barnes-Hut workload (N=2^20)")
     ap.add_argument("--split-precision", type=int, default=0, metavar="STEPS",
                     help="also run the cold-cube split-precision workload (N=65536) for STEPS steps")
+    ap.add_argument("--split-solver", choices=["rkfeagin10", "bs"], action="append",
+                    help="split-precision solvers (default: both)")
+    ap.add_argument("--split-prec", choices=["fp64", "fp32", "fp32_corr"], action="append",
+                    help="split-precision force precisions (default: all three)")
+    ap.add_argument("--split-stream", default="", metavar="BASE",
+                    help="run the split-precision workload in chunks through frame streams BASE_<solver>_<prec>")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch the kernels measured by the hardware-counter runs, then exit")
     ap.add_argument("--bh-probe", metavar="VARIANT",
@@ -838,7 +903,9 @@ def main(argv=None) -> int:
         log("vector ops")
         out["vector_ops"] = vector_ops_bench(1 << 22)
     if a.split_precision > 0:
-        out["split_precision"] = split_precision_run(1 << 16, a.split_precision)
+        out["split_precision"] = split_precision_run(
+            1 << 16, a.split_precision, tuple(a.split_solver or ("rkfeagin10", "bs")),
+            tuple(a.split_prec or ("fp64", "fp32", "fp32_corr")), a.split_stream)
     print(json.dumps(out), flush=True)
     return 0
 

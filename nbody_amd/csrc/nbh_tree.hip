@@ -100,6 +100,7 @@ struct nbh_tree {
 	int4*     grp = nullptr;       // [n] walk groups (node, first leaf, leaves, super-group)
 	int2*     sup = nullptr;       // [n / 512 + 1] super-groups (node, first leaf)
 	double*   box = nullptr;       // [6 * (kBoxParts + 1)] partial boxes, then the bounding box
+	double4*  pk = nullptr;        // [n] position and mass of each body, packed for the sorted gather
 	int       ngroups = 0;         // walk groups and super-groups of the last Morton build
 	int       nsupers = 0;
 	int       level_count[kRadixLevels] = {};   // host copy of the level counts (refit)
@@ -695,19 +696,21 @@ __device__ inline uint64_t morton_cell(double x, double lo, double scale)
 
 // box[6] = bounding box lo.xyz hi.xyz of the bodies: scale = 2^21 / longest side
 __global__ void __launch_bounds__(256)
-k_morton_codes(const double* __restrict__ y, size_t n, const double* __restrict__ box, uint64_t* __restrict__ keys,
-               uint32_t* __restrict__ vals)
+k_morton_codes(const double* __restrict__ y, const double* __restrict__ mass, size_t n, const double* __restrict__ box,
+               uint64_t* __restrict__ keys, uint32_t* __restrict__ vals, double4* __restrict__ pk)
 {
 	const double side = fmax(fmax(box[3] - box[0], box[4] - box[1]), box[5] - box[2]);
 	const double scale = side > 0.0 ? 2097152.0 / side : 0.0;
 	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
 	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
 	{
-		const uint64_t cx = morton_cell(y[i], box[0], scale);
-		const uint64_t cy = morton_cell(y[n + i], box[1], scale);
-		const uint64_t cz = morton_cell(y[2 * n + i], box[2], scale);
+		const double x = y[i], yy = y[n + i], z = y[2 * n + i];
+		const uint64_t cx = morton_cell(x, box[0], scale);
+		const uint64_t cy = morton_cell(yy, box[1], scale);
+		const uint64_t cz = morton_cell(z, box[2], scale);
 		keys[i] = (morton_spread(cx) << 2) | (morton_spread(cy) << 1) | morton_spread(cz);
 		vals[i] = static_cast<uint32_t>(i);
+		pk[i] = make_double4(x, yy, z, mass[i]);
 	}
 }
 
@@ -781,7 +784,8 @@ constexpr int kRadixSuper = 1024;
 // Internal node i of [0, n - 1): its key range, split and children (ids), the
 // parent links of the children, its level (the common prefix length of its
 // keys, which grows strictly from a node to its children, so nodes of one level
-// are independent), the count of each level, and the marks of walk groups and
+// are independent; -1 inside a walk group, whose nodes k_radix_group_moments
+// takes), the count of each level, and the marks of walk groups and
 // super-groups at the first sorted position of their leaves: (node id, leaves).
 __global__ void __launch_bounds__(256)
 k_radix_nodes(const uint64_t* __restrict__ k, long long n, int2* __restrict__ child, int* __restrict__ parent,
@@ -819,9 +823,9 @@ k_radix_nodes(const uint64_t* __restrict__ k, long long n, int2* __restrict__ ch
 		child[id] = make_int2(left, right);
 		parent[left] = id;
 		parent[right] = id;
-		level[id] = dnode;
-		atomicAdd(&hist[dnode], 1);
 		const long long cnt = hi - lo + 1;
+		level[id] = cnt > kRadixGroup ? dnode : -1;
+		if(cnt > kRadixGroup) { atomicAdd(&hist[dnode], 1); }
 		const long long cl = gamma - lo + 1, cr = hi - gamma;
 		if(id == kRoot && cnt <= kRadixGroup) { gmark[lo] = make_int2(id, static_cast<int>(cnt)); }
 		if(id == kRoot && cnt <= kRadixSuper && 2 * cnt >= kRadixSuper) { smark[lo] = make_int2(id, static_cast<int>(cnt)); }
@@ -956,7 +960,7 @@ k_radix_level_lists(const int* __restrict__ level, size_t n, const int* __restri
 	if(id < n)
 	{
 		lv = level[id];
-		rank = atomicAdd(&cnt[lv], 1);
+		if(lv >= 0) { rank = atomicAdd(&cnt[lv], 1); }
 	}
 	__syncthreads();
 	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x)
@@ -967,13 +971,68 @@ k_radix_level_lists(const int* __restrict__ level, size_t n, const int* __restri
 	if(lv >= 0) { list[base[lv] + rank] = static_cast<int>(id); }
 }
 
+// Walk record of node id from its moments (as k_pack_records).
+__device__ inline void put_record(int id, const double4* __restrict__ xyzr, const double* __restrict__ nmass,
+                                  const double* __restrict__ quad, nbh_walk_rec* __restrict__ wr)
+{
+	nbh_walk_rec r;
+	r.c = xyzr[id];
+	r.m = nmass[id];
+	for(int k = 0; k < 6; ++k) { r.q[k] = static_cast<float>(quad[6 * static_cast<size_t>(id) + k]); }
+	wr[id] = r;
+}
+
+// Leaves of the Morton tree (as k_leaves) with their walk records.  kBuild: the
+// body at sorted position s is order[s], read from its packed record; refit:
+// the body numbers of the build, positions from y and the masses kept.
+template <bool kBuild>
+__global__ void __launch_bounds__(256)
+k_radix_leaves(const uint32_t* __restrict__ order, const double4* __restrict__ pk, const double* __restrict__ y,
+               size_t n, double4* __restrict__ xyzr, double* __restrict__ nmass, double* __restrict__ bmin,
+               double* __restrict__ bmax, int32_t* __restrict__ body_n, nbh_walk_rec* __restrict__ wr)
+{
+	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+	for(size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < n; s += stride)
+	{
+		const size_t idx = n + s;
+		double x, yy, z, m;
+		if constexpr(kBuild)
+		{
+			const uint32_t b = order[s];
+			const double4 p = pk[b];
+			x = p.x;
+			yy = p.y;
+			z = p.z;
+			m = p.w;
+			body_n[idx] = static_cast<int32_t>(b);
+			nmass[idx] = m;
+		}
+		else
+		{
+			const uint32_t b = static_cast<uint32_t>(body_n[idx]);
+			x = y[b];
+			yy = y[n + b];
+			z = y[2 * n + b];
+			m = nmass[idx];
+		}
+		xyzr[idx] = make_double4(x, yy, z, 0.0);
+		bmin[3 * idx] = x; bmin[3 * idx + 1] = yy; bmin[3 * idx + 2] = z;
+		bmax[3 * idx] = x; bmax[3 * idx + 1] = yy; bmax[3 * idx + 2] = z;
+		nbh_walk_rec r;
+		r.c = make_double4(x, yy, z, 0.0);
+		r.m = m;
+		for(int k = 0; k < 6; ++k) { r.q[k] = 0.0f; }
+		wr[idx] = r;
+	}
+}
+
 // Moments of the internal ids list[0, count) of one level, from their children
 // (the heap's update_node and update_quad); the levels run from the longest
 // prefix to the root, one launch each.
 __global__ void __launch_bounds__(256)
 k_radix_level(const int* __restrict__ list, int count, const int2* __restrict__ child, double4* __restrict__ xyzr,
               double* __restrict__ nmass, double* __restrict__ bmin, double* __restrict__ bmax,
-              double* __restrict__ quad, double ratio_sqr)
+              double* __restrict__ quad, double ratio_sqr, nbh_walk_rec* __restrict__ wr)
 {
 	const int stride = gridDim.x * blockDim.x;
 	for(int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride)
@@ -982,6 +1041,63 @@ k_radix_level(const int* __restrict__ list, int count, const int2* __restrict__ 
 		const int2 c = child[id];
 		update_node(id, c.x, c.y, xyzr, nmass, bmin, bmax, ratio_sqr);
 		update_quad(id, c.x, c.y, xyzr, nmass, quad);
+		put_record(id, xyzr, nmass, quad, wr);
+	}
+}
+
+// Moments of the internal nodes of each walk group (a subtree of at most
+// kRadixGroup leaves), one wave per group: the group's internal nodes listed
+// breadth-first in LDS, then taken depth by depth from the deepest.
+__global__ void __launch_bounds__(kRadixGroup)
+k_radix_group_moments(const int4* __restrict__ grp, int ngroups, int n, const int2* __restrict__ child,
+                      double4* __restrict__ xyzr, double* __restrict__ nmass, double* __restrict__ bmin,
+                      double* __restrict__ bmax, double* __restrict__ quad, double ratio_sqr,
+                      nbh_walk_rec* __restrict__ wr)
+{
+	__shared__ int q[kRadixGroup];          // internal nodes, breadth-first (at most kRadixGroup - 1)
+	__shared__ int start[kRadixGroup + 1];  // first entry of each depth
+	const int g = blockIdx.x;
+	if(g >= ngroups) { return; }
+	const int lane = threadIdx.x;
+	const int root = grp[g].x;
+	if(root >= n) { return; }              // a group of one leaf
+	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	if(lane == 0)
+	{
+		q[0] = root;
+		start[0] = 0;
+	}
+	int head = 0, tail = 1, depth = 0;
+	__syncthreads();
+	while(head < tail)
+	{
+		const int cnt = tail - head;
+		const bool act = lane < cnt;
+		const int2 c = act ? child[q[head + lane]] : make_int2(n, n);
+		const bool li = c.x < n, ri = c.y < n;
+		const uint64_t bl = __ballot(li), br = __ballot(ri);
+		int pos = tail + __popcll(bl & lt) + __popcll(br & lt);
+		if(li) { q[pos++] = c.x; }
+		if(ri) { q[pos] = c.y; }
+		head = tail;
+		tail += __popcll(bl) + __popcll(br);
+		++depth;
+		if(lane == 0) { start[depth] = head; }
+		__syncthreads();
+	}
+	// depth d holds q[start[d], start[d + 1]); start[depth] == tail
+	for(int d = depth - 1; d >= 0; --d)
+	{
+		const int i = start[d] + lane;
+		if(i < start[d + 1])
+		{
+			const int id = q[i];
+			const int2 c = child[id];
+			update_node(id, c.x, c.y, xyzr, nmass, bmin, bmax, ratio_sqr);
+			update_quad(id, c.x, c.y, xyzr, nmass, quad);
+			put_record(id, xyzr, nmass, quad, wr);
+		}
+		__syncthreads();
 	}
 }
 
@@ -991,7 +1107,7 @@ __global__ void __launch_bounds__(1024)
 k_radix_levels_small(const int* __restrict__ list, const int* __restrict__ info, int lv_hi, int lv_lo,
                      const int2* __restrict__ child, double4* __restrict__ xyzr, double* __restrict__ nmass,
                      double* __restrict__ bmin, double* __restrict__ bmax, double* __restrict__ quad,
-                     double ratio_sqr)
+                     double ratio_sqr, nbh_walk_rec* __restrict__ wr)
 {
 	for(int lv = lv_hi; lv >= lv_lo; --lv)
 	{
@@ -1002,6 +1118,7 @@ k_radix_levels_small(const int* __restrict__ list, const int* __restrict__ info,
 			const int2 c = child[id];
 			update_node(id, c.x, c.y, xyzr, nmass, bmin, bmax, ratio_sqr);
 			update_quad(id, c.x, c.y, xyzr, nmass, quad);
+			put_record(id, xyzr, nmass, quad, wr);
 		}
 		__syncthreads();
 	}
@@ -2759,12 +2876,13 @@ static nbh_status radix_alloc(nbh_tree* t)
 		return NBH_OK;
 	}
 	const size_t n = t->n;
-	constexpr int kBufs = 11;
+	constexpr int kBufs = 12;
 	void* p[kBufs] = {};
 	const size_t sizes[kBufs] = { n * sizeof(int2), 2 * n * sizeof(int), n * sizeof(int), n * sizeof(int),
 	                              (3 * kRadixLevels + 2) * sizeof(int), n * sizeof(int2), n * sizeof(int2),
 	                              2 * n * sizeof(uint32_t), n * sizeof(int4),
-	                              (n / (kRadixSuper / 2) + 1) * sizeof(int2), 6 * (kBoxParts + 1) * sizeof(double) };
+	                              (n / (kRadixSuper / 2) + 1) * sizeof(int2), 6 * (kBoxParts + 1) * sizeof(double),
+	                              n * sizeof(double4) };
 	hipError_t e = hipSuccess;
 	for(int k = 0; k < kBufs && e == hipSuccess; ++k)
 	{
@@ -2789,24 +2907,39 @@ static nbh_status radix_alloc(nbh_tree* t)
 	t->grp = static_cast<int4*>(p[8]);
 	t->sup = static_cast<int2*>(p[9]);
 	t->box = static_cast<double*>(p[10]);
+	t->pk = static_cast<double4*>(p[11]);
 	return NBH_OK;
 }
 
 // Levels with fewer internal nodes than this run together in one work-group.
 constexpr int kRadixSmallLevel = 4096;
 
-// Moments of the Morton tree (its structure and level lists already built),
-// level by level from the longest prefix up to the root, then the packed walk
-// records.  build: leaves from `order` (sorted body numbers), else from the body
-// numbers of the last build.
-static nbh_status radix_moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const double* mass, const uint32_t* order,
-                                double ratio, bool build)
+// Moments and walk records of the Morton tree (its structure, groups and level
+// lists already built): inside the walk groups one wave per group, then the
+// nodes above them level by level from the longest prefix up to the root.
+// build: leaves from `order` (sorted body numbers) and the packed bodies, else
+// from the body numbers of the last build and the positions y.
+static nbh_status radix_moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const uint32_t* order, double ratio,
+                                bool build)
 {
 	const size_t n = t->n;
 	t->ratio = ratio;
 	const double ratio_sqr = ratio * ratio;
-	hipLaunchKernelGGL(k_leaves, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, order, y, mass, n,
-	                   t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, build ? 1 : 0);
+	if(build)
+	{
+		hipLaunchKernelGGL(k_radix_leaves<true>, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, order,
+		                   t->pk, y, n, t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, t->wrec);
+	}
+	else
+	{
+		hipLaunchKernelGGL(k_radix_leaves<false>, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, order,
+		                   t->pk, y, n, t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, t->wrec);
+	}
+	NBH_LAUNCHED();
+	// inside the walk groups, then the levels above them
+	hipLaunchKernelGGL(k_radix_group_moments, dim3(static_cast<unsigned>(t->ngroups)), dim3(kRadixGroup), 0,
+	                   ctx->stream, t->grp, t->ngroups, static_cast<int>(n), t->child, t->xyzr, t->mass, t->bmin,
+	                   t->bmax, t->quad, ratio_sqr, t->wrec);
 	NBH_LAUNCHED();
 	int offset[kRadixLevels];
 	int run = 0;
@@ -2827,7 +2960,7 @@ static nbh_status radix_moments(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 		{
 			hipLaunchKernelGGL(k_radix_level, dim3(nbh::grid_for(ctx, t->level_count[lv], 256)), dim3(256), 0,
 			                   ctx->stream, t->level_list + offset[lv], t->level_count[lv], t->child, t->xyzr, t->mass,
-			                   t->bmin, t->bmax, t->quad, ratio_sqr);
+			                   t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec);
 			NBH_LAUNCHED();
 			--lv;
 			continue;
@@ -2835,13 +2968,10 @@ static nbh_status radix_moments(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 		int lo = lv;
 		while(lo > 0 && t->level_count[lo - 1] < kRadixSmallLevel) { --lo; }
 		hipLaunchKernelGGL(k_radix_levels_small, dim3(1), dim3(1024), 0, ctx->stream, t->level_list, t->level_info, lv,
-		                   lo, t->child, t->xyzr, t->mass, t->bmin, t->bmax, t->quad, ratio_sqr);
+		                   lo, t->child, t->xyzr, t->mass, t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec);
 		NBH_LAUNCHED();
 		lv = lo - 1;
 	}
-	hipLaunchKernelGGL(k_pack_records, dim3(nbh::grid_for(ctx, 2 * n, 256)), dim3(256), 0, ctx->stream, t->xyzr,
-	                   t->mass, t->quad, t->wrec, 2 * n);
-	NBH_LAUNCHED();
 	return NBH_OK;
 }
 
@@ -2860,8 +2990,8 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	NBH_LAUNCHED();
 	hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, ctx->stream, t->box + 6, static_cast<int>(parts), t->box);
 	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_morton_codes, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, y, n, t->box,
-	                   t->keys, t->vals);
+	hipLaunchKernelGGL(k_morton_codes, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, y, mass, n,
+	                   t->box, t->keys, t->vals, t->pk);
 	NBH_LAUNCHED();
 	uint32_t* order = nullptr;
 	// 63-bit keys: 8 passes, so the sorted keys end where they started
@@ -2901,8 +3031,10 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	                   t->level_list);
 	NBH_LAUNCHED();
 	// the level counts and the group counts drive the launches that follow
-	int host_info[3 * kRadixLevels + 2];
-	NBH_HIP(hipMemcpyAsync(host_info, t->level_info, sizeof(host_info), hipMemcpyDeviceToHost, ctx->stream));
+	constexpr size_t kInfoBytes = (3 * kRadixLevels + 2) * sizeof(int);
+	NBH_TRY(nbh::pinned(ctx, kInfoBytes));
+	const int* host_info = reinterpret_cast<const int*>(ctx->pinned);
+	NBH_HIP(hipMemcpyAsync(ctx->pinned, t->level_info, kInfoBytes, hipMemcpyDeviceToHost, ctx->stream));
 	NBH_HIP(hipStreamSynchronize(ctx->stream));
 	for(int lv = 0; lv < kRadixLevels; ++lv) { t->level_count[lv] = host_info[lv]; }
 	t->ngroups = host_info[3 * kRadixLevels];
@@ -2910,7 +3042,7 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	NBH_CHECK(t->ngroups >= 1 && static_cast<size_t>(t->ngroups) <= n, "bad group count %d", t->ngroups);
 	NBH_CHECK(t->nsupers >= 0 && static_cast<size_t>(t->nsupers) <= n / (kRadixSuper / 2) + 1,
 	          "bad super-group count %d", t->nsupers);
-	NBH_TRY(radix_moments(ctx, t, y, mass, order, ratio, true));
+	NBH_TRY(radix_moments(ctx, t, y, order, ratio, true));
 	t->radix = true;
 	t->built = true;
 	return NBH_OK;
@@ -2948,7 +3080,7 @@ nbh_status nbh_tree_destroy(nbh_tree* t)
 	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->wrec, t->body_n, t->counters,
 	                 t->two_lists, t->two_count, t->two_coef, t->far_list, t->far_count,
 	                 t->child, t->parent, t->level, t->level_list, t->level_info, t->gmark, t->smark, t->flags,
-	                 t->grp, t->sup, t->box };
+	                 t->grp, t->sup, t->box, t->pk };
 	for(void* p : ptrs)
 	{
 		if(p != nullptr) { (void)hipFree(p); }
@@ -3020,7 +3152,7 @@ nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* t, const double* y, double rat
 	nbh::scoped_timer timer(ctx, "tree_refit");
 	if(t->radix)
 	{
-		return radix_moments(ctx, t, y, nullptr, nullptr, ratio, false);
+		return radix_moments(ctx, t, y, nullptr, ratio, false);
 	}
 	return moments(ctx, t, y, nullptr, ratio, false);
 }
