@@ -945,8 +945,10 @@ k_radix_level_offsets(const int* __restrict__ count, int* __restrict__ offset, i
 	cursor[t] = 0;
 }
 
-// Internal ids listed level by level, one id per work-item: ranks within the
-// work-group from LDS counters, one global reservation per level present.
+// Internal ids above the walk groups listed level by level, kRadixListTile ids
+// per work-group: counts per level in LDS, one global reservation per level
+// present, then ranks from LDS counters.
+constexpr int kRadixListTile = 4096;
 __global__ void __launch_bounds__(256)
 k_radix_level_lists(const int* __restrict__ level, size_t n, const int* __restrict__ offset, int* __restrict__ cursor,
                     int* __restrict__ list)
@@ -955,20 +957,25 @@ k_radix_level_lists(const int* __restrict__ level, size_t n, const int* __restri
 	__shared__ int base[kRadixLevels];
 	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x) { cnt[b] = 0; }
 	__syncthreads();
-	const size_t id = 1 + static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-	int lv = -1, rank = 0;
-	if(id < n)
+	const size_t lo = 1 + static_cast<size_t>(blockIdx.x) * kRadixListTile;
+	const size_t hi = lo + kRadixListTile < n ? lo + kRadixListTile : n;
+	for(size_t id = lo + threadIdx.x; id < hi; id += blockDim.x)
 	{
-		lv = level[id];
-		if(lv >= 0) { rank = atomicAdd(&cnt[lv], 1); }
+		const int lv = level[id];
+		if(lv >= 0) { atomicAdd(&cnt[lv], 1); }
 	}
 	__syncthreads();
 	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x)
 	{
 		base[b] = cnt[b] != 0 ? offset[b] + atomicAdd(&cursor[b], cnt[b]) : 0;
+		cnt[b] = 0;
 	}
 	__syncthreads();
-	if(lv >= 0) { list[base[lv] + rank] = static_cast<int>(id); }
+	for(size_t id = lo + threadIdx.x; id < hi; id += blockDim.x)
+	{
+		const int lv = level[id];
+		if(lv >= 0) { list[base[lv] + atomicAdd(&cnt[lv], 1)] = static_cast<int>(id); }
+	}
 }
 
 // Walk record of node id from its moments (as k_pack_records).
@@ -980,50 +987,6 @@ __device__ inline void put_record(int id, const double4* __restrict__ xyzr, cons
 	r.m = nmass[id];
 	for(int k = 0; k < 6; ++k) { r.q[k] = static_cast<float>(quad[6 * static_cast<size_t>(id) + k]); }
 	wr[id] = r;
-}
-
-// Leaves of the Morton tree (as k_leaves) with their walk records.  kBuild: the
-// body at sorted position s is order[s], read from its packed record; refit:
-// the body numbers of the build, positions from y and the masses kept.
-template <bool kBuild>
-__global__ void __launch_bounds__(256)
-k_radix_leaves(const uint32_t* __restrict__ order, const double4* __restrict__ pk, const double* __restrict__ y,
-               size_t n, double4* __restrict__ xyzr, double* __restrict__ nmass, double* __restrict__ bmin,
-               double* __restrict__ bmax, int32_t* __restrict__ body_n, nbh_walk_rec* __restrict__ wr)
-{
-	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-	for(size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < n; s += stride)
-	{
-		const size_t idx = n + s;
-		double x, yy, z, m;
-		if constexpr(kBuild)
-		{
-			const uint32_t b = order[s];
-			const double4 p = pk[b];
-			x = p.x;
-			yy = p.y;
-			z = p.z;
-			m = p.w;
-			body_n[idx] = static_cast<int32_t>(b);
-			nmass[idx] = m;
-		}
-		else
-		{
-			const uint32_t b = static_cast<uint32_t>(body_n[idx]);
-			x = y[b];
-			yy = y[n + b];
-			z = y[2 * n + b];
-			m = nmass[idx];
-		}
-		xyzr[idx] = make_double4(x, yy, z, 0.0);
-		bmin[3 * idx] = x; bmin[3 * idx + 1] = yy; bmin[3 * idx + 2] = z;
-		bmax[3 * idx] = x; bmax[3 * idx + 1] = yy; bmax[3 * idx + 2] = z;
-		nbh_walk_rec r;
-		r.c = make_double4(x, yy, z, 0.0);
-		r.m = m;
-		for(int k = 0; k < 6; ++k) { r.q[k] = 0.0f; }
-		wr[idx] = r;
-	}
 }
 
 // Moments of the internal ids list[0, count) of one level, from their children
@@ -1045,23 +1008,80 @@ k_radix_level(const int* __restrict__ list, int count, const int2* __restrict__ 
 	}
 }
 
-// Moments of the internal nodes of each walk group (a subtree of at most
-// kRadixGroup leaves), one wave per group: the group's internal nodes listed
-// breadth-first in LDS, then taken depth by depth from the deepest.
+// Leaves and internal nodes of each walk group (a subtree of at most
+// kRadixGroup leaves), one wave per group, with their walk records.  The
+// group's leaves (as k_leaves; kBuild: the body at sorted position s is
+// order[s], read from its packed record; refit: the body numbers of the build,
+// positions from y and the masses kept) and node moments are held in LDS: leaf
+// first + j in slot j, internal id k in slot kRadixGroup + k - first - 1 (the
+// internal ids of a subtree over the sorted leaves [a, b] lie in [a + 1, b + 1]:
+// each internal node's index is one end of its key range).  The internal nodes
+// are listed breadth-first, then taken depth by depth from the deepest (the
+// heap's update_node and update_quad on the LDS slots), and written out.
+template <bool kBuild>
 __global__ void __launch_bounds__(kRadixGroup)
-k_radix_group_moments(const int4* __restrict__ grp, int ngroups, int n, const int2* __restrict__ child,
-                      double4* __restrict__ xyzr, double* __restrict__ nmass, double* __restrict__ bmin,
-                      double* __restrict__ bmax, double* __restrict__ quad, double ratio_sqr,
-                      nbh_walk_rec* __restrict__ wr)
+k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restrict__ child,
+                      const uint32_t* __restrict__ order, const double4* __restrict__ pk, const double* __restrict__ y,
+                      int32_t* __restrict__ body_n, double4* __restrict__ xyzr, double* __restrict__ nmass,
+                      double* __restrict__ bmin, double* __restrict__ bmax, double* __restrict__ quad,
+                      double ratio_sqr, nbh_walk_rec* __restrict__ wr)
 {
+	constexpr int kSlots = 2 * kRadixGroup;
+	__shared__ double4 s_c[kSlots];
+	__shared__ double s_m[kSlots];
+	__shared__ double s_lo[3 * kSlots];
+	__shared__ double s_hi[3 * kSlots];
+	__shared__ double s_q[6 * kSlots];
+	__shared__ int2 s_child[kRadixGroup];
 	__shared__ int q[kRadixGroup];          // internal nodes, breadth-first (at most kRadixGroup - 1)
 	__shared__ int start[kRadixGroup + 1];  // first entry of each depth
-	const int g = blockIdx.x;
-	if(g >= ngroups) { return; }
 	const int lane = threadIdx.x;
-	const int root = grp[g].x;
+	const int4 gr = grp[blockIdx.x];
+	const int root = gr.x, first = gr.y, cnt = gr.z;
+	if(lane < cnt)
+	{
+		const size_t sidx = static_cast<size_t>(first) + lane;
+		const size_t idx = static_cast<size_t>(n) + sidx;
+		double x, yy, z, m;
+		if constexpr(kBuild)
+		{
+			const uint32_t b = order[sidx];
+			const double4 p = pk[b];
+			x = p.x;
+			yy = p.y;
+			z = p.z;
+			m = p.w;
+			body_n[idx] = static_cast<int32_t>(b);
+			nmass[idx] = m;
+		}
+		else
+		{
+			const size_t b = static_cast<uint32_t>(body_n[idx]);
+			x = y[b];
+			yy = y[n + b];
+			z = y[2 * static_cast<size_t>(n) + b];
+			m = nmass[idx];
+		}
+		const double4 c = make_double4(x, yy, z, 0.0);
+		xyzr[idx] = c;
+		bmin[3 * idx] = x; bmin[3 * idx + 1] = yy; bmin[3 * idx + 2] = z;
+		bmax[3 * idx] = x; bmax[3 * idx + 1] = yy; bmax[3 * idx + 2] = z;
+		nbh_walk_rec r;
+		r.c = c;
+		r.m = m;
+		for(int k = 0; k < 6; ++k) { r.q[k] = 0.0f; }
+		wr[idx] = r;
+		s_c[lane] = c;
+		s_m[lane] = m;
+		s_lo[3 * lane] = x; s_lo[3 * lane + 1] = yy; s_lo[3 * lane + 2] = z;
+		s_hi[3 * lane] = x; s_hi[3 * lane + 1] = yy; s_hi[3 * lane + 2] = z;
+		for(int k = 0; k < 6; ++k) { s_q[6 * lane + k] = 0.0; }
+		const int id = first + 1 + lane;
+		s_child[lane] = id < n ? child[id] : make_int2(0, 0);
+	}
 	if(root >= n) { return; }              // a group of one leaf
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	auto slot = [&](int id) { return id >= n ? id - n - first : kRadixGroup + id - first - 1; };
 	if(lane == 0)
 	{
 		q[0] = root;
@@ -1071,9 +1091,9 @@ k_radix_group_moments(const int4* __restrict__ grp, int ngroups, int n, const in
 	__syncthreads();
 	while(head < tail)
 	{
-		const int cnt = tail - head;
-		const bool act = lane < cnt;
-		const int2 c = act ? child[q[head + lane]] : make_int2(n, n);
+		const int width = tail - head;
+		const bool act = lane < width;
+		const int2 c = act ? s_child[q[head + lane] - first - 1] : make_int2(n, n);
 		const bool li = c.x < n, ri = c.y < n;
 		const uint64_t bl = __ballot(li), br = __ballot(ri);
 		int pos = tail + __popcll(bl & lt) + __popcll(br & lt);
@@ -1092,10 +1112,27 @@ k_radix_group_moments(const int4* __restrict__ grp, int ngroups, int n, const in
 		if(i < start[d + 1])
 		{
 			const int id = q[i];
-			const int2 c = child[id];
-			update_node(id, c.x, c.y, xyzr, nmass, bmin, bmax, ratio_sqr);
-			update_quad(id, c.x, c.y, xyzr, nmass, quad);
-			put_record(id, xyzr, nmass, quad, wr);
+			const int2 c = s_child[id - first - 1];
+			const int k = slot(id), l = slot(c.x), r = slot(c.y);
+			update_node(k, l, r, s_c, s_m, s_lo, s_hi, ratio_sqr);
+			update_quad(k, l, r, s_c, s_m, s_q);
+			const size_t gid = static_cast<size_t>(id);
+			xyzr[gid] = s_c[k];
+			nmass[gid] = s_m[k];
+			nbh_walk_rec rec;
+			rec.c = s_c[k];
+			rec.m = s_m[k];
+			for(int e = 0; e < 3; ++e)
+			{
+				bmin[3 * gid + e] = s_lo[3 * k + e];
+				bmax[3 * gid + e] = s_hi[3 * k + e];
+			}
+			for(int e = 0; e < 6; ++e)
+			{
+				quad[6 * gid + e] = s_q[6 * k + e];
+				rec.q[e] = static_cast<float>(s_q[6 * k + e]);
+			}
+			wr[gid] = rec;
 		}
 		__syncthreads();
 	}
@@ -1359,6 +1396,7 @@ __device__ inline int xcd_group(int block, int nblocks)
 	return (block % kXcds) * (nblocks / kXcds) + block / kXcds;
 }
 
+
 // Half the diagonal of a box.
 __device__ inline double half_diagonal(double x0, double y0, double z0, double x1, double y1, double z1)
 {
@@ -1371,10 +1409,11 @@ __device__ inline double half_diagonal(double x0, double y0, double z0, double x
 // one node per lane (lane 2i takes the left node of entry i, lane 2i + 1 the
 // right one); every node the step opens pushes one entry, its two children.
 // Popped in this order, the stack never holds more than (levels + 2) * 32
-// entries.  In the Morton tree (kRadix) an entry is the id of the opened node,
-// whose children come from `child`, and 0 stands for the root alone.
+// entries.  In the Morton tree (kRadix) an entry encodes the children of the
+// opened node (radix_kids), so that popping reads no memory, and 0 stands for
+// the root alone.
 template <bool kRadix = false>
-__device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid, const int2* __restrict__ child = nullptr)
+__device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid, int n = 0)
 {
 	const int cnt = sp < kGroup / 2 ? sp : kGroup / 2;
 	const int i = lane >> 1;
@@ -1391,8 +1430,11 @@ __device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid,
 			}
 			else
 			{
-				const int2 c = child[e];
-				node = (lane & 1) ? c.y : c.x;
+				// g = split + 1; bit 30: the left child is leaf n + g - 1 (else
+				// internal id g), bit 31: the right child is leaf n + g (else g + 1)
+				const unsigned u = static_cast<unsigned>(e);
+				const int g = static_cast<int>(u & 0x3fffffffu);
+				node = (lane & 1) ? ((u >> 31) ? n + g : g + 1) : (((u >> 30) & 1u) ? n + g - 1 : g);
 				valid = true;
 			}
 		}
@@ -1406,9 +1448,19 @@ __device__ inline int walk_pop(const int* stack, int& sp, int lane, bool& valid,
 	return node;
 }
 
-// Stack entry of an opened node, and the entry the walk starts from.
+// Stack entry of an opened node (in the Morton tree: from its children kids,
+// which are leaf n + s / internal id s + 1 and leaf n + s + 1 / internal id s + 2
+// for the split s), and the entry the walk starts from.
 template <bool kRadix>
-__device__ inline int walk_entry(int node) { return kRadix ? node : 2 * node; }
+__device__ inline int walk_entry(int node, int2 kids = make_int2(0, 0), int n = 0)
+{
+	if constexpr(kRadix)
+	{
+		const unsigned g = static_cast<unsigned>(kids.x < n ? kids.x : kids.x - n + 1);
+		return static_cast<int>(g | (kids.x >= n ? 1u << 30 : 0u) | (kids.y >= n ? 1u << 31 : 0u));
+	}
+	return 2 * node;
+}
 template <bool kRadix>
 __device__ inline int walk_start() { return kRadix ? 0 : kRoot; }
 
@@ -1437,18 +1489,32 @@ __device__ inline float4 sweep_entry(const double4& rec, double m, double ccx, d
 
 // Pop the next chunk of the walk stack (walk_pop) and request its node records;
 // returns the number of stack entries taken.
-template <bool kRadix = false>
+// In the Morton tree the children of internal nodes are requested with the
+// records (kids), for the stack entries of the nodes the step opens; kQ: the
+// second moments too (q), from the same 64-byte record.
+template <bool kRadix = false, bool kQ = false>
 __device__ inline int walk_fetch(const int* stack, int& sp, int lane, const nbh_walk_rec* __restrict__ wr, int& node,
-                                 bool& valid, double4& rec, double& m, const int2* __restrict__ child = nullptr)
+                                 bool& valid, double4& rec, double& m, int2& kids, const int2* __restrict__ child = nullptr,
+                                 int n = 0, float* q = nullptr)
 {
 	const int cnt = sp < kGroup / 2 ? sp : kGroup / 2;
-	node = walk_pop<kRadix>(stack, sp, lane, valid, child);
+	node = walk_pop<kRadix>(stack, sp, lane, valid, n);
 	rec = make_double4(0.0, 0.0, 0.0, 0.0);
 	m = 0.0;
+	kids = make_int2(0, 0);
+	if constexpr(kQ)
+	{
+		for(int k = 0; k < 6; ++k) { q[k] = 0.0f; }
+	}
 	if(valid)
 	{
 		rec = wr[node].c;
 		m = wr[node].m;
+		if constexpr(kQ)
+		{
+			for(int k = 0; k < 6; ++k) { q[k] = wr[node].q[k]; }
+		}
+		if(kRadix && node < n) { kids = child[node]; }
 	}
 	return cnt;
 }
@@ -1732,6 +1798,8 @@ struct WalkTree {
 	const int2* child = nullptr;
 	const int4* grp = nullptr;
 	const int2* sup = nullptr;
+	int         ngroups = 0;
+	int         nsupers = 0;
 };
 
 struct TwoLevel {
@@ -1745,20 +1813,35 @@ struct TwoLevel {
 	int    far_cap = 0;
 };
 
-// Next 64 listed nodes from position pos, and their records.
+// Next 64 listed nodes from position pos and their records; `entry` holds this
+// lane's list entry, read one list step ahead, and takes that of the next step.
+template <bool kRadix = false, bool kQ = false>
 __device__ inline void list_fetch(const int* __restrict__ list, int pos, int count, int lane,
-                                  const nbh_walk_rec* __restrict__ wr, int& node, bool& valid, double4& rec, double& m)
+                                  const nbh_walk_rec* __restrict__ wr, int& entry, int& node, bool& valid,
+                                  double4& rec, double& m, int2& kids, const int2* __restrict__ child = nullptr,
+                                  int n = 0, float* q = nullptr)
 {
 	const int i = pos + lane;
 	valid = i < count;
-	node = valid ? list[i] : kRoot;
+	node = valid ? entry : kRoot;
 	rec = make_double4(0.0, 0.0, 0.0, 0.0);
 	m = 0.0;
+	kids = make_int2(0, 0);
+	if constexpr(kQ)
+	{
+		for(int k = 0; k < 6; ++k) { q[k] = 0.0f; }
+	}
 	if(valid)
 	{
 		rec = wr[node].c;
 		m = wr[node].m;
+		if constexpr(kQ)
+		{
+			for(int k = 0; k < 6; ++k) { q[k] = wr[node].q[k]; }
+		}
+		if(kRadix && node < n) { kids = child[node]; }
 	}
+	entry = i + kGroup < count ? list[i + kGroup] : kRoot;
 }
 
 // kQuad: internal cells taken as monopoles also contribute their quadrupole
@@ -1769,19 +1852,16 @@ __device__ inline void list_fetch(const int* __restrict__ list, int pos, int cou
 // kLocal (with kQuad): a far cell's monopole field is expanded too, to third
 // order (local_expansion), instead of being swept per body.  kTwo (with kLocal):
 // the group takes its super-group's list instead of walking (TwoLevel).
-template <bool kQuad, bool kLocal = false, bool kTwo = false, bool kRadix = false>
-__global__ void __launch_bounds__(kGroup)
-k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
-             const int32_t* __restrict__ body_n, const double* __restrict__ y, double* __restrict__ f, int n,
-             float near_k, unsigned long long* __restrict__ counters, TwoLevel two = TwoLevel(),
-             WalkTree tree = WalkTree())
+template <bool kQuad, bool kLocal, bool kTwo, bool kRadix>
+__device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* __restrict__ buf,
+                                      quad_lds<kQuad ? kQuadBuf : 1>& qbuf, int* __restrict__ farn,
+                                      const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
+                                      const double* __restrict__ bmax, const int32_t* __restrict__ body_n,
+                                      const double* __restrict__ y, double* __restrict__ f, int n, float near_k,
+                                      unsigned long long* __restrict__ counters, const TwoLevel& two,
+                                      const WalkTree& tree)
 {
-	__shared__ int stack[kWalkStack];
-	__shared__ float4 buf[kWalkBuf];
-	__shared__ quad_lds<kQuad ? kQuadBuf : 1> qbuf;
-	__shared__ int farn[kQuad ? kFarRing : 1];
 	const int lane = threadIdx.x;
-	const int gi = xcd_group(blockIdx.x, gridDim.x);    // group ordinal
 	// the group's node, first leaf, size and super-group
 	int gnode = n / kGroup + gi, gfirst = gi * kGroup, gcount = kGroup, gsuper = gi >> kSuperShift;
 	if constexpr(kRadix)
@@ -1823,6 +1903,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 		lcnt = use_list ? two.count[si] : 0;
 	}
 	const int* list = kTwo ? two.lists + static_cast<size_t>(si) * two.cap : nullptr;
+	int lentry = kTwo && lane < lcnt ? list[lane] : kRoot;   // list entry of the next list step
 	int sp = use_list ? 0 : 1;
 	int nbuf = 0;
 	int nq = 0;
@@ -1835,17 +1916,20 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 		int node;
 		double4 rec;
 		double m;
+		int2 kids;
+		float nq6[kQuad ? 6 : 1];     // second moments, read with the record
 		// issue this step's node loads, then sweep the buffers the previous steps
 		// filled while they are in flight; listed nodes are taken when the stack
 		// below them is done
 		if(sp == 0)
 		{
-			list_fetch(list, lpos, lcnt, lane, wr, node, valid, rec, m);
+			list_fetch<kRadix, kQuad>(list, lpos, lcnt, lane, wr, lentry, node, valid, rec, m, kids, tree.child, n,
+			                          nq6);
 			lpos += kGroup;
 		}
 		else
 		{
-			walk_fetch<kRadix>(stack, sp, lane, wr, node, valid, rec, m, tree.child);
+			walk_fetch<kRadix, kQuad>(stack, sp, lane, wr, node, valid, rec, m, kids, tree.child, n, nq6);
 		}
 		if(nbuf > kWalkBuf - kGroup)
 		{
@@ -1906,7 +1990,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			const uint64_t bnear = __ballot(nearc);
 			const uint64_t bfar = __ballot(far);
 			const uint64_t bmono = kLocal ? bacc & ~bnear & ~bfar : bacc & ~bnear;
-			if(nearc) { put_quad(qbuf, nq + __popcll(bnear & lt), ent, wr[node].q); }
+			if(nearc) { put_quad(qbuf, nq + __popcll(bnear & lt), ent, nq6); }
 			else if(accept && !(kLocal && far)) { buf[nbuf + __popcll(bmono & lt)] = ent; }
 			if(far)
 			{
@@ -1923,7 +2007,7 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 			if(accept) { buf[nbuf + __popcll(bacc & lt)] = ent; }
 			nbuf += __popcll(bacc);
 		}
-		if(open) { stack[sp + __popcll(bopen2 & lt)] = walk_entry<kRadix>(node); }
+		if(open) { stack[sp + __popcll(bopen2 & lt)] = walk_entry<kRadix>(node, kids, n); }
 		sp += __popcll(bopen2);
 		vis += valid ? 1 : 0;
 		taken += accept ? 1 : 0;
@@ -2015,6 +2099,27 @@ k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmi
 	}
 }
 
+// The group walk kernel, one work-group per group: the heap's groups in
+// XCD-contiguous order, the Morton tree's in launch order.  (Measured at 4M
+// Plummer bodies: persistent work-groups that take groups from counters, per
+// XCD or in one queue, and chunked XCD orders all ran the Morton walk 9-19%
+// slower than this.)
+template <bool kQuad, bool kLocal = false, bool kTwo = false, bool kRadix = false>
+__global__ void __launch_bounds__(kGroup)
+k_bh_group32(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin, const double* __restrict__ bmax,
+             const int32_t* __restrict__ body_n, const double* __restrict__ y, double* __restrict__ f, int n,
+             float near_k, unsigned long long* __restrict__ counters, TwoLevel two = TwoLevel(),
+             WalkTree tree = WalkTree())
+{
+	__shared__ int stack[kWalkStack];
+	__shared__ float4 buf[kWalkBuf];
+	__shared__ quad_lds<kQuad ? kQuadBuf : 1> qbuf;
+	__shared__ int farn[kQuad ? kFarRing : 1];
+	const int gi = kRadix ? static_cast<int>(blockIdx.x) : xcd_group(blockIdx.x, gridDim.x);
+	group_walk_one<kQuad, kLocal, kTwo, kRadix>(gi, stack, buf, qbuf, farn, wr, bmin, bmax, body_n, y, f, n, near_k,
+	                                             counters, two, tree);
+}
+
 // Super-group walk of the two-level local walk (TwoLevel): the group walk's
 // loop with the super box, far cells into the super expansion, every other
 // accepted node into the list.
@@ -2027,7 +2132,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 	__shared__ int stack[kWalkStack];
 	__shared__ int farn[kFarRing];
 	const int lane = threadIdx.x;
-	const int si = xcd_group(blockIdx.x, gridDim.x);
+	const int si = kRadix ? static_cast<int>(blockIdx.x) : xcd_group(blockIdx.x, gridDim.x);
 	// node of the super-group
 	const size_t sg = kRadix ? static_cast<size_t>(tree.sup[si].x) : (static_cast<size_t>(n) >> (6 + kSuperShift)) + si;
 	const double* glo = bmin + 3 * sg;
@@ -2054,7 +2159,8 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 		int node;
 		double4 rec;
 		double m;
-		walk_fetch<kRadix>(stack, sp, lane, wr, node, valid, rec, m, tree.child);
+		int2 kids;
+		walk_fetch<kRadix>(stack, sp, lane, wr, node, valid, rec, m, kids, tree.child, n);
 		if(ftail - fhead >= kGroup)
 		{
 			far_batch<true>(farn, wr, fhead, kGroup, lane, ccx, ccy, ccz, acc);
@@ -2110,7 +2216,7 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 			const int slot = lcount + __popcll(blist & lt);
 			if(slot < two.cap) { list[slot] = node; }
 		}
-		if(open) { stack[sp + __popcll(bopen & lt)] = walk_entry<kRadix>(node); }
+		if(open) { stack[sp + __popcll(bopen & lt)] = walk_entry<kRadix>(node, kids, n); }
 		ftail += __popcll(bfar);
 		lcount += __popcll(blist);
 		sp += __popcll(bopen);
@@ -2551,7 +2657,8 @@ k_bh_group32_mlp(const nbh_walk_rec* __restrict__ wr, const double* __restrict__
 		double4 rec;
 		double m;
 		// node loads first, then the pending sweeps (same order as k_bh_group32)
-		walk_fetch(stack, sp, lane, wr, node, valid, rec, m);
+		int2 kids;
+		walk_fetch(stack, sp, lane, wr, node, valid, rec, m, kids);
 		if(nbuf > kWalkBuf - kGroup)
 		{
 			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
@@ -2912,34 +3019,33 @@ static nbh_status radix_alloc(nbh_tree* t)
 }
 
 // Levels with fewer internal nodes than this run together in one work-group.
-constexpr int kRadixSmallLevel = 4096;
+constexpr int kRadixSmallLevel = 512;
 
 // Moments and walk records of the Morton tree (its structure, groups and level
-// lists already built): inside the walk groups one wave per group, then the
-// nodes above them level by level from the longest prefix up to the root.
-// build: leaves from `order` (sorted body numbers) and the packed bodies, else
-// from the body numbers of the last build and the positions y.
+// lists already built): the leaves and the nodes inside the walk groups one wave
+// per group, then the nodes above them level by level from the longest prefix
+// up to the root.  build: leaves from `order` (sorted body numbers) and the
+// packed bodies, else from the body numbers of the last build and the
+// positions y.
 static nbh_status radix_moments(nbh_ctx* ctx, nbh_tree* t, const double* y, const uint32_t* order, double ratio,
                                 bool build)
 {
 	const size_t n = t->n;
 	t->ratio = ratio;
 	const double ratio_sqr = ratio * ratio;
+	// the leaves and the nodes inside the walk groups, then the levels above them
 	if(build)
 	{
-		hipLaunchKernelGGL(k_radix_leaves<true>, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, order,
-		                   t->pk, y, n, t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, t->wrec);
+		hipLaunchKernelGGL(k_radix_group_moments<true>, dim3(static_cast<unsigned>(t->ngroups)), dim3(kRadixGroup), 0,
+		                   ctx->stream, t->grp, static_cast<int>(n), t->child, order, t->pk, y, t->body_n, t->xyzr,
+		                   t->mass, t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec);
 	}
 	else
 	{
-		hipLaunchKernelGGL(k_radix_leaves<false>, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, order,
-		                   t->pk, y, n, t->xyzr, t->mass, t->bmin, t->bmax, t->body_n, t->wrec);
+		hipLaunchKernelGGL(k_radix_group_moments<false>, dim3(static_cast<unsigned>(t->ngroups)), dim3(kRadixGroup),
+		                   0, ctx->stream, t->grp, static_cast<int>(n), t->child, order, t->pk, y, t->body_n,
+		                   t->xyzr, t->mass, t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec);
 	}
-	NBH_LAUNCHED();
-	// inside the walk groups, then the levels above them
-	hipLaunchKernelGGL(k_radix_group_moments, dim3(static_cast<unsigned>(t->ngroups)), dim3(kRadixGroup), 0,
-	                   ctx->stream, t->grp, t->ngroups, static_cast<int>(n), t->child, t->xyzr, t->mass, t->bmin,
-	                   t->bmax, t->quad, ratio_sqr, t->wrec);
 	NBH_LAUNCHED();
 	int offset[kRadixLevels];
 	int run = 0;
@@ -3026,9 +3132,9 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	hipLaunchKernelGGL(k_radix_level_offsets, dim3(1), dim3(kRadixLevels), 0, ctx->stream, t->level_info,
 	                   t->level_info + kRadixLevels, t->level_info + 2 * kRadixLevels);
 	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_radix_level_lists, dim3(static_cast<unsigned>((n - 1 + 255) / 256)), dim3(256), 0,
-	                   ctx->stream, t->level, n, t->level_info + kRadixLevels, t->level_info + 2 * kRadixLevels,
-	                   t->level_list);
+	hipLaunchKernelGGL(k_radix_level_lists, dim3(static_cast<unsigned>((n - 1 + kRadixListTile - 1) / kRadixListTile)),
+	                   dim3(256), 0, ctx->stream, t->level, n, t->level_info + kRadixLevels,
+	                   t->level_info + 2 * kRadixLevels, t->level_list);
 	NBH_LAUNCHED();
 	// the level counts and the group counts drive the launches that follow
 	constexpr size_t kInfoBytes = (3 * kRadixLevels + 2) * sizeof(int);
@@ -3202,6 +3308,8 @@ static WalkTree walk_tree(const nbh_tree* t)
 		wt.child = t->child;
 		wt.grp = t->grp;
 		wt.sup = t->sup;
+		wt.ngroups = t->ngroups;
+		wt.nsupers = t->nsupers;
 	}
 	return wt;
 }
@@ -3226,19 +3334,22 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 		const WalkTree wt = walk_tree(t);
 		if(mode == kGroupQuad)
 		{
-			hipLaunchKernelGGL((k_bh_group32<true, false, false, true>), dim3(static_cast<unsigned>(t->ngroups)), dim3(kGroup), 0,
+			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			hipLaunchKernelGGL((k_bh_group32<true, false, false, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters,
 			                   TwoLevel(), wt);
 		}
 		else if(mode == kGroupLocal)
 		{
-			hipLaunchKernelGGL((k_bh_group32<true, true, false, true>), dim3(static_cast<unsigned>(t->ngroups)), dim3(kGroup), 0,
+			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			hipLaunchKernelGGL((k_bh_group32<true, true, false, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters,
 			                   TwoLevel(), wt);
 		}
 		else
 		{
-			hipLaunchKernelGGL((k_bh_group32<false, false, false, true>), dim3(static_cast<unsigned>(t->ngroups)), dim3(kGroup), 0,
+			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			hipLaunchKernelGGL((k_bh_group32<false, false, false, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, 0.0f, t->counters,
 			                   TwoLevel(), wt);
 		}
@@ -3247,17 +3358,20 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 	}
 	if(mode == kGroupQuad)
 	{
-		hipLaunchKernelGGL(k_bh_group32<true>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin,
+		const unsigned grid = static_cast<unsigned>(n / kGroup);
+		hipLaunchKernelGGL((k_bh_group32<true>), dim3(grid), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin,
 		                   t->bmax, t->body_n, y, f, n, near_k, t->counters);
 	}
 	else if(mode == kGroupLocal)
 	{
-		hipLaunchKernelGGL((k_bh_group32<true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec,
+		const unsigned grid = static_cast<unsigned>(n / kGroup);
+		hipLaunchKernelGGL((k_bh_group32<true, true>), dim3(grid), dim3(kGroup), 0, ctx->stream, t->wrec,
 		                   t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters);
 	}
 	else
 	{
-		hipLaunchKernelGGL(k_bh_group32<false>, dim3(n / kGroup), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin,
+		const unsigned grid = static_cast<unsigned>(n / kGroup);
+		hipLaunchKernelGGL((k_bh_group32<false>), dim3(grid), dim3(kGroup), 0, ctx->stream, t->wrec, t->bmin,
 		                   t->bmax, t->body_n, y, f, n, 0.0f, t->counters);
 	}
 	NBH_LAUNCHED();
@@ -3477,14 +3591,15 @@ static nbh_status two_level(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const
 		nbh::scoped_timer group_timer(ctx, "bh_group_walk");
 		if(t->radix)
 		{
-			hipLaunchKernelGGL((k_bh_group32<true, true, true, true>), dim3(static_cast<unsigned>(t->ngroups)),
-			                   dim3(kGroup), 0,
+			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			hipLaunchKernelGGL((k_bh_group32<true, true, true, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two,
 			                   walk_tree(t));
 		}
 		else
 		{
-			hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(n / kGroup), dim3(kGroup), 0, ctx->stream,
+			const unsigned grid = static_cast<unsigned>(n / kGroup);
+			hipLaunchKernelGGL((k_bh_group32<true, true, true>), dim3(grid), dim3(kGroup), 0, ctx->stream,
 			                   t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two);
 		}
 		NBH_LAUNCHED();

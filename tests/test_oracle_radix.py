@@ -131,3 +131,40 @@ def test_radix_moments_by_brute_force(kind, n):
     # leaves: the bodies themselves
     assert np.array_equal(t["cm"][n:], pos) and np.array_equal(t["mass"][n:], mass)
     assert np.all(t["crit"][n:] == 0.0) and np.all(t["quad"][n:] == 0.0)
+
+
+@pytest.mark.parametrize("kind,n", [("normal", 300), ("clustered", 500), ("duplicates", 400)])
+def test_radix_subtree_ids_within_range(kind, n):
+    """The internal ids of the subtree of any node over the sorted leaves [a, b]
+    lie in [a + 1, b + 1] (each internal node's index is one end of its key
+    range), the property by which the GPU build holds a walk group's nodes in
+    LDS slots numbered from the group's first leaf."""
+    y, m = _state(kind, n, 3 * n)
+    t = oracle_lib.radix_build(y, m, 1.5)
+    first, last, visited = _ranges(t, n)
+    for k in visited:
+        assert k - 1 in (first[k], last[k])           # index k - 1 is one end of the range
+        stack = [k]
+        while stack:
+            j = stack.pop()
+            assert first[k] + 1 <= j <= last[k] + 1
+            stack += [int(c) for c in t["child"][j] if c < n]
+
+
+@pytest.mark.parametrize("kind,n", [("normal", 257), ("clustered", 500), ("duplicates", 300)])
+def test_radix_children_from_split(kind, n):
+    """The children of every internal node follow from its split s alone: leaf
+    n + s or internal id s + 1 on the left, leaf n + s + 1 or internal id s + 2 on
+    the right (the walks store a node's children as s + 1 and two leaf flags)."""
+    y, m = _state(kind, n, 5 * n)
+    t = oracle_lib.radix_build(y, m, 1.5)
+    for k in range(1, n):
+        l, r = (int(c) for c in t["child"][k])
+        g = l if l < n else l - n + 1                   # split + 1
+        assert l == (n + g - 1 if l >= n else g)
+        assert r == (n + g if r >= n else g + 1)
+        # decode of the stack entry the walks push
+        e = g | ((l >= n) << 30) | ((r >= n) << 31)
+        gd = e & 0x3FFFFFFF
+        assert ((n + gd - 1) if (e >> 30) & 1 else gd) == l
+        assert ((n + gd) if e >> 31 else gd + 1) == r
