@@ -897,7 +897,11 @@ def main(argv=None) -> int:
         out["two_plummer"] = bh_bench(1 << 20, 3, 1, [("monopole theta=0.5, group walk", 2.0, False),
                                                       ("quadrupole theta=0.6", 1.0 / 0.6, "quadrupole"),
                                                       ("quadrupole theta=0.6, local far field", 1.0 / 0.6,
-                                                       "quadrupole", {"bh_far_field": "local"})],
+                                                       "quadrupole", {"bh_far_field": "local"}),
+                                                      ("morton tree: quadrupole theta=0.58, local far field, near 3",
+                                                       1.0 / 0.58, "quadrupole",
+                                                       {"bh_far_field": "local", "bh_quad_near": 3.0,
+                                                        "bh_tree": "morton"})],
                                       kind="two_plummer", seed=3, rk4_steps=3)
     if not a.no_vector_ops:
         log("vector ops")

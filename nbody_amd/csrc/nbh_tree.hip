@@ -93,15 +93,18 @@ struct nbh_tree {
 	int*      parent = nullptr;    // [2n] parent id of every non-root node
 	int*      level = nullptr;     // [n] level of internal ids: the common prefix length of their keys
 	int*      level_list = nullptr;   // [n] internal ids level by level
-	int*      level_info = nullptr;   // [3 * kRadixLevels + 2] level counts, offsets, cursors; groups, super-groups
+	int*      level_info = nullptr;   // [3 * kRadixLevels + 3] level counts, offsets, cursors; the group counts
 	int2*     gmark = nullptr;     // [n] group marks at their first sorted leaf (build scratch)
 	int2*     smark = nullptr;     // [n] super-group marks
-	uint32_t* flags = nullptr;     // [2n] group and super-group flags, scanned in place
-	int4*     grp = nullptr;       // [n] walk groups (node, first leaf, leaves, super-group)
+	int2*     wmark = nullptr;     // [n] walk group marks
+	uint32_t* flags = nullptr;     // [3n] build group, super-group and walk group flags, scanned in place
+	int4*     grp = nullptr;       // [n] build groups (node, first leaf, leaves)
+	int4*     wgrp = nullptr;      // [n] walk groups (node, first leaf, leaves, super-group)
 	int2*     sup = nullptr;       // [n / 512 + 1] super-groups (node, first leaf)
 	double*   box = nullptr;       // [6 * (kBoxParts + 1)] partial boxes, then the bounding box
 	double4*  pk = nullptr;        // [n] position and mass of each body, packed for the sorted gather
-	int       ngroups = 0;         // walk groups and super-groups of the last Morton build
+	int       ngroups = 0;         // build groups, walk groups and super-groups of the last Morton build
+	int       nwalk = 0;
 	int       nsupers = 0;
 	int       level_count[kRadixLevels] = {};   // host copy of the level counts (refit)
 	int       two_slots = 0;       // super-groups the two-level buffers hold
@@ -772,13 +775,15 @@ __device__ inline int radix_delta(const uint64_t* __restrict__ k, long long n, l
 	return __clzll(a ^ b);
 }
 
-// Walk groups and super-groups of the Morton tree: the maximal subtrees of at
-// most kRadixGroup leaves, and those of at most kRadixSuper leaves that hold at
-// least half as many (smaller ones leave their groups to walk in one level).
-// Their leaves are consecutive in Morton order and their boxes are node boxes,
-// so they never straddle the core of the state the way a fixed run of the
-// Morton order can.
+// Groups of the Morton tree, all maximal subtrees: build groups of at most
+// kRadixGroup leaves (one wave computes their nodes' moments), walk groups of at
+// most kRadixWalk leaves (one wave walks for them, two bodies per lane), and
+// super-groups of at most kRadixSuper leaves that hold at least half as many
+// (smaller ones leave their walk groups to walk in one level).  Their leaves are
+// consecutive in Morton order and their boxes are node boxes, so they never
+// straddle the core of the state the way a fixed run of the Morton order can.
 constexpr int kRadixGroup = 64;
+constexpr int kRadixWalk = 128;
 constexpr int kRadixSuper = 1024;
 
 // Internal node i of [0, n - 1): its key range, split and children (ids), the
@@ -790,7 +795,7 @@ constexpr int kRadixSuper = 1024;
 __global__ void __launch_bounds__(256)
 k_radix_nodes(const uint64_t* __restrict__ k, long long n, int2* __restrict__ child, int* __restrict__ parent,
               int* __restrict__ level, int* __restrict__ level_count, int2* __restrict__ gmark,
-              int2* __restrict__ smark)
+              int2* __restrict__ smark, int2* __restrict__ wmark)
 {
 	__shared__ int hist[kRadixLevels];
 	for(int b = threadIdx.x; b < kRadixLevels; b += blockDim.x) { hist[b] = 0; }
@@ -834,6 +839,12 @@ k_radix_nodes(const uint64_t* __restrict__ k, long long n, int2* __restrict__ ch
 			if(cl <= kRadixGroup) { gmark[lo] = make_int2(left, static_cast<int>(cl)); }
 			if(cr <= kRadixGroup) { gmark[gamma + 1] = make_int2(right, static_cast<int>(cr)); }
 		}
+		if(id == kRoot && cnt <= kRadixWalk) { wmark[lo] = make_int2(id, static_cast<int>(cnt)); }
+		if(cnt > kRadixWalk)
+		{
+			if(cl <= kRadixWalk) { wmark[lo] = make_int2(left, static_cast<int>(cl)); }
+			if(cr <= kRadixWalk) { wmark[gamma + 1] = make_int2(right, static_cast<int>(cr)); }
+		}
 		if(cnt > kRadixSuper)
 		{
 			if(cl <= kRadixSuper && 2 * cl >= kRadixSuper) { smark[lo] = make_int2(left, static_cast<int>(cl)); }
@@ -854,27 +865,30 @@ k_fill_int(int* __restrict__ a, size_t count, int v)
 	for(size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) { a[i] = v; }
 }
 
-// 1 where a mark starts a group / super-group, for the scans that number them.
+// 1 where a mark starts a group / super-group / walk group, for the scans that
+// number them.
 __global__ void __launch_bounds__(256)
-k_radix_flags(const int2* __restrict__ gmark, const int2* __restrict__ smark, size_t n, uint32_t* __restrict__ gflag,
-              uint32_t* __restrict__ sflag)
+k_radix_flags(const int2* __restrict__ gmark, const int2* __restrict__ smark, const int2* __restrict__ wmark,
+              size_t n, uint32_t* __restrict__ gflag, uint32_t* __restrict__ sflag, uint32_t* __restrict__ wflag)
 {
 	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
 	for(size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
 	{
 		gflag[p] = gmark[p].x != 0 ? 1u : 0u;
 		sflag[p] = smark[p].x != 0 ? 1u : 0u;
+		wflag[p] = wmark[p].x != 0 ? 1u : 0u;
 	}
 }
 
-// Groups grp[g] = (node id, first sorted leaf, leaves, super-group or -1) and
-// super-groups sup[s] = (node id, first sorted leaf), both in Morton order
-// (gidx / sidx: exclusive scans of the flags); info[0] = groups, info[1] =
-// super-groups.
+// Build groups grp[g] and walk groups wgrp[w] = (node id, first sorted leaf,
+// leaves, super-group or -1), super-groups sup[s] = (node id, first sorted
+// leaf), all in Morton order (gidx / sidx / widx: exclusive scans of the flags);
+// info[0] = build groups, info[1] = super-groups, info[2] = walk groups.
 __global__ void __launch_bounds__(256)
-k_radix_compact(const int2* __restrict__ gmark, const int2* __restrict__ smark, const uint32_t* __restrict__ gidx,
-                const uint32_t* __restrict__ sidx, size_t n, int4* __restrict__ grp, int2* __restrict__ sup,
-                int* __restrict__ info)
+k_radix_compact(const int2* __restrict__ gmark, const int2* __restrict__ smark, const int2* __restrict__ wmark,
+                const uint32_t* __restrict__ gidx, const uint32_t* __restrict__ sidx,
+                const uint32_t* __restrict__ widx, size_t n, int4* __restrict__ grp, int2* __restrict__ sup,
+                int4* __restrict__ wgrp, int* __restrict__ info)
 {
 	const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
 	for(size_t p = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
@@ -883,22 +897,25 @@ k_radix_compact(const int2* __restrict__ gmark, const int2* __restrict__ smark, 
 		if(g.x != 0) { grp[gidx[p]] = make_int4(g.x, static_cast<int>(p), g.y, -1); }
 		const int2 sm = smark[p];
 		if(sm.x != 0) { sup[sidx[p]] = make_int2(sm.x, static_cast<int>(p)); }
+		const int2 w = wmark[p];
+		if(w.x != 0) { wgrp[widx[p]] = make_int4(w.x, static_cast<int>(p), w.y, -1); }
 		if(p == n - 1)
 		{
 			info[0] = static_cast<int>(gidx[p] + (g.x != 0 ? 1u : 0u));
 			info[1] = static_cast<int>(sidx[p] + (sm.x != 0 ? 1u : 0u));
+			info[2] = static_cast<int>(widx[p] + (w.x != 0 ? 1u : 0u));
 		}
 	}
 }
 
-// The super-group of each group: the last super-group starting at or before the
-// group's first leaf, if its node's leaves reach the group (a group lies inside
-// at most one super-group: both are subtrees).
+// The super-group of each walk group: the last super-group starting at or
+// before the group's first leaf, if its node's leaves reach the group (a walk
+// group lies inside at most one super-group: both are subtrees).
 __global__ void __launch_bounds__(256)
 k_radix_group_super(int4* __restrict__ grp, const int2* __restrict__ sup, const int* __restrict__ info,
                     const int2* __restrict__ child, size_t n)
 {
-	const int ng = info[0], ns = info[1];
+	const int ng = info[2], ns = info[1];
 	for(int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x)
 	{
 		const int first = grp[g].y;
@@ -1877,15 +1894,26 @@ __device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* _
 	const double gx0 = glo[0], gy0 = glo[1], gz0 = glo[2];
 	const double gx1 = ghi[0], gy1 = ghi[1], gz1 = ghi[2];
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
-	// this lane's leaf; lanes beyond a smaller group's size repeat its first body
-	// and write nothing
-	const bool body_lane = lane < gcount;
-	const int tn = n + gfirst + (body_lane ? lane : 0);
-	const double4 me = wr[tn].c;
-	const float tx = static_cast<float>((me.x - ccx) * kSweepLen);   // scaled, for the sweeps
-	const float ty = static_cast<float>((me.y - ccy) * kSweepLen);
-	const float tz = static_cast<float>((me.z - ccz) * kSweepLen);
-	float ax = 0.0f, ay = 0.0f, az = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+	// this lane's bodies: leaf lane of the group, and leaf lane + 64 in the Morton
+	// tree's walk groups of up to 128 bodies (kT = 2); lanes beyond a smaller
+	// group's size repeat its first body and write nothing
+	constexpr int kT = kRadix ? 2 : 1;
+	bool body_lane[kT];
+	int tn[kT];
+	float tx[kT], ty[kT], tz[kT];                   // scaled offsets from the group centre, for the sweeps
+	float ax[kT], ay[kT], az[kT], cx[kT], cy[kT], cz[kT];
+#pragma unroll
+	for(int t = 0; t < kT; ++t)
+	{
+		const int j = lane + kGroup * t;
+		body_lane[t] = j < gcount;
+		tn[t] = n + gfirst + (body_lane[t] ? j : 0);
+		const double4 me = wr[tn[t]].c;
+		tx[t] = static_cast<float>((me.x - ccx) * kSweepLen);
+		ty[t] = static_cast<float>((me.y - ccy) * kSweepLen);
+		tz[t] = static_cast<float>((me.z - ccz) * kSweepLen);
+		ax[t] = ay[t] = az[t] = cx[t] = cy[t] = cz[t] = 0.0f;
+	}
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	unsigned vis = 0, taken = 0, spill = 0, nfar = 0, nnear = 0;
 	constexpr int kAcc = kLocal ? 34 : 9;
@@ -1933,14 +1961,14 @@ __device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* _
 		}
 		if(nbuf > kWalkBuf - kGroup)
 		{
-			sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
+			for(int t = 0; t < kT; ++t) { sweep32(buf, nbuf, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); }
 			nbuf = 0;
 		}
 		if constexpr(kQuad)
 		{
 			if(nq > kQuadBuf - kGroup)
 			{
-				sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
+				for(int t = 0; t < kT; ++t) { sweep_mq(qbuf, nq, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); }
 				nq = 0;
 			}
 			if(ftail - fhead >= kGroup)
@@ -2015,12 +2043,13 @@ __device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* _
 	}
 	if(nbuf > 0)
 	{
-		sweep32(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
+		for(int t = 0; t < kT; ++t) { sweep32(buf, nbuf, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); }
 	}
-	float kx = 0.0f, ky = 0.0f, kz = 0.0f;
+	float kx[kT], ky[kT], kz[kT];
+	for(int t = 0; t < kT; ++t) { kx[t] = ky[t] = kz[t] = 0.0f; }
 	if constexpr(kQuad)
 	{
-		if(nq > 0) { sweep_mq(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz); }
+		if(nq > 0) { for(int t = 0; t < kT; ++t) { sweep_mq(qbuf, nq, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); } }
 		for(; fhead < ftail; fhead += kGroup)
 		{
 			far_batch<kLocal>(farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, ccx, ccy, ccz,
@@ -2030,19 +2059,24 @@ __device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* _
 		{
 			for(int k = 0; k < kAcc; ++k) { acc[k] += __shfl_xor(acc[k], sh, 64); }
 		}
-		const float ux = static_cast<float>(me.x - ccx);   // unscaled offset for the expansion
-		const float uy = static_cast<float>(me.y - ccy);
-		const float uz = static_cast<float>(me.z - ccz);
-		kx = acc[0] + acc[3] * ux + acc[6] * uy + acc[7] * uz;
-		ky = acc[1] + acc[6] * ux + acc[4] * uy + acc[8] * uz;
-		kz = acc[2] + acc[7] * ux + acc[8] * uy + acc[5] * uz;
-		if constexpr(kLocal)
+#pragma unroll
+		for(int t = 0; t < kT; ++t)
 		{
-			float sx, sy, sz;
-			local_terms(acc + 9, ux, uy, uz, sx, sy, sz);
-			kx += sx;
-			ky += sy;
-			kz += sz;
+			const double4 me = wr[tn[t]].c;
+			const float ux = static_cast<float>(me.x - ccx);   // unscaled offset for the expansion
+			const float uy = static_cast<float>(me.y - ccy);
+			const float uz = static_cast<float>(me.z - ccz);
+			kx[t] = acc[0] + acc[3] * ux + acc[6] * uy + acc[7] * uz;
+			ky[t] = acc[1] + acc[6] * ux + acc[4] * uy + acc[8] * uz;
+			kz[t] = acc[2] + acc[7] * ux + acc[8] * uy + acc[5] * uz;
+			if constexpr(kLocal)
+			{
+				float sx, sy, sz;
+				local_terms(acc + 9, ux, uy, uz, sx, sy, sz);
+				kx[t] += sx;
+				ky[t] += sy;
+				kz[t] += sz;
+			}
 		}
 		if constexpr(kTwo)
 		{
@@ -2059,25 +2093,34 @@ __device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* _
 				const float* c = two.coef + 34 * static_cast<size_t>(si);
 				float e[34];
 				for(int k = 0; k < 34; ++k) { e[k] = c[k]; }
-				const float vx = static_cast<float>(me.x - scx), vy = static_cast<float>(me.y - scy);
-				const float vz = static_cast<float>(me.z - scz);
-				float sx, sy, sz;
-				local_terms(e + 9, vx, vy, vz, sx, sy, sz);
-				kx += e[0] + e[3] * vx + e[6] * vy + e[7] * vz + sx;
-				ky += e[1] + e[6] * vx + e[4] * vy + e[8] * vz + sy;
-				kz += e[2] + e[7] * vx + e[8] * vy + e[5] * vz + sz;
+#pragma unroll
+				for(int t = 0; t < kT; ++t)
+				{
+					const double4 me = wr[tn[t]].c;
+					const float vx = static_cast<float>(me.x - scx), vy = static_cast<float>(me.y - scy);
+					const float vz = static_cast<float>(me.z - scz);
+					float sx, sy, sz;
+					local_terms(e + 9, vx, vy, vz, sx, sy, sz);
+					kx[t] += e[0] + e[3] * vx + e[6] * vy + e[7] * vz + sx;
+					ky[t] += e[1] + e[6] * vx + e[4] * vy + e[8] * vz + sy;
+					kz[t] += e[2] + e[7] * vx + e[8] * vy + e[5] * vz + sz;
+				}
 			}
 		}
 	}
-	if(body_lane)
+#pragma unroll
+	for(int t = 0; t < kT; ++t)
 	{
-		const int b = body_n[tn];
-		f[b] = y[3 * n + b];
-		f[n + b] = y[4 * n + b];
-		f[2 * n + b] = y[5 * n + b];
-		f[3 * n + b] = static_cast<double>(ax) - static_cast<double>(cx) + static_cast<double>(kx);
-		f[4 * n + b] = static_cast<double>(ay) - static_cast<double>(cy) + static_cast<double>(ky);
-		f[5 * n + b] = static_cast<double>(az) - static_cast<double>(cz) + static_cast<double>(kz);
+		if(body_lane[t])
+		{
+			const int b = body_n[tn[t]];
+			f[b] = y[3 * n + b];
+			f[n + b] = y[4 * n + b];
+			f[2 * n + b] = y[5 * n + b];
+			f[3 * n + b] = static_cast<double>(ax[t]) - static_cast<double>(cx[t]) + static_cast<double>(kx[t]);
+			f[4 * n + b] = static_cast<double>(ay[t]) - static_cast<double>(cy[t]) + static_cast<double>(ky[t]);
+			f[5 * n + b] = static_cast<double>(az[t]) - static_cast<double>(cz[t]) + static_cast<double>(kz[t]);
+		}
 	}
 	// counters: [0] pair interactions (entries x the group's bodies), [1] node
 	// visits, [2] spills
@@ -2983,13 +3026,13 @@ static nbh_status radix_alloc(nbh_tree* t)
 		return NBH_OK;
 	}
 	const size_t n = t->n;
-	constexpr int kBufs = 12;
+	constexpr int kBufs = 14;
 	void* p[kBufs] = {};
 	const size_t sizes[kBufs] = { n * sizeof(int2), 2 * n * sizeof(int), n * sizeof(int), n * sizeof(int),
-	                              (3 * kRadixLevels + 2) * sizeof(int), n * sizeof(int2), n * sizeof(int2),
-	                              2 * n * sizeof(uint32_t), n * sizeof(int4),
+	                              (3 * kRadixLevels + 3) * sizeof(int), n * sizeof(int2), n * sizeof(int2),
+	                              3 * n * sizeof(uint32_t), n * sizeof(int4),
 	                              (n / (kRadixSuper / 2) + 1) * sizeof(int2), 6 * (kBoxParts + 1) * sizeof(double),
-	                              n * sizeof(double4) };
+	                              n * sizeof(double4), n * sizeof(int2), n * sizeof(int4) };
 	hipError_t e = hipSuccess;
 	for(int k = 0; k < kBufs && e == hipSuccess; ++k)
 	{
@@ -3015,6 +3058,8 @@ static nbh_status radix_alloc(nbh_tree* t)
 	t->sup = static_cast<int2*>(p[9]);
 	t->box = static_cast<double*>(p[10]);
 	t->pk = static_cast<double4*>(p[11]);
+	t->wmark = static_cast<int2*>(p[12]);
+	t->wgrp = static_cast<int4*>(p[13]);
 	return NBH_OK;
 }
 
@@ -3109,24 +3154,28 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	NBH_LAUNCHED();
 	hipLaunchKernelGGL(k_fill_int, dim3(gn), dim3(256), 0, ctx->stream, reinterpret_cast<int*>(t->smark), 2 * n, 0);
 	NBH_LAUNCHED();
+	hipLaunchKernelGGL(k_fill_int, dim3(gn), dim3(256), 0, ctx->stream, reinterpret_cast<int*>(t->wmark), 2 * n, 0);
+	NBH_LAUNCHED();
 	hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, ctx->stream, t->level_info,
-	                   static_cast<size_t>(3 * kRadixLevels + 2), 0);
+	                   static_cast<size_t>(3 * kRadixLevels + 3), 0);
 	NBH_LAUNCHED();
 	hipLaunchKernelGGL(k_radix_nodes, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->keys,
-	                   static_cast<long long>(n), t->child, t->parent, t->level, t->level_info, t->gmark, t->smark);
+	                   static_cast<long long>(n), t->child, t->parent, t->level, t->level_info, t->gmark, t->smark,
+	                   t->wmark);
 	NBH_LAUNCHED();
 	// groups and super-groups numbered in Morton order
 	hipLaunchKernelGGL(k_radix_flags, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->gmark,
-	                   t->smark, n, t->flags, t->flags + n);
+	                   t->smark, t->wmark, n, t->flags, t->flags + n, t->flags + 2 * n);
 	NBH_LAUNCHED();
 	NBH_TRY(scan_exclusive(ctx, t->flags, n, t->scan_part));
 	NBH_TRY(scan_exclusive(ctx, t->flags + n, n, t->scan_part));
+	NBH_TRY(scan_exclusive(ctx, t->flags + 2 * n, n, t->scan_part));
 	int* info = t->level_info + 3 * kRadixLevels;
 	hipLaunchKernelGGL(k_radix_compact, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, t->gmark,
-	                   t->smark, t->flags, t->flags + n, n, t->grp, t->sup, info);
+	                   t->smark, t->wmark, t->flags, t->flags + n, t->flags + 2 * n, n, t->grp, t->sup, t->wgrp, info);
 	NBH_LAUNCHED();
 	hipLaunchKernelGGL(k_radix_group_super, dim3(nbh::grid_for(ctx, n / 16 + 1, 256)), dim3(256), 0, ctx->stream,
-	                   t->grp, t->sup, info, t->child, n);
+	                   t->wgrp, t->sup, info, t->child, n);
 	NBH_LAUNCHED();
 	// internal ids listed by level
 	hipLaunchKernelGGL(k_radix_level_offsets, dim3(1), dim3(kRadixLevels), 0, ctx->stream, t->level_info,
@@ -3137,7 +3186,7 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	                   t->level_info + 2 * kRadixLevels, t->level_list);
 	NBH_LAUNCHED();
 	// the level counts and the group counts drive the launches that follow
-	constexpr size_t kInfoBytes = (3 * kRadixLevels + 2) * sizeof(int);
+	constexpr size_t kInfoBytes = (3 * kRadixLevels + 3) * sizeof(int);
 	NBH_TRY(nbh::pinned(ctx, kInfoBytes));
 	const int* host_info = reinterpret_cast<const int*>(ctx->pinned);
 	NBH_HIP(hipMemcpyAsync(ctx->pinned, t->level_info, kInfoBytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -3145,7 +3194,9 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	for(int lv = 0; lv < kRadixLevels; ++lv) { t->level_count[lv] = host_info[lv]; }
 	t->ngroups = host_info[3 * kRadixLevels];
 	t->nsupers = host_info[3 * kRadixLevels + 1];
+	t->nwalk = host_info[3 * kRadixLevels + 2];
 	NBH_CHECK(t->ngroups >= 1 && static_cast<size_t>(t->ngroups) <= n, "bad group count %d", t->ngroups);
+	NBH_CHECK(t->nwalk >= 1 && t->nwalk <= t->ngroups, "bad walk group count %d", t->nwalk);
 	NBH_CHECK(t->nsupers >= 0 && static_cast<size_t>(t->nsupers) <= n / (kRadixSuper / 2) + 1,
 	          "bad super-group count %d", t->nsupers);
 	NBH_TRY(radix_moments(ctx, t, y, order, ratio, true));
@@ -3186,7 +3237,7 @@ nbh_status nbh_tree_destroy(nbh_tree* t)
 	                 t->xyzr, t->mass, t->bmin, t->bmax, t->quad, t->wrec, t->body_n, t->counters,
 	                 t->two_lists, t->two_count, t->two_coef, t->far_list, t->far_count,
 	                 t->child, t->parent, t->level, t->level_list, t->level_info, t->gmark, t->smark, t->flags,
-	                 t->grp, t->sup, t->box, t->pk };
+	                 t->grp, t->sup, t->box, t->pk, t->wmark, t->wgrp };
 	for(void* p : ptrs)
 	{
 		if(p != nullptr) { (void)hipFree(p); }
@@ -3306,9 +3357,9 @@ static WalkTree walk_tree(const nbh_tree* t)
 	if(t->radix)
 	{
 		wt.child = t->child;
-		wt.grp = t->grp;
+		wt.grp = t->wgrp;
 		wt.sup = t->sup;
-		wt.ngroups = t->ngroups;
+		wt.ngroups = t->nwalk;
 		wt.nsupers = t->nsupers;
 	}
 	return wt;
@@ -3334,21 +3385,21 @@ static nbh_status group_walk(nbh_ctx* ctx, const nbh_tree* t, const double* y, d
 		const WalkTree wt = walk_tree(t);
 		if(mode == kGroupQuad)
 		{
-			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			const unsigned grid = static_cast<unsigned>(t->nwalk);
 			hipLaunchKernelGGL((k_bh_group32<true, false, false, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters,
 			                   TwoLevel(), wt);
 		}
 		else if(mode == kGroupLocal)
 		{
-			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			const unsigned grid = static_cast<unsigned>(t->nwalk);
 			hipLaunchKernelGGL((k_bh_group32<true, true, false, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters,
 			                   TwoLevel(), wt);
 		}
 		else
 		{
-			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			const unsigned grid = static_cast<unsigned>(t->nwalk);
 			hipLaunchKernelGGL((k_bh_group32<false, false, false, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, 0.0f, t->counters,
 			                   TwoLevel(), wt);
@@ -3591,7 +3642,7 @@ static nbh_status two_level(nbh_ctx* ctx, nbh_tree* t, const nbh_mlp* net, const
 		nbh::scoped_timer group_timer(ctx, "bh_group_walk");
 		if(t->radix)
 		{
-			const unsigned grid = static_cast<unsigned>(t->ngroups);
+			const unsigned grid = static_cast<unsigned>(t->nwalk);
 			hipLaunchKernelGGL((k_bh_group32<true, true, true, true>), dim3(grid), dim3(kGroup), 0,
 			                   ctx->stream, t->wrec, t->bmin, t->bmax, t->body_n, y, f, n, near_k, t->counters, two,
 			                   walk_tree(t));
