@@ -100,7 +100,7 @@ struct nbh_tree {
 	uint32_t* flags = nullptr;     // [3n] build group, super-group and walk group flags, scanned in place
 	int4*     grp = nullptr;       // [n] build groups (node, first leaf, leaves)
 	int4*     wgrp = nullptr;      // [n] walk groups (node, first leaf, leaves, super-group)
-	int2*     sup = nullptr;       // [n / 512 + 1] super-groups (node, first leaf)
+	int2*     sup = nullptr;       // [n / (kRadixSuper / 2) + 1] super-groups (node, first leaf)
 	double*   box = nullptr;       // [6 * (kBoxParts + 1)] partial boxes, then the bounding box
 	double4*  pk = nullptr;        // [n] position and mass of each body, packed for the sorted gather
 	int       ngroups = 0;         // build groups, walk groups and super-groups of the last Morton build
@@ -578,56 +578,75 @@ k_leaves(const uint32_t* __restrict__ order, const double* __restrict__ y, const
 __device__ inline double min_ref(double a, double b) { return (b < a) ? b : a; }   // std::min
 __device__ inline double max_ref(double a, double b) { return (a < b) ? b : a; }   // std::max
 
-// Node idx from its children l and r (2 idx and 2 idx + 1 in the heap).
-__device__ inline void update_node(size_t idx, size_t l, size_t r, double4* __restrict__ xyzr,
-                                   double* __restrict__ nmass, double* __restrict__ bmin, double* __restrict__ bmax,
-                                   double ratio_sqr)
+// A node's moments from its children's (mass centre cl / cr, masses ml / mr,
+// boxes lol..hil / lor..hir): mass, mass centre, box and crit_r2 = (radius *
+// ratio)^2 with the heap's operation order.
+__device__ inline void combine_node(const double4& cl, double ml, const double* lol, const double* hil,
+                                    const double4& cr, double mr, const double* lor, const double* hir,
+                                    double ratio_sqr, double4& c, double& m, double* lo, double* hi)
 {
-	double ml = nmass[l], mr = nmass[r];
-	double m = ml + mr;
-	double4 cl = xyzr[l], cr = xyzr[r];
+	m = ml + mr;
 	double cx = (cl.x * ml + cr.x * mr) / m;
 	double cy = (cl.y * ml + cr.y * mr) / m;
 	double cz = (cl.z * ml + cr.z * mr) / m;
-	double lo[3], hi[3];
 	for(int d = 0; d < 3; ++d)
 	{
-		lo[d] = min_ref(bmin[3 * l + d], bmin[3 * r + d]);
-		hi[d] = max_ref(bmax[3 * l + d], bmax[3 * r + d]);
-		bmin[3 * idx + d] = lo[d];
-		bmax[3 * idx + d] = hi[d];
+		lo[d] = min_ref(lol[d], lor[d]);
+		hi[d] = max_ref(hil[d], hir[d]);
 	}
 	double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
 	double gx = (hi[0] + lo[0]) / 2.0 - cx;
 	double gy = (hi[1] + lo[1]) / 2.0 - cy;
 	double gz = (hi[2] + lo[2]) / 2.0 - cz;
 	double rad = sqrt(ex * ex + ey * ey + ez * ez) * 0.5 + sqrt(gx * gx + gy * gy + gz * gz);
+	c = make_double4(cx, cy, cz, (rad * rad) * ratio_sqr);
+}
+
+// Node idx from its children l and r (2 idx and 2 idx + 1 in the heap).
+__device__ inline void update_node(size_t idx, size_t l, size_t r, double4* __restrict__ xyzr,
+                                   double* __restrict__ nmass, double* __restrict__ bmin, double* __restrict__ bmax,
+                                   double ratio_sqr)
+{
+	double4 c;
+	double m, lo[3], hi[3];
+	combine_node(xyzr[l], nmass[l], bmin + 3 * l, bmax + 3 * l, xyzr[r], nmass[r], bmin + 3 * r, bmax + 3 * r,
+	             ratio_sqr, c, m, lo, hi);
+	for(int d = 0; d < 3; ++d)
+	{
+		bmin[3 * idx + d] = lo[d];
+		bmax[3 * idx + d] = hi[d];
+	}
 	nmass[idx] = m;
-	xyzr[idx] = make_double4(cx, cy, cz, (rad * rad) * ratio_sqr);
+	xyzr[idx] = c;
 }
 
 // Second central moments Q = sum m (x - cm)(x - cm)^T of node idx from its
 // children (parallel-axis theorem); components xx yy zz xy xz yz.  Not part of
 // the reference tree: input of the learned correction.
-__device__ inline void update_quad(size_t idx, size_t l, size_t r, const double4* __restrict__ xyzr,
-                                   const double* __restrict__ nmass, double* __restrict__ quad)
+__device__ inline void combine_quad(const double4& c, const double4& cl, double ml, const double* ql,
+                                    const double4& cr, double mr, const double* qr, double* q)
 {
-	const double4 c = xyzr[idx];
-	double q[6];
 	for(int k = 0; k < 6; ++k) { q[k] = 0.0; }
 	for(int side = 0; side < 2; ++side)
 	{
-		const size_t ch = side == 0 ? l : r;
-		const double4 cc = xyzr[ch];
-		const double m = nmass[ch];
+		const double4 cc = side == 0 ? cl : cr;
+		const double m = side == 0 ? ml : mr;
+		const double* qc = side == 0 ? ql : qr;
 		const double dx = cc.x - c.x, dy = cc.y - c.y, dz = cc.z - c.z;
-		q[0] += quad[6 * ch + 0] + m * dx * dx;
-		q[1] += quad[6 * ch + 1] + m * dy * dy;
-		q[2] += quad[6 * ch + 2] + m * dz * dz;
-		q[3] += quad[6 * ch + 3] + m * dx * dy;
-		q[4] += quad[6 * ch + 4] + m * dx * dz;
-		q[5] += quad[6 * ch + 5] + m * dy * dz;
+		q[0] += qc[0] + m * dx * dx;
+		q[1] += qc[1] + m * dy * dy;
+		q[2] += qc[2] + m * dz * dz;
+		q[3] += qc[3] + m * dx * dy;
+		q[4] += qc[4] + m * dx * dz;
+		q[5] += qc[5] + m * dy * dz;
 	}
+}
+
+__device__ inline void update_quad(size_t idx, size_t l, size_t r, const double4* __restrict__ xyzr,
+                                   const double* __restrict__ nmass, double* __restrict__ quad)
+{
+	double q[6];
+	combine_quad(xyzr[idx], xyzr[l], nmass[l], quad + 6 * l, xyzr[r], nmass[r], quad + 6 * r, q);
 	for(int k = 0; k < 6; ++k) { quad[6 * idx + k] = q[k]; }
 }
 
@@ -784,7 +803,7 @@ __device__ inline int radix_delta(const uint64_t* __restrict__ k, long long n, l
 // straddle the core of the state the way a fixed run of the Morton order can.
 constexpr int kRadixGroup = 64;
 constexpr int kRadixWalk = 128;
-constexpr int kRadixSuper = 1024;
+constexpr int kRadixSuper = 2048;
 
 // Internal node i of [0, n - 1): its key range, split and children (ids), the
 // parent links of the children, its level (the common prefix length of its
@@ -1043,18 +1062,22 @@ k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restric
                       double* __restrict__ bmin, double* __restrict__ bmax, double* __restrict__ quad,
                       double ratio_sqr, nbh_walk_rec* __restrict__ wr)
 {
-	constexpr int kSlots = 2 * kRadixGroup;
-	__shared__ double4 s_c[kSlots];
-	__shared__ double s_m[kSlots];
-	__shared__ double s_lo[3 * kSlots];
-	__shared__ double s_hi[3 * kSlots];
-	__shared__ double s_q[6 * kSlots];
+	// leaves: position (box and mass centre) and mass; internal slots: all moments
+	__shared__ double4 s_lc[kRadixGroup];
+	__shared__ double s_lm[kRadixGroup];
+	__shared__ double4 s_c[kRadixGroup];
+	__shared__ double s_m[kRadixGroup];
+	__shared__ double s_lo[3 * kRadixGroup];
+	__shared__ double s_hi[3 * kRadixGroup];
+	__shared__ double s_q[6 * kRadixGroup];
 	__shared__ int2 s_child[kRadixGroup];
 	__shared__ int q[kRadixGroup];          // internal nodes, breadth-first (at most kRadixGroup - 1)
 	__shared__ int start[kRadixGroup + 1];  // first entry of each depth
+	__shared__ double s_zero[6];            // the second moments of a leaf
 	const int lane = threadIdx.x;
 	const int4 gr = grp[blockIdx.x];
 	const int root = gr.x, first = gr.y, cnt = gr.z;
+	if(lane < 6) { s_zero[lane] = 0.0; }
 	if(lane < cnt)
 	{
 		const size_t sidx = static_cast<size_t>(first) + lane;
@@ -1088,17 +1111,13 @@ k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restric
 		r.m = m;
 		for(int k = 0; k < 6; ++k) { r.q[k] = 0.0f; }
 		wr[idx] = r;
-		s_c[lane] = c;
-		s_m[lane] = m;
-		s_lo[3 * lane] = x; s_lo[3 * lane + 1] = yy; s_lo[3 * lane + 2] = z;
-		s_hi[3 * lane] = x; s_hi[3 * lane + 1] = yy; s_hi[3 * lane + 2] = z;
-		for(int k = 0; k < 6; ++k) { s_q[6 * lane + k] = 0.0; }
+		s_lc[lane] = c;
+		s_lm[lane] = m;
 		const int id = first + 1 + lane;
 		s_child[lane] = id < n ? child[id] : make_int2(0, 0);
 	}
 	if(root >= n) { return; }              // a group of one leaf
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	auto slot = [&](int id) { return id >= n ? id - n - first : kRadixGroup + id - first - 1; };
 	if(lane == 0)
 	{
 		q[0] = root;
@@ -1122,32 +1141,70 @@ k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restric
 		if(lane == 0) { start[depth] = head; }
 		__syncthreads();
 	}
-	// depth d holds q[start[d], start[d + 1]); start[depth] == tail
+	// depth d holds q[start[d], start[d + 1]); start[depth] == tail.  A child is
+	// leaf first + j (id n + first + j) or internal slot id - first - 1.
 	for(int d = depth - 1; d >= 0; --d)
 	{
 		const int i = start[d] + lane;
 		if(i < start[d + 1])
 		{
 			const int id = q[i];
-			const int2 c = s_child[id - first - 1];
-			const int k = slot(id), l = slot(c.x), r = slot(c.y);
-			update_node(k, l, r, s_c, s_m, s_lo, s_hi, ratio_sqr);
-			update_quad(k, l, r, s_c, s_m, s_q);
+			const int k = id - first - 1;
+			const int2 ch = s_child[k];
+			double4 cc[2];
+			double mm[2];
+			double lo2[2][3], hi2[2][3];
+			const double* qq[2];
+			for(int side = 0; side < 2; ++side)
+			{
+				const int cid = side == 0 ? ch.x : ch.y;
+				if(cid >= n)
+				{
+					const int j = cid - n - first;
+					cc[side] = s_lc[j];
+					mm[side] = s_lm[j];
+					lo2[side][0] = hi2[side][0] = cc[side].x;
+					lo2[side][1] = hi2[side][1] = cc[side].y;
+					lo2[side][2] = hi2[side][2] = cc[side].z;
+					qq[side] = s_zero;
+				}
+				else
+				{
+					const int j = cid - first - 1;
+					cc[side] = s_c[j];
+					mm[side] = s_m[j];
+					for(int e = 0; e < 3; ++e)
+					{
+						lo2[side][e] = s_lo[3 * j + e];
+						hi2[side][e] = s_hi[3 * j + e];
+					}
+					qq[side] = s_q + 6 * j;
+				}
+			}
+			double4 c;
+			double m, lo[3], hi[3], qv[6];
+			combine_node(cc[0], mm[0], lo2[0], hi2[0], cc[1], mm[1], lo2[1], hi2[1], ratio_sqr, c, m, lo, hi);
+			combine_quad(c, cc[0], mm[0], qq[0], cc[1], mm[1], qq[1], qv);
+			s_c[k] = c;
+			s_m[k] = m;
 			const size_t gid = static_cast<size_t>(id);
-			xyzr[gid] = s_c[k];
-			nmass[gid] = s_m[k];
+			xyzr[gid] = c;
+			nmass[gid] = m;
 			nbh_walk_rec rec;
-			rec.c = s_c[k];
-			rec.m = s_m[k];
+			rec.c = c;
+			rec.m = m;
 			for(int e = 0; e < 3; ++e)
 			{
-				bmin[3 * gid + e] = s_lo[3 * k + e];
-				bmax[3 * gid + e] = s_hi[3 * k + e];
+				s_lo[3 * k + e] = lo[e];
+				s_hi[3 * k + e] = hi[e];
+				bmin[3 * gid + e] = lo[e];
+				bmax[3 * gid + e] = hi[e];
 			}
 			for(int e = 0; e < 6; ++e)
 			{
-				quad[6 * gid + e] = s_q[6 * k + e];
-				rec.q[e] = static_cast<float>(s_q[6 * k + e]);
+				s_q[6 * k + e] = qv[e];
+				quad[6 * gid + e] = qv[e];
+				rec.q[e] = static_cast<float>(qv[e]);
 			}
 			wr[gid] = rec;
 		}
@@ -3457,7 +3514,7 @@ static size_t two_bytes(size_t supers, int cap)
 }
 
 // Super-groups of the tree's last build: 1024-body heap nodes, or the Morton
-// tree's subtrees of 512 to 1024 bodies.
+// tree's subtrees of 1024 to 2048 bodies.
 static size_t super_count(const nbh_tree* t)
 {
 	return t->radix ? static_cast<size_t>(t->nsupers) : t->n >> (6 + kSuperShift);
