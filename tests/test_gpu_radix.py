@@ -233,3 +233,29 @@ def test_radix_engine_key():
     assert np.all(np.isfinite(dd.bodies()[0]))
     s.destroy()
     e.destroy()
+
+
+def test_radix_two_level_stack_overflow_falls_back(ctx, monkeypatch):
+    """The Morton tree's super-group walk run out of stack (forced through the
+    library's NBH_TEST_SUPER_STACK hook, 64 entries): the walk groups of such a
+    super-group ignore its partial list and expansion and walk one level, so
+    every body gets either the one-level local walk's result or the normal
+    two-level result, bit for bit, some of them the former, and the fallbacks are
+    counted in walk counter out[2]."""
+    n, near_k = 1 << 17, 3.0
+    y, m = _state("plummer", n, 46)
+    t = RTree(ctx, y, m, 1.0 / 0.3)         # a small opening angle: the super-group walks open many nodes
+    one = t.walk("local", near_k)
+    monkeypatch.setenv("NBH_TEST_SUPER_STACK", "64")
+    two = t.walk("local2", near_k)
+    c = t.counters()
+    monkeypatch.delenv("NBH_TEST_SUPER_STACK")
+    normal = t.walk("local2", near_k)
+    assert t.counters()[2] == 0 and not np.array_equal(normal, one)
+    a1, a2, an = (v[3 * n:].reshape(3, n) for v in (one, two, normal))
+    same_one = np.all(a2 == a1, axis=0)
+    same_normal = np.all(a2 == an, axis=0)
+    assert np.all(same_one | same_normal)
+    fell = int(np.sum(same_one & ~same_normal))
+    assert fell >= 1 and c[2] >= 1, (fell, c[2])
+    t.close()
