@@ -100,7 +100,7 @@ struct nbh_tree {
 	uint32_t* flags = nullptr;     // [3n] build group, super-group and walk group flags, scanned in place
 	int4*     grp = nullptr;       // [n] build groups (node, first leaf, leaves)
 	int4*     wgrp = nullptr;      // [n] walk groups (node, first leaf, leaves, super-group)
-	int2*     sup = nullptr;       // [n / (kRadixSuper / 2) + 1] super-groups (node, first leaf)
+	int2*     sup = nullptr;       // [n / kRadixWalk + 1] super-groups (node, first leaf)
 	double*   box = nullptr;       // [6 * (kBoxParts + 1)] partial boxes, then the bounding box
 	double4*  pk = nullptr;        // [n] position and mass of each body, packed for the sorted gather
 	int       ngroups = 0;         // build groups, walk groups and super-groups of the last Morton build
@@ -797,13 +797,14 @@ __device__ inline int radix_delta(const uint64_t* __restrict__ k, long long n, l
 // Groups of the Morton tree, all maximal subtrees: build groups of at most
 // kRadixGroup leaves (one wave computes their nodes' moments), walk groups of at
 // most kRadixWalk leaves (one wave walks for them, two bodies per lane), and
-// super-groups of at most kRadixSuper leaves that hold at least half as many
-// (smaller ones leave their walk groups to walk in one level).  Their leaves are
+// super-groups of at most kRadixSuper leaves that hold more than one walk group
+// (a walk group in no super-group walks in one level).  Their leaves are
 // consecutive in Morton order and their boxes are node boxes, so they never
 // straddle the core of the state the way a fixed run of the Morton order can.
 constexpr int kRadixGroup = 64;
 constexpr int kRadixWalk = 128;
 constexpr int kRadixSuper = 2048;
+constexpr int kRadixSuperMin = kRadixWalk + 1;   // at least two walk groups
 
 // Internal node i of [0, n - 1): its key range, split and children (ids), the
 // parent links of the children, its level (the common prefix length of its
@@ -852,7 +853,7 @@ k_radix_nodes(const uint64_t* __restrict__ k, long long n, int2* __restrict__ ch
 		if(cnt > kRadixGroup) { atomicAdd(&hist[dnode], 1); }
 		const long long cl = gamma - lo + 1, cr = hi - gamma;
 		if(id == kRoot && cnt <= kRadixGroup) { gmark[lo] = make_int2(id, static_cast<int>(cnt)); }
-		if(id == kRoot && cnt <= kRadixSuper && 2 * cnt >= kRadixSuper) { smark[lo] = make_int2(id, static_cast<int>(cnt)); }
+		if(id == kRoot && cnt <= kRadixSuper && cnt >= kRadixSuperMin) { smark[lo] = make_int2(id, static_cast<int>(cnt)); }
 		if(cnt > kRadixGroup)
 		{
 			if(cl <= kRadixGroup) { gmark[lo] = make_int2(left, static_cast<int>(cl)); }
@@ -866,8 +867,8 @@ k_radix_nodes(const uint64_t* __restrict__ k, long long n, int2* __restrict__ ch
 		}
 		if(cnt > kRadixSuper)
 		{
-			if(cl <= kRadixSuper && 2 * cl >= kRadixSuper) { smark[lo] = make_int2(left, static_cast<int>(cl)); }
-			if(cr <= kRadixSuper && 2 * cr >= kRadixSuper) { smark[gamma + 1] = make_int2(right, static_cast<int>(cr)); }
+			if(cl <= kRadixSuper && cl >= kRadixSuperMin) { smark[lo] = make_int2(left, static_cast<int>(cl)); }
+			if(cr <= kRadixSuper && cr >= kRadixSuperMin) { smark[gamma + 1] = make_int2(right, static_cast<int>(cr)); }
 		}
 	}
 	__syncthreads();
@@ -3088,7 +3089,7 @@ static nbh_status radix_alloc(nbh_tree* t)
 	const size_t sizes[kBufs] = { n * sizeof(int2), 2 * n * sizeof(int), n * sizeof(int), n * sizeof(int),
 	                              (3 * kRadixLevels + 3) * sizeof(int), n * sizeof(int2), n * sizeof(int2),
 	                              3 * n * sizeof(uint32_t), n * sizeof(int4),
-	                              (n / (kRadixSuper / 2) + 1) * sizeof(int2), 6 * (kBoxParts + 1) * sizeof(double),
+	                              (n / kRadixWalk + 1) * sizeof(int2), 6 * (kBoxParts + 1) * sizeof(double),
 	                              n * sizeof(double4), n * sizeof(int2), n * sizeof(int4) };
 	hipError_t e = hipSuccess;
 	for(int k = 0; k < kBufs && e == hipSuccess; ++k)
@@ -3254,7 +3255,7 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	t->nwalk = host_info[3 * kRadixLevels + 2];
 	NBH_CHECK(t->ngroups >= 1 && static_cast<size_t>(t->ngroups) <= n, "bad group count %d", t->ngroups);
 	NBH_CHECK(t->nwalk >= 1 && t->nwalk <= t->ngroups, "bad walk group count %d", t->nwalk);
-	NBH_CHECK(t->nsupers >= 0 && static_cast<size_t>(t->nsupers) <= n / (kRadixSuper / 2) + 1,
+	NBH_CHECK(t->nsupers >= 0 && static_cast<size_t>(t->nsupers) <= n / kRadixWalk + 1,
 	          "bad super-group count %d", t->nsupers);
 	NBH_TRY(radix_moments(ctx, t, y, order, ratio, true));
 	t->radix = true;
@@ -3514,7 +3515,7 @@ static size_t two_bytes(size_t supers, int cap)
 }
 
 // Super-groups of the tree's last build: 1024-body heap nodes, or the Morton
-// tree's subtrees of 1024 to 2048 bodies.
+// tree's maximal subtrees of 129 to 2048 bodies.
 static size_t super_count(const nbh_tree* t)
 {
 	return t->radix ? static_cast<size_t>(t->nsupers) : t->n >> (6 + kSuperShift);
