@@ -594,7 +594,7 @@ def traffic_from_profile(n: int):
 
 
 CLOCK_GHZ = 2.4
-BH_COUNTERS = "bh_two_level_walk_counters_plummer_{n}.md"
+BH_COUNTERS = ("bh_two_level_walk_counters_plummer_{n}.md", "bh_morton_two_level_walk_plummer_{n}.md")
 BH_COUNTER_VARIANT = {"theta=0.62 near=3": "quadrupole theta=0.62, local far field, near 3"}
 for _th, _near in (("0.5", "3"), ("0.55", "3"), ("0.58", "3"), ("0.6", "3"), ("0.62", "3"), ("0.7", "3"),
                    ("0.55", "2.5"), ("0.58", "2.5"), ("0.6", "2.5")):
@@ -608,18 +608,20 @@ def bh_roofline_from_profile(n: int, prefer=None):
     'valu_wave_instr_per_launch <key>: <count>' and 'traffic_bytes_per_launch
     <key>: <bytes>'), with the bench variant they were measured on: the variant
     `prefer` where the summary holds it, else the first one it holds; or None."""
-    path = ROOT / "profiles" / BH_COUNTERS.format(n=n)
-    if not path.exists():
-        return None
-    got = {}
-    for line in path.read_text().splitlines():
-        for tag, field in (("valu_wave_instr_per_launch ", "valu"), ("traffic_bytes_per_launch ", "traffic")):
-            if line.startswith(tag) and ":" in line:
-                key, val = line[len(tag):].split(":", 1)
-                got.setdefault(key.strip(), {})[field] = float(val)
-    found = [{"variant": BH_COUNTER_VARIANT[key], "valu": vals["valu"], "traffic": vals["traffic"],
-              "source": f"profiles/{path.name}"}
-             for key, vals in got.items() if key in BH_COUNTER_VARIANT and "valu" in vals and "traffic" in vals]
+    found = []
+    for name in BH_COUNTERS:
+        path = ROOT / "profiles" / name.format(n=n)
+        if not path.exists():
+            continue
+        got = {}
+        for line in path.read_text().splitlines():
+            for tag, field in (("valu_wave_instr_per_launch ", "valu"), ("traffic_bytes_per_launch ", "traffic")):
+                if line.startswith(tag) and ":" in line:
+                    key, val = line[len(tag):].split(":", 1)
+                    got.setdefault(key.strip(), {})[field] = float(val)
+        found += [{"variant": BH_COUNTER_VARIANT[key], "valu": vals["valu"], "traffic": vals["traffic"],
+                   "source": f"profiles/{path.name}"}
+                  for key, vals in got.items() if key in BH_COUNTER_VARIANT and "valu" in vals and "traffic" in vals]
     for f in found:
         if f["variant"] == prefer:
             return f
