@@ -597,17 +597,18 @@ CLOCK_GHZ = 2.4
 BH_COUNTERS = ("bh_two_level_walk_counters_plummer_{n}.md", "bh_morton_two_level_walk_plummer_{n}.md")
 BH_COUNTER_VARIANT = {"theta=0.62 near=3": "quadrupole theta=0.62, local far field, near 3"}
 for _th, _near in (("0.5", "3"), ("0.55", "3"), ("0.58", "3"), ("0.6", "3"), ("0.62", "3"), ("0.7", "3"),
-                   ("0.55", "2.5"), ("0.58", "2.5"), ("0.6", "2.5")):
+                   ("0.55", "2.5"), ("0.58", "2.5"), ("0.6", "2.5"), ("0.5", "2"), ("0.55", "2"), ("0.58", "2")):
     BH_COUNTER_VARIANT[f"morton theta={_th} near={_near}"] = \
         f"morton tree: quadrupole theta={_th}, local far field, near {_near}"
 
 
 def bh_roofline_from_profile(n: int, prefer=None):
     """VALU wave instructions and HBM bytes per launch of the two-level group walk
-    at this N, from the counter summary in profiles/ (lines
+    at this N, from the counter summaries in profiles/ (lines
     'valu_wave_instr_per_launch <key>: <count>' and 'traffic_bytes_per_launch
     <key>: <bytes>'), with the bench variant they were measured on: the variant
-    `prefer` where the summary holds it, else the first one it holds; or None."""
+    `prefer` where a summary holds it, else one on the same tree, else the first
+    one; or None."""
     found = []
     for name in BH_COUNTERS:
         path = ROOT / "profiles" / name.format(n=n)
@@ -624,6 +625,10 @@ def bh_roofline_from_profile(n: int, prefer=None):
                   for key, vals in got.items() if key in BH_COUNTER_VARIANT and "valu" in vals and "traffic" in vals]
     for f in found:
         if f["variant"] == prefer:
+            return f
+    morton = prefer is not None and prefer.startswith("morton tree:")
+    for f in found:
+        if f["variant"].startswith("morton tree:") == morton:
             return f
     return found[0] if found else None
 
@@ -652,7 +657,7 @@ def bh_variants(learned: bool = True):
     # the same two-level local walk on the Morton radix tree, whose cells are
     # compact where the heap's count-balanced cells span from the core outwards
     for th, near in ((0.5, 3), (0.55, 3), (0.58, 3), (0.6, 3), (0.62, 3), (0.7, 3), (0.55, 2.5), (0.58, 2.5),
-                     (0.6, 2.5)):
+                     (0.6, 2.5), (0.5, 2), (0.55, 2), (0.58, 2)):
         v.append((f"morton tree: quadrupole theta={th:g}, local far field, near {near:g}", 1.0 / th, "quadrupole",
                   {**local, "bh_quad_near": float(near), "bh_tree": "morton"}))
     if learned:

@@ -38,3 +38,11 @@ def test_roofline_figures_follow_the_preferred_variant():
 
 def test_direct_traffic_figure_present():
     assert bench.traffic_from_profile(1 << 18) > 0
+
+
+def test_roofline_falls_back_to_the_same_tree():
+    n = 1 << 22
+    r = bench.bh_roofline_from_profile(n, "morton tree: quadrupole theta=0.7, local far field, near 3")
+    assert r["variant"].startswith("morton tree:")
+    r = bench.bh_roofline_from_profile(n, "quadrupole theta=0.8, local far field, near 5")
+    assert not r["variant"].startswith("morton tree:")
