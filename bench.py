@@ -596,8 +596,8 @@ def traffic_from_profile(n: int):
 CLOCK_GHZ = 2.4
 BH_COUNTERS = ("bh_two_level_walk_counters_plummer_{n}.md", "bh_morton_two_level_walk_plummer_{n}.md")
 BH_COUNTER_VARIANT = {"theta=0.62 near=3": "quadrupole theta=0.62, local far field, near 3"}
-for _th, _near in (("0.5", "3"), ("0.55", "3"), ("0.58", "3"), ("0.6", "3"), ("0.62", "3"), ("0.7", "3"),
-                   ("0.55", "2.5"), ("0.58", "2.5"), ("0.6", "2.5"), ("0.5", "2"), ("0.55", "2"), ("0.58", "2")):
+for _th, _near in (("0.5", "3"), ("0.6", "3"), ("0.7", "3"), ("0.55", "2.5"), ("0.58", "2.5"), ("0.6", "2.5"),
+                   ("0.5", "2"), ("0.55", "2"), ("0.58", "2"), ("0.6", "2")):
     BH_COUNTER_VARIANT[f"morton theta={_th} near={_near}"] = \
         f"morton tree: quadrupole theta={_th}, local far field, near {_near}"
 
@@ -656,8 +656,8 @@ def bh_variants(learned: bool = True):
           {**local, "bh_quad_near": 4.0})]
     # the same two-level local walk on the Morton radix tree, whose cells are
     # compact where the heap's count-balanced cells span from the core outwards
-    for th, near in ((0.5, 3), (0.55, 3), (0.58, 3), (0.6, 3), (0.62, 3), (0.7, 3), (0.55, 2.5), (0.58, 2.5),
-                     (0.6, 2.5), (0.5, 2), (0.55, 2), (0.58, 2)):
+    for th, near in ((0.5, 3), (0.6, 3), (0.7, 3), (0.55, 2.5), (0.58, 2.5), (0.6, 2.5), (0.5, 2), (0.55, 2),
+                     (0.58, 2), (0.6, 2)):
         v.append((f"morton tree: quadrupole theta={th:g}, local far field, near {near:g}", 1.0 / th, "quadrupole",
                   {**local, "bh_quad_near": float(near), "bh_tree": "morton"}))
     if learned:

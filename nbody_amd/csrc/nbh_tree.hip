@@ -2241,7 +2241,9 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 	const double gx0 = glo[0], gy0 = glo[1], gz0 = glo[2];
 	const double gx1 = ghi[0], gy1 = ghi[1], gz1 = ghi[2];
 	const double ccx = 0.5 * (gx0 + gx1), ccy = 0.5 * (gy0 + gy1), ccz = 0.5 * (gz0 + gz1);
-	const float near_d = static_cast<float>(static_cast<double>(near_k) * half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1));
+	const double super_hd64 = half_diagonal(gx0, gy0, gz0, gx1, gy1, gz1);
+	const float near_d = static_cast<float>(static_cast<double>(near_k) * super_hd64);
+	const float super_hd = static_cast<float>(super_hd64);
 	const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 	int* list = two.lists + static_cast<size_t>(si) * two.cap;
 	float acc[34];
@@ -2290,7 +2292,11 @@ k_bh_super(const nbh_walk_rec* __restrict__ wr, const double* __restrict__ bmin,
 				// a rejected node whose bodies all lie within near_d of the centre
 				// (ball of the cell radius sqrt(crit_r2) / ratio about its mass
 				// centre) is left to the groups; others are opened here
-				open = !accept && !(du + static_cast<float>(sqrt(rec.w) * inv_ratio) < near_d);
+				// (in the Morton tree also one wider than half the super box: it is
+				// opened here once rather than by every walk group, which lowers the
+				// error at the same cost)
+				const float rad = static_cast<float>(sqrt(rec.w) * inv_ratio);
+				open = !accept && (!(du + rad < near_d) || (kRadix && rad > 0.5f * super_hd));
 			}
 		}
 		const bool tolist = valid && !far && !open;

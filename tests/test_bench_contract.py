@@ -24,7 +24,7 @@ def test_variant_labels_unique_and_counter_keys_name_variants():
 
 def test_roofline_figures_follow_the_preferred_variant():
     n = 1 << 22
-    morton = "morton tree: quadrupole theta=0.6, local far field, near 2.5"
+    morton = "morton tree: quadrupole theta=0.6, local far field, near 2"
     heap = "quadrupole theta=0.62, local far field, near 3"
     r = bench.bh_roofline_from_profile(n, morton)
     assert r["variant"] == morton and r["source"].endswith("bh_morton_two_level_walk_plummer_4194304.md")
