@@ -192,12 +192,14 @@ nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* tree, const double* y, double 
    n + j.  Its cells are compact where the heap's count-balanced nodes of a
    centrally concentrated state span from the core to the outskirts, so a walk
    takes far fewer of them at the same ratio.  Only the group walks
-   (nbh_bh_fcompute_group, _quad, _local, _local2) run on it; their groups and
-   super-groups are runs of 64 and 1024 bodies in Morton order.  A walk step
-   whose children would not fit on the walk stack takes the nodes as monopoles
-   (counted in nbh_bh_walk_counters out[2]).  Any n >= 2 builds; the walks need
-   n to be a multiple of 64, and the two-level walk walks in one level unless n
-   is a multiple of 1024. */
+   (nbh_bh_fcompute_group, _quad, _local, _local2) run on it: their walk groups
+   are the maximal subtrees of at most 128 bodies (one wave, two bodies per
+   lane), the two-level walk's super-groups the maximal subtrees of at most 2048
+   bodies that hold more than one walk group (a walk group in none walks in one
+   level).  A walk step whose children would not fit on the walk stack takes
+   the nodes as monopoles (counted in nbh_bh_walk_counters out[2]).  Any n >= 2
+   builds and walks.  The level counts are read back to the host once per build
+   (through pinned memory), to size the launches that follow. */
 nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* tree, const double* y, const double* mass,
                                 double ratio);
 /* Children of internal ids (child[2k], child[2k + 1] for id k = 1..n-1; entry 0
@@ -219,7 +221,8 @@ nbh_status nbh_bh_fcompute(nbh_ctx* ctx, const nbh_tree* tree, const double* y,
    body sees a refinement of the cells its own walk would take (error <= that
    of nbh_bh_fcompute at the same ratio).  Replaces the per-thread walk of
    kfcompute_heap_bh_stackless (nbody_engine_cuda_impl.cu:372-451) on the fast
-   path; n >= 64. */
+   path; n >= 64 on the heap tree (on a Morton tree the groups are its walk
+   groups, nbh_tree_build_radix). */
 nbh_status nbh_bh_fcompute_group(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f);
 /* nbh_bh_fcompute_group plus the quadrupole term of every internal cell taken as
    a monopole (second central moments of the cell about its mass centre, kept
@@ -243,13 +246,16 @@ nbh_status nbh_bh_fcompute_group_quad(nbh_ctx* ctx, const nbh_tree* tree, const 
 nbh_status nbh_bh_fcompute_group_local(nbh_ctx* ctx, const nbh_tree* tree, const double* y, double* f,
                                        float near_k);
 /* nbh_bh_fcompute_group_local in two levels: the tree is walked once per
-   super-group of 1024 bodies (16 groups) with the super-group's box; accepted
-   cells at least max(near_k, 5) half-diagonals of that box from its centre form one
-   expansion about it (monopole to third order, quadrupole to first), and the
-   other accepted nodes a list that each group takes instead of walking (leaves
-   and nearer cells per body, farther ones into the group's own expansion).  A
-   node accepted for the super-group's box is accepted for every group box
-   inside it, so the groups keep the acceptance rule.  Lists longer than 32768
+   super-group (heap tree: 1024 bodies, 16 groups; Morton tree: see
+   nbh_tree_build_radix) with the super-group's box; accepted cells at least
+   max(near_k, 5) half-diagonals of that box from its centre form one expansion
+   about it (monopole to third order, quadrupole to first), and the other
+   accepted nodes a list that each group takes instead of walking (leaves and
+   nearer cells per body, farther ones into the group's own expansion).  A node
+   accepted for the super-group's box is accepted for every group box inside
+   it, so the groups keep the acceptance rule; on the Morton tree the
+   super-group walk also opens every rejected node wider than half its box
+   rather than listing it.  Lists longer than 32768
    entries fall back to the one-level walk for that super-group (counted in
    walk counter out[2]).  The lists (n * 128 bytes at the default capacity) are
    allocated by nbh_tree_set_list_cap, or by the first call when that was not
