@@ -83,7 +83,8 @@ class RTree:
         return list(c)
 
 
-@pytest.mark.parametrize("kind,n,seed", [("plummer", 64, 1), ("plummer", 3000, 2), ("plummer", 4096, 3),
+@pytest.mark.parametrize("kind,n,seed", [("plummer", 2, 7), ("plummer", 3, 8), ("plummer", 7, 9), ("plummer", 100, 10),
+                                         ("plummer", 64, 1), ("plummer", 3000, 2), ("plummer", 4096, 3),
                                          ("ball", 1 << 15, 4), ("plummer", 1 << 15, 5), ("coincident", 2048, 6)])
 def test_radix_build_bit_exact(ctx, kind, n, seed):
     y, m = _state(kind, n, seed)
@@ -148,7 +149,8 @@ def _rms(f, a0, n, idx):
 
 
 @pytest.mark.parametrize("walk", ["group", "quad", "local", "local2"])
-@pytest.mark.parametrize("kind,n", [("plummer", 2048), ("plummer", 3001), ("coincident", 2500)])
+@pytest.mark.parametrize("kind,n", [("plummer", 5), ("plummer", 100), ("plummer", 2048), ("plummer", 3001),
+                                    ("coincident", 2500)])
 def test_radix_walk_coverage(ctx, walk, kind, n):
     """Opening ratio 1e8: every cell is opened down to its bodies, so each walk
     is an FP32 direct sum (every body counted once), within the direct FP32
