@@ -38,7 +38,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wextra",
               "-Wno-unused-parameter", f"-I{INC}", f"-I{HOST}", "-I/opt/rocm/include",
               "-D__HIP_PLATFORM_AMD__"]
-ORACLE_FLAGS = ["-O2", "-fPIC", "-ffp-contract=off", "-fopenmp", "-Wall", "-Wextra", "-std=c11"]
+ORACLE_FLAGS = ["-O3", "-fPIC", "-ffp-contract=off", "-fopenmp", "-Wall", "-Wextra", "-std=c11"]
 
 
 def _run(cmd: list[str]) -> None:
