@@ -1616,6 +1616,42 @@ __device__ inline void sweep32(const float4* __restrict__ buf, int count, float 
 	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
 }
 
+// sweep32 for kT bodies of one lane at once: each buffer entry is read once
+// for all of them, and their independent chains interleave.  Every body's sums
+// take the same operations in the same order as sweep32's.
+template <int kT>
+__device__ inline void sweep32_n(const float4* __restrict__ buf, int count, const float* tx, const float* ty,
+                                 const float* tz, float* ax, float* ay, float* az, float* cx, float* cy, float* cz)
+{
+	float px[kT], py[kT], pz[kT];
+#pragma unroll
+	for(int t = 0; t < kT; ++t) { px[t] = py[t] = pz[t] = 0.0f; }
+#pragma unroll 2
+	for(int j = 0; j < count; ++j)
+	{
+		const float4 s = buf[j];
+#pragma unroll
+		for(int t = 0; t < kT; ++t)
+		{
+			const float dx = s.x - tx[t];
+			const float dy = s.y - ty[t];
+			const float dz = s.z - tz[t];
+			const float ri = rsq_clamped(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+			const float w = (s.w * ri) * (ri * ri);
+			px[t] = fmaf(dx, w, px[t]);
+			py[t] = fmaf(dy, w, py[t]);
+			pz[t] = fmaf(dz, w, pz[t]);
+		}
+	}
+#pragma unroll
+	for(int t = 0; t < kT; ++t)
+	{
+		float v = px[t] - cx[t]; float u = ax[t] + v; cx[t] = (u - ax[t]) - v; ax[t] = u;
+		v = py[t] - cy[t]; u = ay[t] + v; cy[t] = (u - ay[t]) - v; ay[t] = u;
+		v = pz[t] - cz[t]; u = az[t] + v; cz[t] = (u - az[t]) - v; az[t] = u;
+	}
+}
+
 // Cells waiting for the monopole + quadrupole sweep, 40 bytes each in LDS:
 // e0 = (mass-centre offset from the group centre, M), e1 = 2.5 x (Q'xx Q'yy
 // Q'zz Q'xy), e2 = 2.5 x (Q'xz Q'yz), with Q' = 3 Q - tr(Q) I the traceless
@@ -1659,6 +1695,47 @@ __device__ inline void sweep_mq(const quad_lds<kN>& qb, int count, float tx, flo
 	float v = px - cx; float t = ax + v; cx = (t - ax) - v; ax = t;
 	v = py - cy; t = ay + v; cy = (t - ay) - v; ay = t;
 	v = pz - cz; t = az + v; cz = (t - az) - v; az = t;
+}
+
+// sweep_mq for kT bodies of one lane at once (as sweep32_n: one read of each
+// cell, the same operations per body as sweep_mq).
+template <int kT, int kN>
+__device__ inline void sweep_mq_n(const quad_lds<kN>& qb, int count, const float* tx, const float* ty, const float* tz,
+                                  float* ax, float* ay, float* az, float* cx, float* cy, float* cz)
+{
+	float px[kT], py[kT], pz[kT];
+#pragma unroll
+	for(int t = 0; t < kT; ++t) { px[t] = py[t] = pz[t] = 0.0f; }
+	for(int j = 0; j < count; ++j)
+	{
+		const float4 e0 = qb.e0[j], e1 = qb.e1[j];
+		const float2 e2 = qb.e2[j];
+#pragma unroll
+		for(int t = 0; t < kT; ++t)
+		{
+			const float rx = e0.x - tx[t], ry = e0.y - ty[t], rz = e0.z - tz[t];
+			const float u = rsq_clamped(fmaf(rz, rz, fmaf(ry, ry, rx * rx)));
+			const float u2 = u * u;
+			const float u3 = u2 * u;
+			const float u4 = u2 * u2;
+			const float Qx = fmaf(e1.x, rx, fmaf(e1.w, ry, e2.x * rz));
+			const float Qy = fmaf(e1.w, rx, fmaf(e1.y, ry, e2.y * rz));
+			const float Qz = fmaf(e2.x, rx, fmaf(e2.y, ry, e1.z * rz));
+			const float s = fmaf(rx, Qx, fmaf(ry, Qy, rz * Qz));
+			const float k = fmaf(s, u4, e0.w) * u3;
+			const float g = -0.4f * (u4 * u);
+			px[t] = fmaf(rx, k, fmaf(Qx, g, px[t]));
+			py[t] = fmaf(ry, k, fmaf(Qy, g, py[t]));
+			pz[t] = fmaf(rz, k, fmaf(Qz, g, pz[t]));
+		}
+	}
+#pragma unroll
+	for(int t = 0; t < kT; ++t)
+	{
+		float v = px[t] - cx[t]; float w = ax[t] + v; cx[t] = (w - ax[t]) - v; ax[t] = w;
+		v = py[t] - cy[t]; w = ay[t] + v; cy[t] = (w - ay[t]) - v; ay[t] = w;
+		v = pz[t] - cz[t]; w = az[t] + v; cz[t] = (w - az[t]) - v; az[t] = w;
+	}
 }
 
 // Cells waiting for the monopole + quadrupole sweep; a sweep runs before a
@@ -2019,14 +2096,14 @@ __device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* _
 		}
 		if(nbuf > kWalkBuf - kGroup)
 		{
-			for(int t = 0; t < kT; ++t) { sweep32(buf, nbuf, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); }
+			sweep32_n<kT>(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
 			nbuf = 0;
 		}
 		if constexpr(kQuad)
 		{
 			if(nq > kQuadBuf - kGroup)
 			{
-				for(int t = 0; t < kT; ++t) { sweep_mq(qbuf, nq, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); }
+				sweep_mq_n<kT>(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz);
 				nq = 0;
 			}
 			if(ftail - fhead >= kGroup)
@@ -2101,13 +2178,13 @@ __device__ inline void group_walk_one(int gi, int* __restrict__ stack, float4* _
 	}
 	if(nbuf > 0)
 	{
-		for(int t = 0; t < kT; ++t) { sweep32(buf, nbuf, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); }
+		sweep32_n<kT>(buf, nbuf, tx, ty, tz, ax, ay, az, cx, cy, cz);
 	}
 	float kx[kT], ky[kT], kz[kT];
 	for(int t = 0; t < kT; ++t) { kx[t] = ky[t] = kz[t] = 0.0f; }
 	if constexpr(kQuad)
 	{
-		if(nq > 0) { for(int t = 0; t < kT; ++t) { sweep_mq(qbuf, nq, tx[t], ty[t], tz[t], ax[t], ay[t], az[t], cx[t], cy[t], cz[t]); } }
+		if(nq > 0) { sweep_mq_n<kT>(qbuf, nq, tx, ty, tz, ax, ay, az, cx, cy, cz); }
 		for(; fhead < ftail; fhead += kGroup)
 		{
 			far_batch<kLocal>(farn, wr, fhead, ftail - fhead < kGroup ? ftail - fhead : kGroup, lane, ccx, ccy, ccz,
