@@ -202,6 +202,15 @@ nbh_status nbh_tree_refit(nbh_ctx* ctx, nbh_tree* tree, const double* y, double 
    (through pinned memory), to size the launches that follow. */
 nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* tree, const double* y, const double* mass,
                                 double ratio);
+/* Lean Morton builds (lean != 0; no counterpart in the reference): the next
+   nbh_tree_build_radix / nbh_tree_refit of this tree write the walk records of
+   every node but its FP64 node arrays (mass centres, masses, boxes, second
+   moments) only for the nodes the build reads back: the roots of its 64-body
+   build groups, the nodes above them and one-body groups.  The group walks give
+   the same results bit for bit; nbh_tree_get then returns valid entries for
+   those nodes only.  About 0.86 GB less written per build at 4M bodies.
+   Default off. */
+nbh_status nbh_tree_set_radix_lean(nbh_ctx* ctx, nbh_tree* tree, int lean);
 /* Children of internal ids (child[2k], child[2k + 1] for id k = 1..n-1; entry 0
    unused) and parents (parent[id] for ids 2..2n-1) of the last Morton tree;
    either pointer may be NULL. */

@@ -109,6 +109,7 @@ struct nbh_tree {
 	int       level_count[kRadixLevels] = {};   // host copy of the level counts (refit)
 	int       two_slots = 0;       // super-groups the two-level buffers hold
 	bool      radix = false;       // the last build made the Morton tree
+	bool      radix_lean = false;  // Morton node arrays only where the build reads them back
 	double    ratio = 0.0;         // opening ratio of the last build / refit
 	bool      built = false;
 };
@@ -1055,13 +1056,16 @@ k_radix_level(const int* __restrict__ list, int count, const int2* __restrict__ 
 // each internal node's index is one end of its key range).  The internal nodes
 // are listed breadth-first, then taken depth by depth from the deepest (the
 // heap's update_node and update_quad on the LDS slots), and written out.
+// lean: every node's walk record is written (and each leaf's body number and
+// mass), the FP64 node arrays only for the group's root, which the levels above
+// read, or for a group of one leaf.
 template <bool kBuild>
 __global__ void __launch_bounds__(kRadixGroup)
 k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restrict__ child,
                       const uint32_t* __restrict__ order, const double4* __restrict__ pk, const double* __restrict__ y,
                       int32_t* __restrict__ body_n, double4* __restrict__ xyzr, double* __restrict__ nmass,
                       double* __restrict__ bmin, double* __restrict__ bmax, double* __restrict__ quad,
-                      double ratio_sqr, nbh_walk_rec* __restrict__ wr)
+                      double ratio_sqr, nbh_walk_rec* __restrict__ wr, int lean)
 {
 	// leaves: position (box and mass centre) and mass; internal slots: all moments
 	__shared__ double4 s_lc[kRadixGroup];
@@ -1104,9 +1108,12 @@ k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restric
 			m = nmass[idx];
 		}
 		const double4 c = make_double4(x, yy, z, 0.0);
-		xyzr[idx] = c;
-		bmin[3 * idx] = x; bmin[3 * idx + 1] = yy; bmin[3 * idx + 2] = z;
-		bmax[3 * idx] = x; bmax[3 * idx + 1] = yy; bmax[3 * idx + 2] = z;
+		if(!lean || root >= n)
+		{
+			xyzr[idx] = c;
+			bmin[3 * idx] = x; bmin[3 * idx + 1] = yy; bmin[3 * idx + 2] = z;
+			bmax[3 * idx] = x; bmax[3 * idx + 1] = yy; bmax[3 * idx + 2] = z;
+		}
 		nbh_walk_rec r;
 		r.c = c;
 		r.m = m;
@@ -1189,8 +1196,6 @@ k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restric
 			s_c[k] = c;
 			s_m[k] = m;
 			const size_t gid = static_cast<size_t>(id);
-			xyzr[gid] = c;
-			nmass[gid] = m;
 			nbh_walk_rec rec;
 			rec.c = c;
 			rec.m = m;
@@ -1198,16 +1203,24 @@ k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restric
 			{
 				s_lo[3 * k + e] = lo[e];
 				s_hi[3 * k + e] = hi[e];
-				bmin[3 * gid + e] = lo[e];
-				bmax[3 * gid + e] = hi[e];
 			}
 			for(int e = 0; e < 6; ++e)
 			{
 				s_q[6 * k + e] = qv[e];
-				quad[6 * gid + e] = qv[e];
 				rec.q[e] = static_cast<float>(qv[e]);
 			}
 			wr[gid] = rec;
+			if(!lean || id == root)
+			{
+				xyzr[gid] = c;
+				nmass[gid] = m;
+				for(int e = 0; e < 3; ++e)
+				{
+					bmin[3 * gid + e] = lo[e];
+					bmax[3 * gid + e] = hi[e];
+				}
+				for(int e = 0; e < 6; ++e) { quad[6 * gid + e] = qv[e]; }
+			}
 		}
 		__syncthreads();
 	}
@@ -3224,13 +3237,13 @@ static nbh_status radix_moments(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	{
 		hipLaunchKernelGGL(k_radix_group_moments<true>, dim3(static_cast<unsigned>(t->ngroups)), dim3(kRadixGroup), 0,
 		                   ctx->stream, t->grp, static_cast<int>(n), t->child, order, t->pk, y, t->body_n, t->xyzr,
-		                   t->mass, t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec);
+		                   t->mass, t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec, t->radix_lean ? 1 : 0);
 	}
 	else
 	{
 		hipLaunchKernelGGL(k_radix_group_moments<false>, dim3(static_cast<unsigned>(t->ngroups)), dim3(kRadixGroup),
 		                   0, ctx->stream, t->grp, static_cast<int>(n), t->child, order, t->pk, y, t->body_n,
-		                   t->xyzr, t->mass, t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec);
+		                   t->xyzr, t->mass, t->bmin, t->bmax, t->quad, ratio_sqr, t->wrec, t->radix_lean ? 1 : 0);
 	}
 	NBH_LAUNCHED();
 	int offset[kRadixLevels];
@@ -3438,6 +3451,14 @@ nbh_status nbh_tree_build(nbh_ctx* ctx, nbh_tree* t, const double* y, const doub
 	}
 	NBH_TRY(moments(ctx, t, y, mass, ratio, true));
 	t->built = true;
+	return NBH_OK;
+}
+
+nbh_status nbh_tree_set_radix_lean(nbh_ctx* ctx, nbh_tree* t, int lean)
+{
+	NBH_TRY(nbh::enter(ctx));
+	NBH_CHECK(t != nullptr && t->ctx == ctx, "tree does not belong to this context");
+	t->radix_lean = lean != 0;
 	return NBH_OK;
 }
 

@@ -875,6 +875,12 @@ bool nbody_engine_hip_bh::init(nbody_data* data)
 	{
 		return false;
 	}
+	// the Morton tree's group walks read its walk records only: its builds skip
+	// the FP64 node arrays of the nodes inside the 64-body build groups
+	if(m_morton && !check(nbh_tree_set_radix_lean(m_ctx, m_tree, 1), "tree setup"))
+	{
+		return false;
+	}
 	// the two-level walk's lists (default capacity, 32768 entries per 1024 bodies)
 	// are allocated here, not inside the first force evaluation
 	if(m_group_walk && m_quadrupole && m_local_far && m_precision == NBH_FP32 &&

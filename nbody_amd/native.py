@@ -65,6 +65,7 @@ HIP_API = {
     "nbh_tree_build": (i32, [vp, vp, vp, vp, dbl]),
     "nbh_tree_refit": (i32, [vp, vp, vp, dbl]),
     "nbh_tree_build_radix": (i32, [vp, vp, vp, vp, dbl]),
+    "nbh_tree_set_radix_lean": (i32, [vp, vp, i32]),
     "nbh_tree_get_children": (i32, [vp, vp, vp, vp]),
     "nbh_bh_fcompute": (i32, [vp, vp, vp, vp, vp, i32, i32]),
     "nbh_bh_fcompute_group": (i32, [vp, vp, vp, vp]),

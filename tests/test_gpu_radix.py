@@ -39,10 +39,12 @@ def _state(kind, n, seed):
 
 
 class RTree:
-    def __init__(self, ctx, y, m, ratio, radix=True):
+    def __init__(self, ctx, y, m, ratio, radix=True, lean=False):
         self.ctx, self.n = ctx, len(m)
         self.h = C.c_void_p()
         native.check(ctx.lib.nbh_tree_create(ctx.h, self.n, C.byref(self.h)), "tree_create")
+        if lean:
+            native.check(ctx.lib.nbh_tree_set_radix_lean(ctx.h, self.h, 1), "lean")
         self.dy, self.dm = ctx.put(y), ctx.put(m)
         build = ctx.lib.nbh_tree_build_radix if radix else ctx.lib.nbh_tree_build
         native.check(build(ctx.h, self.h, self.dy.ptr, self.dm.ptr, ratio), "build")
@@ -100,6 +102,32 @@ def test_radix_build_bit_exact(ctx, kind, n, seed):
     for k in ("cm", "mass", "crit", "bmin", "bmax"):
         assert np.array_equal(g[k][ids], o[k][ids]), k
     t.close()
+
+
+@pytest.mark.parametrize("kind,n,seed", [("plummer", 3, 8), ("plummer", 100, 10), ("plummer", 3000, 2),
+                                         ("plummer", 1 << 15, 5), ("coincident", 2500, 6)])
+def test_radix_lean_walks_equal(ctx, kind, n, seed):
+    """A lean build (FP64 node arrays only where the build reads them back) gives
+    the group walks the same records: every walk equals the full build's bit for
+    bit, after the build and after a refit of moved bodies, and the root's
+    moments agree."""
+    y, m = _state(kind, n, seed)
+    ratio = 1.6
+    full, lean = RTree(ctx, y, m, ratio), RTree(ctx, y, m, ratio, lean=True)
+    y2 = y.copy()
+    y2[:3 * n] += np.random.default_rng(3).standard_normal(3 * n) * 1e-3
+    for step in range(2):
+        for walk in ("group", "quad", "local", "local2"):
+            assert np.array_equal(full.walk(walk, 2.0), lean.walk(walk, 2.0)), (step, walk)
+        a, b = full.arrays(), lean.arrays()
+        for k in ("cm", "mass", "crit", "bmin", "bmax"):
+            assert np.array_equal(a[k][1], b[k][1]), k
+        assert np.array_equal(a["body"], b["body"])
+        for t in (full, lean):
+            t.dy.write(y2)
+            native.check(ctx.lib.nbh_tree_refit(ctx.h, t.h, t.dy.ptr, ratio), "refit")
+    full.close()
+    lean.close()
 
 
 def test_radix_refit_keeps_structure(ctx):
