@@ -44,7 +44,7 @@ constexpr int kScanOneThreads = 1024;
 constexpr int kScanOneItems = 16;                    // scans of up to 16384 counts in one work-group
 constexpr int kLocalSeg = 4096;                      // segments split in LDS, one work-group each
 constexpr int kLocalThreads = 1024;
-constexpr int kBoxParts = 256;                       // partial boxes of the Morton tree's bounding box
+constexpr int kBoxParts = 2048;                      // partial boxes of the Morton tree's bounding box
 constexpr int kRadixLevels = 128;                    // prefix lengths of index-extended 63-bit keys: 0..127
 
 }  // namespace
@@ -770,16 +770,36 @@ k_box_partial(const double* __restrict__ y, size_t n, double* __restrict__ part)
 	if(threadIdx.x < 6) { part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0]; }
 }
 
-__global__ void __launch_bounds__(64)
+// The partials are taken strided by 256 threads, then reduced in LDS (min and
+// max are exact, so the order does not change the box).
+__global__ void __launch_bounds__(256)
 k_box_final(const double* __restrict__ part, int count, double* __restrict__ box)
 {
-	if(threadIdx.x < 6)
+	__shared__ double sh[6][256];
+	double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	for(int k = threadIdx.x; k < count; k += blockDim.x)
 	{
-		const int d = threadIdx.x;
-		double v = part[d];
-		for(int k = 1; k < count; ++k) { v = d < 3 ? fmin(v, part[6 * k + d]) : fmax(v, part[6 * k + d]); }
-		box[d] = v;
+		for(int d = 0; d < 3; ++d)
+		{
+			lo[d] = fmin(lo[d], part[6 * k + d]);
+			hi[d] = fmax(hi[d], part[6 * k + 3 + d]);
+		}
 	}
+	for(int d = 0; d < 3; ++d) { sh[d][threadIdx.x] = lo[d]; sh[3 + d][threadIdx.x] = hi[d]; }
+	__syncthreads();
+	for(int w = 128; w > 0; w >>= 1)
+	{
+		if(static_cast<int>(threadIdx.x) < w)
+		{
+			for(int d = 0; d < 3; ++d)
+			{
+				sh[d][threadIdx.x] = fmin(sh[d][threadIdx.x], sh[d][threadIdx.x + w]);
+				sh[3 + d][threadIdx.x] = fmax(sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + w]);
+			}
+		}
+		__syncthreads();
+	}
+	if(threadIdx.x < 6) { box[threadIdx.x] = sh[threadIdx.x][0]; }
 }
 
 // Length of the common prefix of sorted keys i and j (with the index as a
@@ -1050,15 +1070,15 @@ k_radix_level(const int* __restrict__ list, int count, const int2* __restrict__ 
 // kRadixGroup leaves), one wave per group, with their walk records.  The
 // group's leaves (as k_leaves; kBuild: the body at sorted position s is
 // order[s], read from its packed record; refit: the body numbers of the build,
-// positions from y and the masses kept) and node moments are held in LDS: leaf
+// positions from y and the masses of the leaf records) and node moments are held in LDS: leaf
 // first + j in slot j, internal id k in slot kRadixGroup + k - first - 1 (the
 // internal ids of a subtree over the sorted leaves [a, b] lie in [a + 1, b + 1]:
 // each internal node's index is one end of its key range).  The internal nodes
 // are listed breadth-first, then taken depth by depth from the deepest (the
 // heap's update_node and update_quad on the LDS slots), and written out.
-// lean: every node's walk record is written (and each leaf's body number and
-// mass), the FP64 node arrays only for the group's root, which the levels above
-// read, or for a group of one leaf.
+// lean: every node's walk record is written (and each leaf's body number), the
+// FP64 node arrays only for the group's root, which the levels above read, or
+// for a group of one leaf.
 template <bool kBuild>
 __global__ void __launch_bounds__(kRadixGroup)
 k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restrict__ child,
@@ -1097,15 +1117,16 @@ k_radix_group_moments(const int4* __restrict__ grp, int n, const int2* __restric
 			z = p.z;
 			m = p.w;
 			body_n[idx] = static_cast<int32_t>(b);
-			nmass[idx] = m;
+			if(!lean || root >= n) { nmass[idx] = m; }
 		}
 		else
 		{
+			// the mass from the leaf's walk record, which every build writes
 			const size_t b = static_cast<uint32_t>(body_n[idx]);
 			x = y[b];
 			yy = y[n + b];
 			z = y[2 * static_cast<size_t>(n) + b];
-			m = nmass[idx];
+			m = wr[idx].m;
 		}
 		const double4 c = make_double4(x, yy, z, 0.0);
 		if(!lean || root >= n)
@@ -3293,7 +3314,7 @@ nbh_status nbh_tree_build_radix(nbh_ctx* ctx, nbh_tree* t, const double* y, cons
 	const unsigned parts = std::min<unsigned>(kBoxParts, nbh::grid_for(ctx, n, 256));
 	hipLaunchKernelGGL(k_box_partial, dim3(parts), dim3(256), 0, ctx->stream, y, n, t->box + 6);
 	NBH_LAUNCHED();
-	hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, ctx->stream, t->box + 6, static_cast<int>(parts), t->box);
+	hipLaunchKernelGGL(k_box_final, dim3(1), dim3(256), 0, ctx->stream, t->box + 6, static_cast<int>(parts), t->box);
 	NBH_LAUNCHED();
 	hipLaunchKernelGGL(k_morton_codes, dim3(nbh::grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, y, mass, n,
 	                   t->box, t->keys, t->vals, t->pk);
